@@ -1,0 +1,371 @@
+// Stable LSD radix sort of 64-bit keys (optional 32-bit payload), onesweep form.
+//
+// Replaces the serial PBBS sort of src/util/radixSort.hpp:49-116 (iSort with
+// 8-bit digits, MAX_RADIX = 8).  Structure (Adinets & Merrill, "Onesweep",
+// 2022):
+//   1. one pass over the keys builds the digit histograms of every pass at once
+//      (per-wave LDS histograms, then global atomics);
+//   2. a tiny kernel turns each pass histogram into exclusive digit bases;
+//   3. per digit pass, one kernel: a workgroup takes the next tile from an atomic
+//      counter (so every earlier tile is already running), ranks its 4096 keys
+//      stably with wavefront ballots (match-by-digit over 8 ballots), publishes
+//      its per-digit counts, resolves the tile's global digit offsets by
+//      decoupled look-back over the predecessors' published counts, reorders the
+//      tile in LDS and writes digit runs out coalesced.
+// Stability: keys are laid out wave-striped (wave w, item i, lane l holds key
+// tile + w*1024 + i*64 + l) and ranked item by item in lane order, waves in
+// order, so equal digits keep their input order -- the same order the
+// reference's radixStep (counting sort, :50-66) produces.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace gchi {
+
+constexpr int kRsThreads = 256;
+constexpr int kRsWaves = kRsThreads / kWave;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;  // 4096 keys
+constexpr int kRsRadix = 256;
+constexpr int kRsMaxPasses = 8;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1;
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+struct PassDesc {
+    int shift[kRsMaxPasses];
+    int bits[kRsMaxPasses];
+};
+
+__global__ __launch_bounds__(kRsThreads) void rs_histogram(const uint64_t *keys, uint64_t n, int npasses,
+                                                           PassDesc pd, uint32_t *ghist) {
+    __shared__ uint32_t s_h[kRsWaves][kRsMaxPasses][kRsRadix];  // 32 KB
+    const int wave = threadIdx.x / kWave;
+    for (int i = threadIdx.x; i < kRsWaves * kRsMaxPasses * kRsRadix; i += kRsThreads)
+        (&s_h[0][0][0])[i] = 0;
+    __syncthreads();
+    uint64_t stride = (uint64_t)gridDim.x * kRsThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kRsThreads + threadIdx.x; i < n; i += stride) {
+        uint64_t k = keys[i];
+        for (int p = 0; p < npasses; p++) {
+            uint32_t d = (uint32_t)(k >> pd.shift[p]) & ((1u << pd.bits[p]) - 1u);
+            atomicAdd(&s_h[wave][p][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npasses * kRsRadix; i += kRsThreads) {
+        int p = i / kRsRadix, d = i % kRsRadix;
+        uint32_t c = 0;
+        for (int w = 0; w < kRsWaves; w++) c += s_h[w][p][d];
+        if (c) atomicAdd(&ghist[i], c);
+    }
+}
+
+// ghist[p][d] -> exclusive digit bases, in place (one workgroup per pass).
+__global__ void rs_hist_scan(uint32_t *ghist) {
+    __shared__ uint32_t s[kRsRadix];
+    uint32_t *h = ghist + blockIdx.x * kRsRadix;
+    s[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kRsRadix; d++) {
+            uint32_t c = s[d];
+            s[d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    h[threadIdx.x] = s[threadIdx.x];
+}
+
+template <bool kVals>
+__global__ __launch_bounds__(kRsThreads) void rs_onesweep_pass(const uint64_t *__restrict__ kin,
+                                                               const uint32_t *__restrict__ vin,
+                                                               uint64_t *__restrict__ kout,
+                                                               uint32_t *__restrict__ vout, uint64_t n, int shift,
+                                                               int nbits, const uint32_t *__restrict__ gbase,
+                                                               uint32_t *status, uint32_t *tile_counter,
+                                                               uint32_t *err) {
+    __shared__ uint32_t s_wh[kRsWaves][kRsRadix];
+    __shared__ uint32_t s_toff[kRsRadix];
+    __shared__ uint32_t s_gb[kRsRadix];
+    __shared__ uint64_t s_keys[kRsTile];
+    __shared__ uint32_t s_vals[kVals ? kRsTile : 1];
+    __shared__ uint32_t s_scan[kRsWaves];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+#pragma unroll
+    for (int w = 0; w < kRsWaves; w++) s_wh[w][tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tbase = (uint64_t)tile * kRsTile;
+    const uint32_t tile_n = (uint32_t)min<uint64_t>(kRsTile, n - tbase);
+    const uint32_t dmask = (1u << nbits) - 1u;
+
+    uint64_t k[kRsItems];
+    uint32_t v[kRsItems];
+    uint32_t rank[kRsItems];
+    const uint64_t wbase = tbase + (uint64_t)wave * (kRsItems * kWave);
+#pragma unroll
+    for (int i = 0; i < kRsItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        k[i] = idx < n ? kin[idx] : 0ull;
+        if (kVals) v[i] = idx < n ? vin[idx] : 0u;
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int i = 0; i < kRsItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < nbits; b++) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & lt);
+        uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t b0 = 0;
+        if (valid) b0 = s_wh[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == cnt - 1) s_wh[wave][d] = b0 + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[i] = b0 + below;
+    }
+    __syncthreads();
+
+    // per-digit: wave exclusive offsets and tile count (thread tid owns digit tid)
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kRsWaves; w++) {
+        uint32_t c = s_wh[w][tid];
+        s_wh[w][tid] = run;
+        run += c;
+    }
+    const uint32_t tcount = run;
+    uint32_t *my_status = status + (size_t)tile * kRsRadix + tid;
+    if (tile == 0)
+        st_relaxed_agent(my_status, kFlagInc | tcount);
+    else
+        st_relaxed_agent(my_status, kFlagAgg | tcount);
+
+    // tile-local exclusive scan over the 256 digits
+    uint32_t incl = tcount;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    if (lane == kWave - 1) s_scan[wave] = incl;
+    __syncthreads();
+    uint32_t wb = 0;
+    for (int w = 0; w < wave; w++) wb += s_scan[w];
+    const uint32_t toff = wb + incl - tcount;
+    s_toff[tid] = toff;
+
+    // decoupled look-back for digit tid
+    uint32_t prefix = 0;
+    if (tile > 0) {
+        int64_t p = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        while (p >= 0) {
+            uint32_t sv = ld_relaxed_agent(status + (size_t)p * kRsRadix + tid);
+            uint32_t flag = sv & ~kValMask;
+            if (flag == 0) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            prefix += sv & kValMask;
+            if (flag == kFlagInc) break;
+            p--;
+        }
+        st_relaxed_agent(my_status, kFlagInc | ((prefix + tcount) & kValMask));
+    }
+    s_gb[tid] = gbase[tid] + prefix - toff;
+    __syncthreads();
+
+    // reorder the tile in LDS by digit
+#pragma unroll
+    for (int i = 0; i < kRsItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
+            uint32_t pos = s_toff[d] + s_wh[wave][d] + rank[i];
+            s_keys[pos] = k[i];
+            if (kVals) s_vals[pos] = v[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRsItems; j++) {
+        uint32_t p = (uint32_t)j * kRsThreads + tid;
+        if (p < tile_n) {
+            uint64_t key = s_keys[p];
+            uint32_t d = (uint32_t)(key >> shift) & dmask;
+            uint32_t g = s_gb[d] + p;
+            kout[g] = key;
+            if (kVals) vout[g] = s_vals[p];
+        }
+    }
+}
+
+__global__ void copy_u64(const uint64_t *a, uint64_t *b, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+__global__ void copy_u32(const uint32_t *a, uint32_t *b, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+static int npasses_for(int key_bits) { return (key_bits + 7) / 8; }
+
+struct SortWork {
+    uint64_t *ktmp;
+    uint32_t *vtmp;
+    uint32_t *ghist;
+    uint32_t *status;
+    uint32_t *counters;  // [passes] tile counters + [1] error word
+};
+
+template <typename C>
+static void sort_layout(C &c, uint64_t n, int key_bits, bool vals, SortWork *w) {
+    int np = npasses_for(key_bits);
+    uint64_t ntiles = div_up(n, kRsTile);
+    auto *kt = c.template take<uint64_t>(n);
+    auto *vt = c.template take<uint32_t>(vals ? n : 1);
+    auto *gh = c.template take<uint32_t>((size_t)np * kRsRadix);
+    auto *st = c.template take<uint32_t>((size_t)np * ntiles * kRsRadix);
+    auto *ct = c.template take<uint32_t>(np + 1);
+    if (w) *w = SortWork{kt, vt, gh, st, ct};
+}
+
+size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
+    SizerAdapter a;
+    sort_layout(a, n, key_bits, vals, nullptr);
+    return a.z.off;
+}
+
+// Sorts keys (and vals) in place on their low key_bits bits.
+int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, void *work, size_t work_bytes,
+                   hipStream_t s) {
+    if (n <= 1 || key_bits <= 0) return GCHI_OK;
+    GCHI_CHECK(keys, GCHI_ERR_INVALID, "sort: null keys");
+    GCHI_CHECK(key_bits <= 64, GCHI_ERR_INVALID, "sort: key_bits > 64");
+    GCHI_CHECK(n < (1ull << 30), GCHI_ERR_INVALID, "sort: n must be < 2^30");
+    const bool hv = vals != nullptr;
+    GCHI_CHECK(work_bytes >= sort_workspace_bytes(n, key_bits, hv), GCHI_ERR_WORKSPACE,
+               "sort: workspace too small");
+    Carver c(work, work_bytes);
+    SortWork w;
+    sort_layout(c, n, key_bits, hv, &w);
+    const int np = npasses_for(key_bits);
+    const uint64_t ntiles = div_up(n, kRsTile);
+    PassDesc pd;
+    for (int p = 0; p < kRsMaxPasses; p++) {
+        pd.shift[p] = 8 * p;
+        pd.bits[p] = p < np ? std::min(8, key_bits - 8 * p) : 0;
+    }
+    launch_fill_u32(w.ghist, 0u, (size_t)np * kRsRadix, s);
+    launch_fill_u32(w.status, 0u, (size_t)np * ntiles * kRsRadix, s);
+    launch_fill_u32(w.counters, 0u, np + 1, s);
+    unsigned hblocks = (unsigned)std::min<uint64_t>(ntiles, 1024);
+    hipLaunchKernelGGL(rs_histogram, dim3(hblocks), dim3(kRsThreads), 0, s, keys, n, np, pd, w.ghist);
+    hipLaunchKernelGGL(rs_hist_scan, dim3(np), dim3(kRsRadix), 0, s, w.ghist);
+    GCHI_LAUNCH_CHECK();
+    uint64_t *ka = keys, *kb = w.ktmp;
+    uint32_t *va = vals, *vb = w.vtmp;
+    for (int p = 0; p < np; p++) {
+        uint32_t *st = w.status + (size_t)p * ntiles * kRsRadix;
+        if (hv)
+            hipLaunchKernelGGL(rs_onesweep_pass<true>, dim3((unsigned)ntiles), dim3(kRsThreads), 0, s, ka, va, kb,
+                               vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p,
+                               w.counters + np);
+        else
+            hipLaunchKernelGGL(rs_onesweep_pass<false>, dim3((unsigned)ntiles), dim3(kRsThreads), 0, s, ka,
+                               (const uint32_t *)nullptr, kb, (uint32_t *)nullptr, n, pd.shift[p], pd.bits[p],
+                               w.ghist + p * kRsRadix, st, w.counters + p, w.counters + np);
+        GCHI_LAUNCH_CHECK();
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        unsigned b = (unsigned)std::min<uint64_t>(div_up(n, 256), 65536);
+        hipLaunchKernelGGL(copy_u64, dim3(b), dim3(256), 0, s, ka, keys, n);
+        if (hv) hipLaunchKernelGGL(copy_u32, dim3(b), dim3(256), 0, s, va, vals, n);
+        GCHI_LAUNCH_CHECK();
+    }
+    return GCHI_OK;
+}
+
+// ---- (src,dst) pair sort by (dst,src) ----
+__global__ void pack_dst_src(const uint32_t *src, const uint32_t *dst, uint64_t n, int id_bits, uint64_t *keys) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        keys[i] = ((uint64_t)dst[i] << id_bits) | (uint64_t)src[i];
+}
+__global__ void unpack_dst_src(const uint64_t *keys, uint64_t n, int id_bits, uint32_t *src, uint32_t *dst) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t m = (1ull << id_bits) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint64_t k = keys[i];
+        src[i] = (uint32_t)(k & m);
+        dst[i] = (uint32_t)(k >> id_bits);
+    }
+}
+
+}  // namespace gchi
+
+using namespace gchi;
+
+extern "C" int gchi_sort_workspace(uint64_t n, int key_bits, int with_values, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    *bytes = sort_workspace_bytes(n, key_bits, with_values != 0);
+    return GCHI_OK;
+}
+
+extern "C" int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits, void *d_work,
+                             size_t work_bytes, void *stream) {
+    return radix_sort_u64(d_keys, d_vals, n, key_bits, d_work, work_bytes, as_stream(stream));
+}
+
+extern "C" int gchi_sort_pairs_dst_src_workspace(uint64_t n, int id_bits, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    Sizer z;
+    z.take<uint64_t>(n);
+    z.take<char>(sort_workspace_bytes(n, 2 * id_bits, false));
+    *bytes = z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_dst, uint64_t n, int id_bits,
+                                       uint32_t *d_src_out, uint32_t *d_dst_out, void *d_work, size_t work_bytes,
+                                       void *stream) {
+    GCHI_CHECK(id_bits >= 1 && id_bits <= 32, GCHI_ERR_INVALID, "sort_pairs: id_bits must be in [1,32]");
+    if (n == 0) return GCHI_OK;
+    GCHI_CHECK(d_src && d_dst && d_src_out && d_dst_out, GCHI_ERR_INVALID, "sort_pairs: null pointer");
+    size_t need = 0;
+    gchi_sort_pairs_dst_src_workspace(n, id_bits, &need);
+    GCHI_CHECK(work_bytes >= need, GCHI_ERR_WORKSPACE, "sort_pairs: workspace too small");
+    hipStream_t s = as_stream(stream);
+    Carver c(d_work, work_bytes);
+    uint64_t *keys = c.take<uint64_t>(n);
+    size_t sw_bytes = sort_workspace_bytes(n, 2 * id_bits, false);
+    void *sw = c.take<char>(sw_bytes);
+    unsigned b = (unsigned)std::min<uint64_t>(div_up(n, 256), 65536);
+    hipLaunchKernelGGL(pack_dst_src, dim3(b), dim3(256), 0, s, d_src, d_dst, n, id_bits, keys);
+    GCHI_LAUNCH_CHECK();
+    GCHI_TRY(radix_sort_u64(keys, nullptr, n, 2 * id_bits, sw, sw_bytes, s));
+    hipLaunchKernelGGL(unpack_dst_src, dim3(b), dim3(256), 0, s, keys, n, id_bits, d_src_out, d_dst_out);
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}

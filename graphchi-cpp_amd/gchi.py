@@ -1,0 +1,296 @@
+"""ctypes binding of libgchi.so for the tests and bench.py.
+
+The library's C ABI (include/gchi.h) takes plain device pointers and a HIP
+stream; torch is used here only to own device memory and to provide the
+stream (torch.cuda.current_stream()).  Every call goes to the HIP library:
+there is no CPU fallback, and a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgchi.so")
+
+RMAT_ABCD = (0.57, 0.19, 0.19, 0.05)
+
+
+class GchiError(RuntimeError):
+    pass
+
+
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_i32 = ctypes.c_int
+c_vp = ctypes.c_void_p
+c_sz = ctypes.c_size_t
+
+
+class DGraph(ctypes.Structure):
+    _fields_ = [("nvertices", c_u32), ("reserved", c_u32), ("nedges", c_u64),
+                ("in_off", c_vp), ("in_src", c_vp), ("outdeg", c_vp), ("indeg", c_vp)]
+
+
+class PrOp(ctypes.Structure):
+    _fields_ = [("teleport", ctypes.c_double), ("damping", ctypes.c_double)]
+
+
+class PrTune(ctypes.Structure):
+    _fields_ = [("edges_per_block", c_i32), ("short_row_max", c_i32),
+                ("reserved0", c_i32), ("reserved1", c_i32)]
+
+
+class PrPlanInfo(ctypes.Structure):
+    _fields_ = [("nvertices", c_u32), ("edges_per_block", c_u32), ("nedges", c_u64),
+                ("short_row_max", c_u32), ("regular_blocks", c_u32), ("hub_rows", c_u32),
+                ("hub_chunks", c_u32), ("regular_edges", c_u64), ("hub_edges", c_u64)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "gchi_version": (c_i32, []),
+    "gchi_strerror": (ctypes.c_char_p, [c_i32]),
+    "gchi_last_error": (c_i32, [ctypes.c_char_p, c_sz]),
+    "gchi_device_open": (c_i32, [c_i32]),
+    "gchi_device_info": (c_i32, [ctypes.POINTER(c_i32), ctypes.c_char_p, c_sz]),
+    "gchi_rmat_generate": (c_i32, [c_vp, c_vp, c_u64, c_u64, c_i32, c_u64, ctypes.POINTER(ctypes.c_double), c_vp]),
+    "gchi_rmat_generate_host": (c_i32, [c_vp, c_vp, c_u64, c_u64, c_i32, c_u64, ctypes.POINTER(ctypes.c_double), c_i32]),
+    "gchi_drop_self_loops_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_drop_self_loops": (c_i32, [c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_sort_workspace": (c_i32, [c_u64, c_i32, c_i32, ctypes.POINTER(c_sz)]),
+    "gchi_sort_u64": (c_i32, [c_vp, c_vp, c_u64, c_i32, c_vp, c_sz, c_vp]),
+    "gchi_sort_pairs_dst_src_workspace": (c_i32, [c_u64, c_i32, ctypes.POINTER(c_sz)]),
+    "gchi_sort_pairs_dst_src": (c_i32, [c_vp, c_vp, c_u64, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_csc_build_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_csc_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_pr_plan_create": (c_i32, [ctypes.POINTER(DGraph), ctypes.POINTER(PrTune), ctypes.POINTER(c_vp), c_vp]),
+    "gchi_pr_plan_destroy": (c_i32, [c_vp]),
+    "gchi_pr_plan_info_get": (c_i32, [c_vp, ctypes.POINTER(PrPlanInfo)]),
+    "gchi_pr_init": (c_i32, [c_vp, ctypes.c_float, c_vp, c_vp, c_vp]),
+    "gchi_pr_sweep": (c_i32, [c_vp, ctypes.POINTER(PrOp), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gchi_pr_exec_create": (c_i32, [c_vp, ctypes.POINTER(PrOp), ctypes.c_float, c_i32, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp, ctypes.POINTER(c_vp)]),
+    "gchi_pr_exec_launch": (c_i32, [c_vp, c_vp]),
+    "gchi_pr_exec_destroy": (c_i32, [c_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libgchi.so (raises GchiError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GchiError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def check(status, what=""):
+    if status != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib().gchi_last_error(buf, 1024)
+        msg = lib().gchi_strerror(status).decode()
+        raise GchiError(f"{what}: {msg} ({status}): {buf.value.decode()}")
+
+
+# ---------------------------------------------------------------- torch glue
+def _torch():
+    import torch
+    return torch
+
+
+def dptr(t):
+    return c_vp(t.data_ptr()) if t is not None else c_vp(0)
+
+
+def stream():
+    torch = _torch()
+    return c_vp(torch.cuda.current_stream().cuda_stream)
+
+
+def empty_u32(n):
+    torch = _torch()
+    return torch.empty(max(int(n), 1), dtype=torch.int32, device="cuda")
+
+
+def workspace(nbytes):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+
+
+def to_host_u32(t, n=None):
+    a = t.cpu().numpy().view(np.uint32)
+    return a if n is None else a[:n]
+
+
+def to_dev_u32(a):
+    torch = _torch()
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return torch.from_numpy(a.view(np.int32)).to("cuda")
+
+
+def open_device(dev=0):
+    check(lib().gchi_device_open(dev), "gchi_device_open")
+
+
+def _abcd(abcd):
+    return (ctypes.c_double * 4)(*abcd)
+
+
+# ---------------------------------------------------------------- R-MAT input
+def rmat_host(scale, nedges, seed, abcd=RMAT_ABCD, first=0, nthreads=0):
+    """Raw R-MAT pairs on the host (self-loops kept), as numpy uint32 arrays."""
+    src = np.empty(nedges, dtype=np.uint32)
+    dst = np.empty(nedges, dtype=np.uint32)
+    check(lib().gchi_rmat_generate_host(src.ctypes.data, dst.ctypes.data, first, nedges, scale, seed,
+                                        _abcd(abcd), nthreads), "rmat_host")
+    return src, dst
+
+
+def drop_self_loops_host(src, dst):
+    keep = src != dst
+    return src[keep], dst[keep]
+
+
+def rmat_device(scale, nedges, seed, abcd=RMAT_ABCD, drop_self_loops=True):
+    """R-MAT pairs generated in HBM; returns (src, dst, count, max_id)."""
+    torch = _torch()
+    L = lib()
+    s0, d0 = empty_u32(nedges), empty_u32(nedges)
+    check(L.gchi_rmat_generate(dptr(s0), dptr(d0), 0, nedges, scale, seed, _abcd(abcd), stream()), "rmat")
+    if not drop_self_loops:
+        return s0, d0, nedges, None
+    s1, d1 = empty_u32(nedges), empty_u32(nedges)
+    nb = c_sz()
+    check(L.gchi_drop_self_loops_workspace(nedges, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    mx = torch.zeros(1, dtype=torch.int32, device="cuda")
+    check(L.gchi_drop_self_loops(dptr(s0), dptr(d0), nedges, dptr(s1), dptr(d1), dptr(cnt), dptr(mx),
+                                 dptr(ws), nb.value, stream()), "drop_self_loops")
+    del s0, d0, ws
+    n = int(cnt.item())
+    maxid = int(mx.cpu().numpy().view(np.uint32)[0])
+    return s1[:n], d1[:n], n, maxid
+
+
+# ---------------------------------------------------------------- sort
+def sort_u64(keys, vals=None, key_bits=64):
+    L = lib()
+    n = keys.numel()
+    nb = c_sz()
+    check(L.gchi_sort_workspace(n, key_bits, 1 if vals is not None else 0, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    check(L.gchi_sort_u64(dptr(keys), dptr(vals), n, key_bits, dptr(ws), nb.value, stream()), "sort_u64")
+
+
+def sort_pairs_dst_src(src, dst, id_bits):
+    L = lib()
+    n = src.numel()
+    so, do = empty_u32(n), empty_u32(n)
+    nb = c_sz()
+    check(L.gchi_sort_pairs_dst_src_workspace(n, id_bits, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    check(L.gchi_sort_pairs_dst_src(dptr(src), dptr(dst), n, id_bits, dptr(so), dptr(do), dptr(ws), nb.value,
+                                    stream()), "sort_pairs_dst_src")
+    return so, do
+
+
+# ---------------------------------------------------------------- graph
+class DeviceGraph:
+    """CSC of a graph resident in HBM (owned torch tensors + gchi_dgraph view)."""
+
+    def __init__(self, src, dst, nvertices):
+        L = lib()
+        E = int(src.numel()) if src is not None else 0
+        N = int(nvertices)
+        self.N, self.E = N, E
+        self.in_off = empty_u32(N + 1)
+        self.in_src = empty_u32(E)
+        self.outdeg = empty_u32(N)
+        self.indeg = empty_u32(N)
+        nb = c_sz()
+        check(L.gchi_csc_build_workspace(E, N, ctypes.byref(nb)))
+        ws = workspace(nb.value)
+        check(L.gchi_csc_build(dptr(src), dptr(dst), E, N, dptr(self.in_off), dptr(self.in_src), dptr(self.outdeg),
+                               dptr(self.indeg), dptr(ws), nb.value, stream()), "csc_build")
+        self.dg = DGraph(N, 0, E, self.in_off.data_ptr(), self.in_src.data_ptr(), self.outdeg.data_ptr(),
+                         self.indeg.data_ptr())
+
+
+class PageRankPlan:
+    def __init__(self, graph, edges_per_block=0, short_row_max=0):
+        self.graph = graph
+        self.handle = c_vp()
+        tune = PrTune(edges_per_block, short_row_max, 0, 0)
+        check(lib().gchi_pr_plan_create(ctypes.byref(graph.dg), ctypes.byref(tune), ctypes.byref(self.handle),
+                                        stream()), "pr_plan_create")
+
+    def info(self):
+        inf = PrPlanInfo()
+        check(lib().gchi_pr_plan_info_get(self.handle, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in PrPlanInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().gchi_pr_plan_destroy(self.handle)
+            self.handle = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, init, x, r):
+        check(lib().gchi_pr_init(self.handle, init, dptr(x), dptr(r), stream()), "pr_init")
+
+    def sweep(self, op, x_in, r_in, x_out, r_out, stats):
+        check(lib().gchi_pr_sweep(self.handle, ctypes.byref(op), dptr(x_in), dptr(r_in), dptr(x_out), dptr(r_out),
+                                  dptr(stats), stream()), "pr_sweep")
+
+
+class PageRankExec:
+    """init + nsweeps sweeps captured once into a HIP graph."""
+
+    def __init__(self, plan, op, init, nsweeps):
+        torch = _torch()
+        N = plan.graph.N
+        self.plan, self.nsweeps = plan, nsweeps
+        self.x = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.r = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.stats = torch.zeros(max(2 * nsweeps, 2), dtype=torch.float64, device="cuda")
+        self.op = op
+        self.handle = c_vp()
+        check(lib().gchi_pr_exec_create(plan.handle, ctypes.byref(op), init, nsweeps, dptr(self.x[0]),
+                                        dptr(self.x[1]), dptr(self.r[0]), dptr(self.r[1]), dptr(self.stats),
+                                        ctypes.byref(self.handle)), "pr_exec_create")
+
+    def launch(self):
+        check(lib().gchi_pr_exec_launch(self.handle, stream()), "pr_exec_launch")
+
+    def ranks(self):
+        return self.r[self.nsweeps & 1]
+
+    def close(self):
+        if self.handle:
+            lib().gchi_pr_exec_destroy(self.handle)
+            self.handle = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,181 @@
+/*
+ * gchi.h -- C ABI of the MI355X (gfx950) GraphChi hot-path library (libgchi.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (GCHI_OK == 0, negative gchi_status on error) and never throws or
+ * aborts across the boundary.  Device pointers ("d_" prefix) must point to HBM
+ * on the current HIP device; host pointers ("h_" prefix) to host memory.
+ * Streams are hipStream_t passed as void* (NULL == default stream).
+ *
+ * Workspace convention: entry points that need scratch device memory take
+ * (d_work, work_bytes); the matching *_workspace() call reports the size.  The
+ * library never allocates device memory inside a compute call, so every
+ * compute call can be captured into a HIP graph.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the GraphChi source tree).
+ */
+#ifndef GCHI_H
+#define GCHI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum gchi_status {
+    GCHI_OK = 0,
+    GCHI_ERR_INVALID = -1,      /* bad argument (NULL, size, range)            */
+    GCHI_ERR_HIP = -2,          /* HIP runtime error (message in last_error)   */
+    GCHI_ERR_WORKSPACE = -3,    /* workspace too small                          */
+    GCHI_ERR_UNSUPPORTED = -4,  /* shape/operator not supported on the device   */
+    GCHI_ERR_FORMAT = -5,       /* malformed shard / file content               */
+    GCHI_ERR_OVERFLOW = -6,     /* result does not fit the output type          */
+    GCHI_ERR_NO_DEVICE = -7,    /* no HIP device / library not initialised      */
+    GCHI_ERR_IO = -8,           /* file system error                            */
+    GCHI_ERR_TIMEOUT = -9       /* bounded device wait exceeded its spin limit  */
+} gchi_status;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gchi_version(void);
+/* Short text for a status code. */
+const char *gchi_strerror(int status);
+/* Copies the last error message of the calling thread into buf. */
+int gchi_last_error(char *buf, size_t len);
+
+/* Selects the HIP device used by the calling thread and warms the runtime.
+ * Replaces the engine/iomgr construction of src/engine/graphchi_engine.hpp:148-200. */
+int gchi_device_open(int device);
+/* Number of compute units and GPU arch name of the current device. */
+int gchi_device_info(int *num_cus, char *arch, size_t arch_len);
+
+/* ----------------------------------------------------- synthetic R-MAT input */
+/* Deterministic R-MAT edge generator (counter-based RNG: edge i of a graph is a
+ * pure function of (seed, i)); self-loops are kept here and removed by
+ * gchi_drop_self_loops, which restates sharder::preprocessing_add_edge
+ * (src/preprocessing/sharder.hpp:427-446).  abcd = quadrant probabilities. */
+int gchi_rmat_generate(uint32_t *d_src, uint32_t *d_dst, uint64_t first_edge, uint64_t nedges,
+                       int scale, uint64_t seed, const double abcd[4], void *stream);
+int gchi_rmat_generate_host(uint32_t *h_src, uint32_t *h_dst, uint64_t first_edge, uint64_t nedges,
+                            int scale, uint64_t seed, const double abcd[4], int nthreads);
+
+/* Stable compaction dropping src==dst pairs (sharder.hpp:428-431).  Writes the
+ * kept count to *d_count (uint64) and the maximum vertex id over kept edges to
+ * *d_maxid (uint32; 0 when empty) -- sharder.hpp:445. */
+int gchi_drop_self_loops_workspace(uint64_t nedges, size_t *bytes);
+int gchi_drop_self_loops(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges,
+                         uint32_t *d_src_out, uint32_t *d_dst_out, uint64_t *d_count,
+                         uint32_t *d_maxid, void *d_work, size_t work_bytes, void *stream);
+
+/* ---------------------------------------------------------------- radix sort */
+/* Stable LSD onesweep radix sort of 64-bit keys on their low key_bits bits
+ * (8-bit digits; optional 32-bit payload).  Replaces iSort/radixStep/eBits of
+ * src/util/radixSort.hpp:49-116.  d_keys/d_vals are sorted in place (the
+ * library ping-pongs through the workspace). */
+int gchi_sort_workspace(uint64_t n, int key_bits, int with_values, size_t *bytes);
+int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits,
+                  void *d_work, size_t work_bytes, void *stream);
+
+/* Sorts uint32 (src,dst) pairs by (dst,src) -- the order the sharder's shovel
+ * sort (sharder.hpp:192, dst key) followed by a full tie order produces.
+ * Pairs are read from d_src/d_dst and written sorted to d_src_out/d_dst_out.
+ * id_bits = bits needed for the largest vertex id. */
+int gchi_sort_pairs_dst_src_workspace(uint64_t n, int id_bits, size_t *bytes);
+int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_dst, uint64_t n, int id_bits,
+                            uint32_t *d_src_out, uint32_t *d_dst_out,
+                            void *d_work, size_t work_bytes, void *stream);
+
+/* ------------------------------------------------------------- device graph */
+/* Device-resident in-edge (CSC) view of a graph: in_off[N+1], in_src[E] sorted
+ * by (dst, src) -- the memory-shard adjacency of every interval at once
+ * (src/shards/memoryshard.hpp:384-519) -- plus out-degrees
+ * (src/engine/auxdata/degree_data.hpp:45-181).  Plain struct of device
+ * pointers owned by the caller. */
+typedef struct gchi_dgraph {
+    uint32_t nvertices;        /* N = 1 + max vertex id */
+    uint32_t reserved;
+    uint64_t nedges;           /* E (< 2^32) */
+    const uint32_t *in_off;    /* [N+1] */
+    const uint32_t *in_src;    /* [E]   */
+    const uint32_t *outdeg;    /* [N]   */
+    const uint32_t *indeg;     /* [N]   (optional, may be NULL) */
+} gchi_dgraph;
+
+/* Builds the CSC arrays from an edge list already resident in HBM. */
+int gchi_csc_build_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
+int gchi_csc_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                   uint32_t *d_in_off, uint32_t *d_in_src, uint32_t *d_outdeg, uint32_t *d_indeg,
+                   void *d_work, size_t work_bytes, void *stream);
+
+/* ------------------------------------------------------------------ PageRank */
+/* Affine functional kernel of example_apps/pagerank_functional.cpp:48-83:
+ *   value_to_neighbor : x_u = r_u / outdeg_u            (FP32, :78-81)
+ *   op_neighborval/plus: s_v = sum of x_u over in-edges  (FP32, :62-69)
+ *   compute_vertexvalue: r_v = (float)(teleport + damping*(double)s_v)  (:72-75)
+ * teleport = 0.15/N reproduces pagerank_functional, 0.15 the pagerank.cpp form. */
+typedef struct gchi_pr_op {
+    double teleport;
+    double damping;
+} gchi_pr_op;
+
+/* Launch shape of the pull sweep (filled by gchi_autotune_pr or defaults). */
+typedef struct gchi_pr_tune {
+    int edges_per_block;   /* W: multiple of 256 in [256,4096]; 0 = 2048.  Rows with
+                              in-degree >= W are hub rows, cut into 4096-edge chunks */
+    int short_row_max;     /* rows up to this length use one lane per row; 0 = 32 */
+    int reserved0;
+    int reserved1;
+} gchi_pr_tune;
+
+/* Sweep plan: the graph's in-edges regrouped into regular rows (LDS-staged
+ * blocks of < 2W edges) and hub rows (chunked), plus the sweep's scratch.
+ * Built once per graph (the one-time upload that replaces the per-iteration
+ * shard loads of src/engine/graphchi_engine.hpp:825-936).  A plan's scratch is
+ * shared by its sweeps: sweeps of one plan must not overlap in time. */
+typedef struct gchi_pr_plan gchi_pr_plan;
+typedef struct gchi_pr_plan_info {
+    uint32_t nvertices;
+    uint32_t edges_per_block;
+    uint64_t nedges;
+    uint32_t short_row_max;
+    uint32_t regular_blocks;
+    uint32_t hub_rows;
+    uint32_t hub_chunks;
+    uint64_t regular_edges;
+    uint64_t hub_edges;
+} gchi_pr_plan_info;
+
+int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream);
+int gchi_pr_plan_destroy(gchi_pr_plan *plan);
+int gchi_pr_plan_info_get(const gchi_pr_plan *plan, gchi_pr_plan_info *info);
+
+/* Iteration 0 of the synchronous functional engine (functional_bulksync.hpp:84-87,
+ * 107-113): r = init, x = r/outdeg (0 where outdeg == 0). */
+int gchi_pr_init(const gchi_pr_plan *plan, float init, float *d_x, float *d_r, void *stream);
+
+/* One synchronous gather->apply->scatter sweep (functional_bulksync.hpp:91-113,
+ * functional_engine.hpp:50-121): two kernels (sweep, finalize).
+ * d_stats[0] = sum |r_out - r_in| (FP64), d_stats[1] = sum r_out (FP64), both
+ * reduced in a fixed order (the log_change deltas of
+ * src/api/graphchi_context.hpp:78-102). */
+int gchi_pr_sweep(const gchi_pr_plan *plan, const gchi_pr_op *op, const float *d_x_in, const float *d_r_in,
+                  float *d_x_out, float *d_r_out, double *d_stats, void *stream);
+
+/* init + nsweeps sweeps captured into one HIP graph (instantiated once,
+ * replayed by gchi_pr_exec_launch).  Sweep k reads buffers [k&1] and writes
+ * [(k+1)&1]; the final ranks are in d_r{nsweeps&1}.  d_stats_all receives
+ * 2*nsweeps doubles.  Replaces the iteration loop of graphchi_engine::run
+ * (src/engine/graphchi_engine.hpp:779-956) for functional kernels. */
+typedef struct gchi_pr_exec gchi_pr_exec;
+int gchi_pr_exec_create(const gchi_pr_plan *plan, const gchi_pr_op *op, float init, int nsweeps, float *d_x0,
+                        float *d_x1, float *d_r0, float *d_r1, double *d_stats_all, gchi_pr_exec **out);
+int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
+int gchi_pr_exec_destroy(gchi_pr_exec *exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GCHI_H */

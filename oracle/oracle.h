@@ -1,0 +1,89 @@
+/*
+ * oracle.h -- CPU restatement of the GraphChi hot path, used ONLY as a checker
+ * by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
+ * Nothing in the product (libgchi.so, the host headers, the apps) links it.
+ *
+ * Each function cites the reference code it restates (paths relative to the
+ * GraphChi source tree).  "Pinned by" notes in oracle.c name the checks that
+ * pin each function.
+ */
+#ifndef GCHI_ORACLE_H
+#define GCHI_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CSC from an edge list, in-edges of each dst in ascending (src) order --
+ * the order a memory shard presents a vertex's in-edges
+ * (src/preprocessing/sharder.hpp:640 + src/shards/memoryshard.hpp:448-497). */
+int orc_csc_build(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, uint32_t *in_off,
+                  uint32_t *in_src, uint32_t *outdeg, uint32_t *indeg);
+
+/* Synchronous functional PageRank (functional_bulksync.hpp:84-113 with the
+ * kernel of example_apps/pagerank_functional.cpp:48-83): nsweeps gather/apply/
+ * scatter rounds after the iteration-0 initialisation.  FP32 variant follows
+ * the reference's arithmetic (FP32 sequential gather in ascending src order,
+ * FP64 apply rounded to float, FP32 divide); FP64 variant is the truth.
+ * stats (optional): 2*nsweeps doubles, (sum|dr|, sum r) per sweep. */
+int orc_pagerank_sync_f32(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                          double teleport, double damping, float init, int nsweeps, float *r_out, double *stats);
+int orc_pagerank_sync_f64(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                          double teleport, double damping, double init, int nsweeps, double *r_out);
+
+/* Personalized PageRank in FP64 (kernel: compute_vertexvalue =
+ * (v in S ? (1-damping)/|S| : 0) + damping*sum): power iteration from the
+ * teleport vector until sum|dr| <= tol or max_sweeps.  Returns sweeps done. */
+int orc_ppr_f64(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                const uint32_t *seeds, uint32_t nseeds, double damping, double tol, int max_sweeps, double *r_out);
+
+/* Connected components: labels = minimum vertex id of each weakly connected
+ * component, the fixed point of example_apps/connectedcomponents.cpp:79-121.
+ * Union-find restatement and a synchronous label-propagation restatement
+ * (returns the number of propagation sweeps until no label changes). */
+int orc_cc_unionfind(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, uint32_t *labels);
+int orc_cc_labelprop(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, uint32_t *labels);
+
+/* Stable sort of 64-bit keys (with optional 32-bit payload) by LSD counting
+ * passes of 8-bit digits over the low key_bits bits: iSort/radixStep of
+ * src/util/radixSort.hpp:49-116. */
+int orc_stable_sort_u64(uint64_t n, uint64_t *keys, uint32_t *vals, int key_bits);
+
+/* Sharder restatement (src/preprocessing/sharder.hpp).
+ * orc_auto_nshards: determine_number_of_shards "auto" rule (:555-587).
+ * orc_shard_intervals: replays add/createnextshard/done (:861-927) over the
+ *   dst-sorted edge stream; writes interval ends and per-shard edge counts,
+ *   returns the number of shards created (may be < nshards).
+ * orc_shard_encode: finish_shard's adjacency encoding (:701-822) of one shard
+ *   already sorted by (src,dst); returns bytes written to adj (NULL = size
+ *   only) and the .adjidx entries (vertexid, filepos, edgecounter) triples. */
+int orc_auto_nshards(uint64_t nedges, size_t edge_value_bytes, int membudget_mb);
+int orc_shard_intervals(uint64_t e, const uint32_t *dst_sorted, uint32_t max_id, int nshards, uint32_t *ends,
+                        uint64_t *counts);
+int64_t orc_shard_encode(uint64_t e, const uint32_t *src, const uint32_t *dst, uint8_t *adj, uint64_t *idx_triples,
+                         uint64_t *nidx);
+
+/* order_by_degree (src/preprocessing/util/orderbydegree.hpp:58-100): rank of
+ * each vertex by (in+out degree, id); vertexmap[old] = new id. */
+int orc_degree_order(uint32_t n, const uint32_t *deg, uint32_t *vertexmap);
+
+/* Triangle counting (example_apps/trianglecounting.cpp): degree-order the
+ * graph, count per-vertex incident triangles in the new id space (uint32, as
+ * the reference's .vout) and the total.  Input: undirected edge list. */
+int orc_triangles(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, uint32_t *vertexmap,
+                  uint32_t *per_vertex, uint64_t *total);
+
+/* CPU baseline: the reference's PSW sweep with all shards preloaded
+ * (functional_engine.hpp:50-121): per interval, gather over the memory shard's
+ * in-edges (src-sorted) reading each edge's value, apply, then scatter r/outdeg
+ * into every out-edge's value slot; OpenMP over edges/vertices.  Returns the
+ * wall seconds of the sweeps only. */
+double orc_pr_psw_omp(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int nshards,
+                      double teleport, double damping, float init, int nsweeps, int nthreads, float *r_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+"""Test-graph definitions shared by the golden-fixture script and the tests.
+
+Every graph is rebuilt deterministically: hand-written edge lists, or R-MAT
+pairs from the product's counter-based generator (gchi_rmat_generate_host),
+then the preprocessing the reference applies or the workload names:
+  directed    : self-loops dropped, duplicates kept (sharder.hpp:428-431)
+  undirected  : each generated pair stored in both directions (CC workloads)
+  simple      : self-loops and duplicate undirected pairs removed, one (min,max)
+                record per edge (triangle-counting workloads)
+N = 1 + the largest vertex id, as the sharder defines it (sharder.hpp:445, 925).
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "graphchi-cpp_amd"))
+import gchi  # noqa: E402
+
+ABCD = (0.57, 0.19, 0.19, 0.05)
+
+PR_CASES = [
+    {"name": "pr_chain", "hand": [(0, 1), (1, 2), (2, 3), (3, 4)], "sweeps": 5},
+    {"name": "pr_star", "hand": [(i, 0) for i in range(1, 10)] + [(0, 1)], "sweeps": 6},
+    {"name": "pr_misc", "hand": [(0, 1), (0, 1), (1, 0), (1, 2), (2, 2), (2, 5), (3, 1), (5, 2), (7, 5), (7, 1)],
+     "sweeps": 7},
+    {"name": "pr_rmat10", "rmat": (10, 16, 1), "sweeps": 5, "ppr_seeds": [3, 17, 200], "ppr_niters": 200},
+    {"name": "pr_rmat14_p3", "rmat": (14, 16, 1), "sweeps": 9, "nshards": 3},
+    {"name": "pr_rmat16", "rmat": (16, 16, 1), "sweeps": 9},
+]
+
+SHARD_CASES = [
+    {"name": "sh_rmat12_p1", "rmat": (12, 16, 5), "nshards": 1},
+    {"name": "sh_rmat12_p3", "rmat": (12, 16, 5), "nshards": 3},
+    {"name": "sh_rmat12_p7", "rmat": (12, 16, 5), "nshards": 7},
+    {"name": "sh_rmat17_p1", "rmat": (17, 16, 6), "nshards": 1},
+    {"name": "sh_rmat17_p2", "rmat": (17, 16, 6), "nshards": 2},
+    {"name": "sh_hand_gaps",
+     "hand": [(3, 700), (3, 1), (5, 2)] + [(5, 1000 + i) for i in range(300)] + [(600, 7), (1300, 5), (1300, 4),
+                                                                                   (2000, 3), (2000, 1300)],
+     "nshards": 2},
+]
+
+CC_CASES = [
+    {"name": "cc_hand", "hand": [(0, 1), (2, 3), (3, 4), (6, 5), (9, 8), (8, 7)], "undirected": True},
+    {"name": "cc_rmat12", "rmat": (12, 16, 3), "undirected": True},
+    {"name": "cc_rmat14_p3", "rmat": (14, 16, 3), "undirected": True, "nshards": 3},
+]
+
+TC_CASES = [
+    {"name": "tc_k5", "hand": [(a, b) for a in range(5) for b in range(a + 1, 5)] + [(4, 5), (5, 6)],
+     "simple": True},
+    {"name": "tc_rmat10", "rmat": (10, 16, 4), "simple": True},
+    {"name": "tc_rmat13", "rmat": (13, 16, 4), "simple": True},
+]
+
+
+def rmat_pairs(scale, edge_factor, seed, undirected=False):
+    """Raw R-MAT pairs of a workload: edge_factor * 2^scale directed records
+    (halved before mirroring for undirected workloads)."""
+    nraw = (edge_factor << scale) // (2 if undirected else 1)
+    return gchi.rmat_host(scale, nraw, seed, ABCD)
+
+
+def preprocess(src, dst, undirected=False, simple=False):
+    src = np.asarray(src, dtype=np.uint32)
+    dst = np.asarray(dst, dtype=np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    if simple:
+        a, b = np.minimum(src, dst), np.maximum(src, dst)
+        key = np.unique((a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64))
+        src = (key >> np.uint64(32)).astype(np.uint32)
+        dst = (key & np.uint64(0xffffffff)).astype(np.uint32)
+    elif undirected:
+        src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
+    return src, dst
+
+
+def make(case):
+    if "hand" in case:
+        e = np.array(case["hand"], dtype=np.uint32).reshape(-1, 2)
+        src, dst = e[:, 0], e[:, 1]
+    else:
+        scale, ef, seed = case["rmat"]
+        src, dst = rmat_pairs(scale, ef, seed, undirected=case.get("undirected", False))
+    src, dst = preprocess(src, dst, case.get("undirected", False), case.get("simple", False))
+    n = int(max(src.max(), dst.max())) + 1 if len(src) else 0
+    return src, dst, n
+
+
+def input_sha(src, dst):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(src, dtype=np.uint32).tobytes())
+    h.update(np.ascontiguousarray(dst, dtype=np.uint32).tobytes())
+    return h.hexdigest()
+
+
+def all_cases():
+    return PR_CASES + SHARD_CASES + CC_CASES + TC_CASES
