@@ -213,3 +213,12 @@ extern "C" int gchi_device_info(int *num_cus, char *arch, size_t arch_len) {
     }
     return GCHI_OK;
 }
+
+extern "C" int gchi_memcpy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (bytes == 0) return GCHI_OK;
+    GCHI_CHECK(dst && src, GCHI_ERR_INVALID, "memcpy: null pointer");
+    hipStream_t s = as_stream(stream);
+    GCHI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    return GCHI_OK;
+}

@@ -1,0 +1,316 @@
+// Triangle counting on the degree-ordered graph.
+//
+// Reference: example_apps/trianglecounting.cpp.  The input is relabelled by
+// order_by_degree (src/preprocessing/util/orderbydegree.hpp:58-100): vertices
+// ranked by (in+out degree, id), vertexmap[old] = rank; per-vertex triangle
+// counts are stored in the new id space as uint32 (<G>_degord.4B.vout,
+// trianglecounting.cpp:72) and the total is their sum / 3 (:476-477).
+//
+// Device form:
+//   gchi_tc_prepare : degrees, (degree,id) radix sort -> vertexmap, edges mapped
+//       and oriented from lower to higher rank, sorted and deduplicated into an
+//       oriented CSR (N+(v): the higher-ranked neighbours of v, ascending).
+//   gchi_tc_count   : every triangle a<b<w is found once, at vertex a, as
+//       w in N+(a) n N+(b) for b in N+(a).  One wavefront per vertex a (taken
+//       from an atomic work counter): N+(a) is staged in LDS (binary search in
+//       global memory when longer than 1024), the (b, w in N+(b)) pairs of 64
+//       consecutive b are flattened over the 64 lanes, and each lane
+//       binary-searches its w in N+(a).  Counts: a, b and w each get +1 per
+//       triangle (uint64 atomics), converted to uint32 with an overflow flag.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace gchi {
+
+constexpr int kTcWaves = 4;
+constexpr int kTcLds = 1024;
+
+__global__ void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, uint32_t *deg,
+                           uint32_t *err) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        uint32_t s = src[i], d = dst[i];
+        if (s >= n || d >= n) {
+            atomicOr(err, 1u);
+            continue;
+        }
+        if (s == d) continue;
+        atomicAdd(&deg[s], 1u);
+        atomicAdd(&deg[d], 1u);
+    }
+}
+
+__global__ void tc_degree_keys(const uint32_t *deg, uint32_t n, int idb, uint64_t *keys) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        keys[v] = ((uint64_t)deg[v] << idb) | v;
+}
+
+__global__ void tc_vertexmap(const uint64_t *keys, uint32_t n, int idb, uint32_t *vmap) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    const uint64_t m = (1ull << idb) - 1ull;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) vmap[keys[i] & m] = i;
+}
+
+__global__ void tc_orient_keys(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, const uint32_t *vmap, int idb,
+                               uint64_t sentinel, uint64_t *keys) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        uint32_t s = src[i], d = dst[i];
+        if (s == d || s >= n || d >= n) {  // out-of-range ids are reported by tc_degrees
+            keys[i] = sentinel;
+            continue;
+        }
+        uint32_t a = vmap[s], b = vmap[d];
+        uint32_t lo = min(a, b), hi = max(a, b);
+        keys[i] = ((uint64_t)lo << idb) | hi;
+    }
+}
+
+__global__ void tc_unique_flags(const uint64_t *keys, uint64_t e, uint64_t sentinel, uint32_t *flags) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        uint64_t k = keys[i];
+        flags[i] = (k != sentinel && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+    }
+}
+
+__global__ void tc_scatter_unique(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos, uint64_t e,
+                                  int idb, uint32_t *adj, uint32_t *lo_count) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t m = (1ull << idb) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        if (!flags[i]) continue;
+        uint64_t k = keys[i];
+        adj[pos[i]] = (uint32_t)(k & m);
+        atomicAdd(&lo_count[k >> idb], 1u);
+    }
+}
+
+GCHI_DEV bool bsearch_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint32_t v = a[mid];
+        if (v == x) return true;
+        if (v < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_t *__restrict__ off,
+                                                                   const uint32_t *__restrict__ adj, uint32_t n,
+                                                                   unsigned long long *cnt,
+                                                                   unsigned long long *total, uint32_t *work) {
+    __shared__ uint32_t sA[kTcWaves][kTcLds];
+    __shared__ uint32_t sPre[kTcWaves][kWave];
+    __shared__ uint32_t sBeg[kTcWaves][kWave];
+    __shared__ uint32_t sCnt[kTcWaves][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (;;) {
+        uint32_t a = 0;
+        if (lane == 0) a = atomicAdd(work, 1u);
+        a = __shfl(a, 0, kWave);
+        if (a >= n) break;
+        const uint32_t ab = off[a], da = off[a + 1] - ab;
+        if (da < 2) continue;
+        const uint32_t *A = adj + ab;
+        const bool lds = da <= (uint32_t)kTcLds;
+        if (lds)
+            for (uint32_t k = lane; k < da; k += kWave) sA[wave][k] = A[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t *As = lds ? sA[wave] : A;
+        unsigned long long acc_a = 0;
+        for (uint32_t bc = 0; bc + 1 < da; bc += kWave) {
+            const uint32_t k = bc + lane;
+            const bool valid = k < da;
+            uint32_t b = valid ? As[k] : 0u;
+            uint32_t bb = valid ? off[b] : 0u;
+            uint32_t lb = valid ? off[b + 1] - bb : 0u;
+            uint32_t pre = lb;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                uint32_t t = __shfl_up(pre, o, kWave);
+                if (lane >= o) pre += t;
+            }
+            const uint32_t tot = __shfl(pre, kWave - 1, kWave);
+            sPre[wave][lane] = pre;
+            sBeg[wave][lane] = bb;
+            sCnt[wave][lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t p = lane; p < tot; p += kWave) {
+                uint32_t lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (sPre[wave][mid] > p)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                const uint32_t j = lo;
+                const uint32_t excl = j ? sPre[wave][j - 1] : 0u;
+                const uint32_t w = adj[sBeg[wave][j] + (p - excl)];
+                // w > b_j > a; N+(a) is ascending, b_j sits at index bc + j
+                if (bsearch_u32(As, bc + j + 1, da, w)) {
+                    atomicAdd(&cnt[w], 1ull);
+                    atomicAdd(&sCnt[wave][j], 1u);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t c = valid ? sCnt[wave][lane] : 0u;
+            if (c) atomicAdd(&cnt[b], (unsigned long long)c);
+            acc_a += c;
+            __builtin_amdgcn_wave_barrier();
+        }
+        acc_a = wave_sum(acc_a);
+        if (lane == 0 && acc_a) {
+            atomicAdd(&cnt[a], acc_a);
+            atomicAdd(total, acc_a);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void tc_to_u32(const unsigned long long *cnt, uint32_t n, uint32_t *out, uint32_t *overflow) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        unsigned long long c = cnt[v];
+        if (c > 0xffffffffull) atomicOr(overflow, 1u);
+        out[v] = (uint32_t)c;
+    }
+}
+
+template <typename C>
+static void tc_prep_layout(C &c, uint64_t E, uint32_t N, uint32_t **deg, uint64_t **vkeys, uint64_t **ekeys,
+                           uint32_t **flags, uint32_t **pos, uint32_t **err, void **sw, size_t *swb) {
+    int idb = bits_for(N ? N - 1 : 0);
+    if (idb == 0) idb = 1;
+    *deg = c.template take<uint32_t>(N + 1);
+    *vkeys = c.template take<uint64_t>(N + 1);
+    *ekeys = c.template take<uint64_t>(E + 1);
+    *flags = c.template take<uint32_t>(E + 1);
+    *pos = c.template take<uint32_t>(E + 2);
+    *err = c.template take<uint32_t>(1);
+    size_t s = sort_workspace_bytes(std::max<uint64_t>(E, N), 64, false);
+    s = std::max(s, scan_workspace(std::max<uint64_t>(E, N) + 1));
+    *sw = c.template take<char>(s);
+    *swb = s;
+}
+
+}  // namespace gchi
+
+using namespace gchi;
+
+extern "C" int gchi_tc_prepare_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    SizerAdapter a;
+    uint32_t *d, *f, *p, *e;
+    uint64_t *vk, *ek;
+    void *sw;
+    size_t swb;
+    tc_prep_layout(a, nedges, nvertices, &d, &vk, &ek, &f, &p, &e, &sw, &swb);
+    *bytes = a.z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                               uint32_t *d_vertexmap, uint32_t *d_off, uint32_t *d_adj, uint64_t *h_noriented,
+                               void *d_work, size_t work_bytes, void *stream) {
+    GCHI_CHECK(d_vertexmap && d_off && d_adj && h_noriented && nvertices > 0, GCHI_ERR_INVALID,
+               "tc_prepare: null argument");
+    GCHI_CHECK(nedges == 0 || (d_src && d_dst), GCHI_ERR_INVALID, "tc_prepare: null edges");
+    GCHI_CHECK(nedges < (1ull << 31), GCHI_ERR_UNSUPPORTED, "tc_prepare: E must be < 2^31");
+    hipStream_t s = as_stream(stream);
+    Carver c(d_work, work_bytes);
+    uint32_t *deg, *flags, *pos, *err;
+    uint64_t *vkeys, *ekeys;
+    void *sw;
+    size_t swb;
+    tc_prep_layout(c, nedges, nvertices, &deg, &vkeys, &ekeys, &flags, &pos, &err, &sw, &swb);
+    GCHI_CHECK(c.ok(), GCHI_ERR_WORKSPACE, "tc_prepare: workspace too small");
+    const uint32_t N = nvertices;
+    const uint64_t E = nedges;
+    int idb = bits_for(N - 1);
+    if (idb == 0) idb = 1;
+    const int degb = bits_for(2 * E);
+    GCHI_CHECK(idb + degb <= 64, GCHI_ERR_UNSUPPORTED, "tc_prepare: degree/id keys exceed 64 bits");
+    launch_fill_u32(deg, 0u, N, s);
+    launch_fill_u32(err, 0u, 1, s);
+    if (E)
+        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N, deg, err);
+    hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
+    GCHI_LAUNCH_CHECK();
+    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
+    hipLaunchKernelGGL(tc_vertexmap, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, vkeys, N, idb, d_vertexmap);
+    const uint64_t sentinel = (2 * idb >= 64) ? ~0ull : ((1ull << (2 * idb)) - 1ull);
+    uint32_t *lo_count = deg;  // reuse: degrees are no longer needed
+    launch_fill_u32(lo_count, 0u, N + 1, s);
+    if (E) {
+        hipLaunchKernelGGL(tc_orient_keys, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N,
+                           d_vertexmap, idb, sentinel, ekeys);
+        GCHI_LAUNCH_CHECK();
+        GCHI_TRY(radix_sort_u64(ekeys, nullptr, E, 2 * idb, sw, swb, s));
+        hipLaunchKernelGGL(tc_unique_flags, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, E, sentinel,
+                           flags);
+        GCHI_LAUNCH_CHECK();
+        GCHI_TRY(exclusive_scan_u32(flags, pos, E, sw, swb, s));
+        hipLaunchKernelGGL(tc_scatter_unique, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, flags, pos, E,
+                           idb, d_adj, lo_count);
+        GCHI_LAUNCH_CHECK();
+    }
+    GCHI_TRY(exclusive_scan_u32(lo_count, d_off, N, sw, swb, s));
+    uint32_t h_err = 0, h_m = 0;
+    GCHI_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipMemcpyAsync(&h_m, d_off + N, 4, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(h_err == 0, GCHI_ERR_INVALID, "tc_prepare: vertex id >= nvertices");
+    *h_noriented = h_m;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_tc_count_workspace(uint32_t nvertices, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    Sizer z;
+    z.take<unsigned long long>(nvertices + 1);
+    z.take<unsigned long long>(1);
+    z.take<uint32_t>(2);
+    *bytes = z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices,
+                             uint32_t *d_per_vertex, uint64_t *d_total, uint32_t *d_overflow, void *d_work,
+                             size_t work_bytes, void *stream) {
+    GCHI_CHECK(d_off && d_adj && d_per_vertex && d_total && d_overflow && nvertices > 0, GCHI_ERR_INVALID,
+               "tc_count: null argument");
+    size_t need = 0;
+    gchi_tc_count_workspace(nvertices, &need);
+    GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "tc_count: workspace too small");
+    hipStream_t s = as_stream(stream);
+    Carver c(d_work, work_bytes);
+    auto *cnt = c.take<unsigned long long>(nvertices + 1);
+    auto *tot = c.take<unsigned long long>(1);
+    auto *wk = c.take<uint32_t>(2);
+    launch_fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, nvertices, s);
+    launch_fill_u64(reinterpret_cast<uint64_t *>(tot), 0ull, 1, s);
+    launch_fill_u32(wk, 0u, 2, s);
+    launch_fill_u32(d_overflow, 0u, 1, s);
+    int ncu = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
+    }
+    hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices, cnt,
+                       tot, wk);
+    hipLaunchKernelGGL(tc_to_u32, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, cnt, nvertices,
+                       d_per_vertex, d_overflow);
+    GCHI_HIP(hipMemcpyAsync(d_total, tot, 8, hipMemcpyDeviceToDevice, s));
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}

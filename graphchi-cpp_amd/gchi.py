@@ -54,6 +54,7 @@ _SIGS = {
     "gchi_last_error": (c_i32, [ctypes.c_char_p, c_sz]),
     "gchi_device_open": (c_i32, [c_i32]),
     "gchi_device_info": (c_i32, [ctypes.POINTER(c_i32), ctypes.c_char_p, c_sz]),
+    "gchi_memcpy": (c_i32, [c_vp, c_vp, c_sz, c_vp]),
     "gchi_rmat_generate": (c_i32, [c_vp, c_vp, c_u64, c_u64, c_i32, c_u64, ctypes.POINTER(ctypes.c_double), c_vp]),
     "gchi_rmat_generate_host": (c_i32, [c_vp, c_vp, c_u64, c_u64, c_i32, c_u64, ctypes.POINTER(ctypes.c_double), c_i32]),
     "gchi_drop_self_loops_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
@@ -73,6 +74,24 @@ _SIGS = {
                                     c_vp, ctypes.POINTER(c_vp)]),
     "gchi_pr_exec_launch": (c_i32, [c_vp, c_vp]),
     "gchi_pr_exec_destroy": (c_i32, [c_vp]),
+    "gchi_cc_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_cc_minlabel": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_cc_labelprop_init": (c_i32, [c_vp, c_u32, c_vp]),
+    "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
+    "gchi_tc_prepare_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_tc_prepare": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, ctypes.POINTER(c_u64), c_vp, c_sz,
+                                c_vp]),
+    "gchi_tc_count_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_tc_count": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_shard_auto_count": (c_i32, [c_u64, c_u32, c_i32, ctypes.POINTER(c_i32)]),
+    "gchi_shard_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_i32, ctypes.POINTER(c_vp), c_vp]),
+    "gchi_shard_destroy": (c_i32, [c_vp]),
+    "gchi_shard_info": (c_i32, [c_vp, ctypes.POINTER(c_i32), c_vp, c_vp]),
+    "gchi_shard_edges": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_u64)]),
+    "gchi_shard_degrees": (c_i32, [c_vp, ctypes.POINTER(c_vp)]),
+    "gchi_shard_encode": (c_i32, [c_vp, c_i32, c_vp, ctypes.POINTER(c_u64), c_vp, c_u32, ctypes.POINTER(c_u32),
+                                  c_vp]),
+    "gchi_shard_write_files": (c_i32, [c_vp, ctypes.c_char_p, c_u32, c_vp]),
 }
 
 _lib = None
@@ -84,6 +103,9 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise GchiError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # torch bundles its own HIP runtime (same SONAME libamdhip64.so.7): load it
+        # first so that this process holds exactly one HIP runtime.
+        import torch  # noqa: F401
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)
@@ -287,6 +309,128 @@ class PageRankExec:
     def close(self):
         if self.handle:
             lib().gchi_pr_exec_destroy(self.handle)
+            self.handle = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- connected components
+def cc_minlabel(src, dst, n):
+    L = lib()
+    labels = empty_u32(n)
+    nb = c_sz()
+    check(L.gchi_cc_workspace(n, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    e = src.numel() if src is not None else 0
+    check(L.gchi_cc_minlabel(dptr(src), dptr(dst), e, n, dptr(labels), dptr(ws), nb.value, stream()), "cc_minlabel")
+    return labels
+
+
+def cc_labelprop(src, dst, n, max_sweeps=100000):
+    """Synchronous min-label propagation until no label changes; returns
+    (labels, sweeps including the final no-change sweep)."""
+    torch = _torch()
+    L = lib()
+    lab = [empty_u32(n), empty_u32(n)]
+    changed = torch.zeros(1, dtype=torch.int64, device="cuda")
+    check(L.gchi_cc_labelprop_init(dptr(lab[0]), n, stream()))
+    e = src.numel()
+    for k in range(max_sweeps):
+        check(L.gchi_cc_labelprop_sweep(dptr(src), dptr(dst), e, n, dptr(lab[k & 1]), dptr(lab[(k + 1) & 1]),
+                                        dptr(changed), stream()), "cc_labelprop_sweep")
+        if int(changed.item()) == 0:
+            return lab[(k + 1) & 1], k + 1
+    raise GchiError("cc_labelprop: no fixed point within max_sweeps")
+
+
+# ---------------------------------------------------------------- triangle counting
+class TrianglePrep:
+    """Degree order + oriented CSR (device)."""
+
+    def __init__(self, src, dst, n):
+        L = lib()
+        e = src.numel()
+        self.n = n
+        self.vertexmap = empty_u32(n)
+        self.off = empty_u32(n + 1)
+        self.adj = empty_u32(e)
+        nb = c_sz()
+        check(L.gchi_tc_prepare_workspace(e, n, ctypes.byref(nb)))
+        ws = workspace(nb.value)
+        m = c_u64()
+        check(L.gchi_tc_prepare(dptr(src), dptr(dst), e, n, dptr(self.vertexmap), dptr(self.off), dptr(self.adj),
+                                ctypes.byref(m), dptr(ws), nb.value, stream()), "tc_prepare")
+        self.noriented = m.value
+
+    def count(self):
+        torch = _torch()
+        L = lib()
+        pv = empty_u32(self.n)
+        tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+        ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
+        nb = c_sz()
+        check(L.gchi_tc_count_workspace(self.n, ctypes.byref(nb)))
+        ws = workspace(nb.value)
+        check(L.gchi_tc_count(dptr(self.off), dptr(self.adj), self.n, dptr(pv), dptr(tot), dptr(ovf), dptr(ws),
+                              nb.value, stream()), "tc_count")
+        return pv, tot, ovf
+
+
+# ---------------------------------------------------------------- sharder
+def shard_auto_count(nedges, edge_value_bytes, membudget_mb=800):
+    n = c_i32()
+    check(lib().gchi_shard_auto_count(nedges, edge_value_bytes, membudget_mb, ctypes.byref(n)))
+    return n.value
+
+
+class Shards:
+    def __init__(self, src, dst, max_id, nshards):
+        self.handle = c_vp()
+        self.nshards_req = nshards
+        check(lib().gchi_shard_build(dptr(src), dptr(dst), src.numel() if src is not None else 0, max_id, nshards,
+                                     ctypes.byref(self.handle), stream()), "shard_build")
+        ns = c_i32()
+        check(lib().gchi_shard_info(self.handle, ctypes.byref(ns), None, None))
+        self.ends = np.zeros(ns.value, np.uint32)
+        self.counts = np.zeros(ns.value, np.uint64)
+        check(lib().gchi_shard_info(self.handle, ctypes.byref(ns), self.ends.ctypes.data, self.counts.ctypes.data))
+        self.nshards = ns.value
+
+    def edges(self, p):
+        s, d, c = c_vp(), c_vp(), c_u64()
+        check(lib().gchi_shard_edges(self.handle, p, ctypes.byref(s), ctypes.byref(d), ctypes.byref(c)))
+        return s.value, d.value, c.value
+
+    def edges_host(self, p):
+        torch = _torch()
+        s, d, c = self.edges(p)
+        hs, hd = np.zeros(c, np.uint32), np.zeros(c, np.uint32)
+        if c:
+            check(lib().gchi_memcpy(c_vp(hs.ctypes.data), c_vp(s), 4 * c, stream()), "memcpy")
+            check(lib().gchi_memcpy(c_vp(hd.ctypes.data), c_vp(d), 4 * c, stream()), "memcpy")
+        return hs, hd
+
+    def encode(self, p, max_index=4096):
+        L = lib()
+        nbytes = c_u64()
+        check(L.gchi_shard_encode(self.handle, p, None, ctypes.byref(nbytes), None, 0, None, stream()))
+        adj = np.zeros(max(nbytes.value, 1), np.uint8)
+        idx = np.zeros(3 * max_index, np.uint64)
+        ni = c_u32()
+        check(L.gchi_shard_encode(self.handle, p, adj.ctypes.data, ctypes.byref(nbytes), idx.ctypes.data, max_index,
+                                  ctypes.byref(ni), stream()), "shard_encode")
+        return adj[:nbytes.value].tobytes(), idx[:3 * ni.value].reshape(-1, 3)
+
+    def write_files(self, base, edge_value_bytes):
+        check(lib().gchi_shard_write_files(self.handle, base.encode(), edge_value_bytes, stream()), "shard_write_files")
+
+    def close(self):
+        if self.handle:
+            lib().gchi_shard_destroy(self.handle)
             self.handle = c_vp()
 
     def __del__(self):

@@ -52,6 +52,10 @@ int gchi_device_open(int device);
 /* Number of compute units and GPU arch name of the current device. */
 int gchi_device_info(int *num_cus, char *arch, size_t arch_len);
 
+/* Copies bytes between any host/device pointers on the stream and waits for it
+ * (the pinned one-time upload/readback of the engine). */
+int gchi_memcpy(void *dst, const void *src, size_t bytes, void *stream);
+
 /* ----------------------------------------------------- synthetic R-MAT input */
 /* Deterministic R-MAT edge generator (counter-based RNG: edge i of a graph is a
  * pure function of (seed, i)); self-loops are kept here and removed by
@@ -174,6 +178,63 @@ int gchi_pr_exec_create(const gchi_pr_plan *plan, const gchi_pr_op *op, float in
                         float *d_x1, float *d_r0, float *d_r1, double *d_stats_all, gchi_pr_exec **out);
 int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
 int gchi_pr_exec_destroy(gchi_pr_exec *exec);
+
+/* --------------------------------------------------------- connected components */
+/* Labels = smallest vertex id of each weakly connected component: the fixed
+ * point of ConnectedComponentsProgram (example_apps/connectedcomponents.cpp:79-138),
+ * reached here by a lock-free union-find (roots always hooked under smaller
+ * roots).  Edges are read in either direction. */
+int gchi_cc_workspace(uint32_t nvertices, size_t *bytes);
+int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                     uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream);
+/* The reference's per-iteration form: labels = ids (iteration 0), then one
+ * synchronous min-propagation sweep over every edge in both directions;
+ * *d_changed = number of labels that changed (0 = converged,
+ * connectedcomponents.cpp:125-138). */
+int gchi_cc_labelprop_init(uint32_t *d_labels, uint32_t nvertices, void *stream);
+int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                            const uint32_t *d_labels_in, uint32_t *d_labels_out, uint64_t *d_changed, void *stream);
+
+/* ------------------------------------------------------------ triangle counting */
+/* Degree ordering (order_by_degree, src/preprocessing/util/orderbydegree.hpp:58-100:
+ * rank by (in+out degree, id), d_vertexmap[old] = new id) and the oriented,
+ * deduplicated adjacency N+(v) = neighbours of higher rank, ascending
+ * (d_off[N+1], d_adj[<= E]).  *h_noriented = number of oriented edges. */
+int gchi_tc_prepare_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
+int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                    uint32_t *d_vertexmap, uint32_t *d_off, uint32_t *d_adj, uint64_t *h_noriented,
+                    void *d_work, size_t work_bytes, void *stream);
+/* Per-vertex incident-triangle counts in the new id space (uint32, as
+ * <G>_degord.4B.vout of example_apps/trianglecounting.cpp:72) and the uint64
+ * total (:476-477).  *d_overflow != 0 when a count does not fit 32 bits. */
+int gchi_tc_count_workspace(uint32_t nvertices, size_t *bytes);
+int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_per_vertex,
+                  uint64_t *d_total, uint32_t *d_overflow, void *d_work, size_t work_bytes, void *stream);
+
+/* ------------------------------------------------------------------- sharder */
+/* determine_number_of_shards "auto" (src/preprocessing/sharder.hpp:555-587). */
+int gchi_shard_auto_count(uint64_t nedges, uint32_t edge_value_bytes, int membudget_mb, int *nshards);
+/* Shards an edge list resident in HBM (self-loops already dropped): interval
+ * cut of sharder::add/createnextshard/done (:861-927), per-shard (src,dst)
+ * order of finish_shard (:640), degrees (:748-751).  The handle owns the
+ * shard-major edge arrays in HBM. */
+typedef struct gchi_shards gchi_shards;
+int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t max_id, int nshards,
+                     gchi_shards **out, void *stream);
+int gchi_shard_destroy(gchi_shards *shards);
+int gchi_shard_info(const gchi_shards *shards, int *nshards_created, uint32_t *h_ends, uint64_t *h_counts);
+int gchi_shard_edges(const gchi_shards *shards, int shard, const uint32_t **d_src, const uint32_t **d_dst,
+                     uint64_t *count);
+int gchi_shard_degrees(const gchi_shards *shards, const int32_t **d_degs);
+/* The .adj byte stream of one shard (finish_shard, sharder.hpp:701-822) and its
+ * .adjidx entries as (vertexid, filepos, edgecounter) uint64 triples. */
+int gchi_shard_encode(const gchi_shards *shards, int shard, uint8_t *h_adj, uint64_t *adj_bytes,
+                      uint64_t *h_index, uint32_t max_index, uint32_t *nindex, void *stream);
+/* Writes the reference's on-disk layout (src/api/chifilenames.hpp:65-153):
+ * .intervals, .numvertices, _degs.bin, .adj/.adjidx per shard and
+ * uncompressed default-valued edge-data blocks + .size. */
+int gchi_shard_write_files(const gchi_shards *shards, const char *basefilename, uint32_t edge_value_bytes,
+                           void *stream);
 
 #ifdef __cplusplus
 }

@@ -44,8 +44,9 @@ def test_pagerank_vs_reference_golden(dev, golden, case, tune):
     for kind, teleport in (("pr_norm", 0.15 / n), ("pr_affine", 0.15)):
         r, stats, _ = run_pr(dev, src, dst, n, teleport, case["sweeps"], tune)
         ref = golden[f"{case['name']}__{kind}"]
-        if "hand" in case:
-            np.testing.assert_array_equal(r, ref)
+        max_indeg = int(np.bincount(dst, minlength=n).max())
+        if "hand" in case and max_indeg <= (tune[1] or 32):
+            np.testing.assert_array_equal(r, ref)  # every row summed sequentially in src order
         else:
             assert_close(r, ref)
         # FP64 statistics of the last sweep vs the oracle's (same definition)
