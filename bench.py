@@ -77,7 +77,7 @@ def main():
     g = gchi.DeviceGraph(src, dst, N)
     plan = gchi.PageRankPlan(g, args.edges_per_block, args.short_row_max)
     info = plan.info()
-    op = gchi.PrOp(0.15, 0.85)
+    op = gchi.pr_op(0.15, 0.85)
     log(f"graph N={N} E={E} plan={info}")
 
     if args.warmup > 0:

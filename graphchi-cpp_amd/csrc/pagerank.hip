@@ -21,6 +21,11 @@
 //   hub rows (in-degree >= W): their in-edges are cut into chunks of 4096; one
 //     workgroup reduces a chunk; a finalize kernel sums each hub's chunk
 //     partials in chunk order and applies.
+// Many graphs (a disjoint union with vertex bases) share one plan: blocks are
+// aligned to graph boundaries, so every block and hub belongs to one graph,
+// the apply can use a per-graph constant and FP64 statistics are per graph.
+// Tiles (hub chunks first, then regular blocks) are either one workgroup each
+// or pulled by persistent workgroups from a device work counter.
 // Every sum is in a fixed order, so a sweep is bit-reproducible run to run.
 #include "common.hpp"
 #include "internal.hpp"
@@ -31,27 +36,36 @@
 struct gchi_pr_plan {
     uint32_t N = 0;
     uint64_t E = 0;
+    uint32_t G = 1;        // graphs (disjoint union)
     int W = 2048;
     int short_max = 32;
+    int persistent = 0;
     uint32_t nblk = 0;     // regular blocks
     uint32_t nhub = 0;     // hub rows
     uint32_t nchunk = 0;   // hub chunks
-    uint32_t nfin = 0;     // finalize workgroups
+    uint32_t nfin = 0;     // finalize workgroups (single-graph plans)
+    uint32_t npersist = 0; // workgroups of the persistent sweep
     uint64_t e_reg = 0, e_hub = 0;
     const uint32_t *outdeg = nullptr;
     uint32_t *reg_off = nullptr;
     uint32_t *reg_src = nullptr;
     uint32_t *hub_bits = nullptr;
     uint32_t *blk_row = nullptr;
+    uint32_t *blk_graph = nullptr;
     uint32_t *hub_v = nullptr;
+    uint32_t *hub_graph = nullptr;
     uint32_t *hub_chunk_off = nullptr;
     uint32_t *chunk_beg = nullptr;
     uint32_t *chunk_len = nullptr;
     uint32_t *hub_src = nullptr;
+    uint32_t *gblk = nullptr;  // [G+1] block range per graph
+    uint32_t *ghub = nullptr;  // [G+1] hub range per graph
+    uint32_t *vbase = nullptr; // [G+1] vertex range per graph
     float *chunk_part = nullptr;
     double *blk_stat = nullptr;
+    double *hub_stat = nullptr;
     double *fin_stat = nullptr;
-    uint32_t *fin_counter = nullptr;
+    uint32_t *counters = nullptr;  // [0] finalize ticket, [1] tile counter
     void *mem = nullptr;
 };
 
@@ -67,22 +81,41 @@ constexpr int kFinThreads = 256;
 struct PrConsts {
     double teleport;
     double damping;
+    const double *g_tele;      // per-graph constant term, or null
+    const uint32_t *seed_bits; // seed bitmap, or null
+    double seed_tele;
 };
+
+struct PrArgs {
+    const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *hub_bits, *chunk_beg, *chunk_len, *hub_src, *outdeg;
+    const float *x_in, *r_in;
+    float *x_out, *r_out, *chunk_part;
+    double *blk_stat;
+    uint32_t *tile_counter;
+    uint32_t nchunk, ntiles;
+    int short_max;
+};
+
+GCHI_DEV double row_teleport(const PrConsts &c, uint32_t g, uint32_t v) {
+    double t = c.g_tele ? c.g_tele[g] : c.teleport;
+    if (c.seed_bits && ((c.seed_bits[v >> 5] >> (v & 31)) & 1u)) t += c.seed_tele;
+    return t;
+}
 
 // r = (float)(teleport + damping * (double)s), without contraction into an FMA:
 // the reference evaluates a double multiply and a double add, then rounds.
-GCHI_DEV float pr_apply(PrConsts c, float s) {
+GCHI_DEV float pr_apply(double teleport, double damping, float s) {
 #pragma clang fp contract(off)
-    double t = c.damping * (double)s;
-    double r = c.teleport + t;
+    double t = damping * (double)s;
+    double r = teleport + t;
     return (float)r;
 }
 
 GCHI_DEV float pr_scatter(float r, uint32_t outdeg) { return outdeg ? __fdiv_rn(r, (float)outdeg) : 0.0f; }
 
-GCHI_DEV void pr_finish_row(uint32_t v, float s, PrConsts c, const float *r_in, const uint32_t *outdeg, float *r_out,
-                            float *x_out, double &res, double &sum) {
-    float r = pr_apply(c, s);
+GCHI_DEV void pr_finish_row(uint32_t v, uint32_t g, float s, const PrConsts &c, const float *r_in,
+                            const uint32_t *outdeg, float *r_out, float *x_out, double &res, double &sum) {
+    float r = pr_apply(row_teleport(c, g, v), c.damping, s);
     float ro = r_in[v];
     r_out[v] = r;
     x_out[v] = pr_scatter(r, outdeg[v]);
@@ -90,48 +123,24 @@ GCHI_DEV void pr_finish_row(uint32_t v, float s, PrConsts c, const float *r_in, 
     sum += (double)r;
 }
 
-template <typename T>
-GCHI_DEV T block_sum_ordered(T v, T *s_red) {
-    // wave butterfly, then waves in index order
-    v = wave_sum(v);
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    if (lane == 0) s_red[wave] = v;
-    __syncthreads();
-    T t = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x / kWave); w++) t += s_red[w];
-    __syncthreads();
-    return t;  // valid in thread 0
-}
-
-__global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(
-    const uint32_t *__restrict__ reg_off, const uint32_t *__restrict__ reg_src, const uint32_t *__restrict__ blk_row,
-    const uint32_t *__restrict__ hub_bits, const uint32_t *__restrict__ chunk_beg,
-    const uint32_t *__restrict__ chunk_len, const uint32_t *__restrict__ hub_src, const uint32_t *__restrict__ outdeg,
-    const float *__restrict__ x_in, const float *__restrict__ r_in, float *__restrict__ x_out,
-    float *__restrict__ r_out, float *__restrict__ chunk_part, double *__restrict__ blk_stat, uint32_t nchunk,
-    int short_max, PrConsts c) {
-    extern __shared__ float s_val[];  // 2W floats
-    __shared__ uint32_t s_long[kMaxLong];
-    __shared__ uint32_t s_nlong;
-    __shared__ double s_red[2 * kPrWaves];
-    __shared__ float s_redf[kPrWaves];
+// One tile: a hub chunk (tile < nchunk) or a regular block.
+GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *s_val, uint32_t *s_long,
+                      uint32_t *s_nlong, double *s_red, float *s_redf) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-
-    if (blockIdx.x < nchunk) {
+    if (tile < a.nchunk) {
         // ---- hub chunk: partial sum of <= 4096 in-edges of one hub row ----
-        const uint32_t cb = chunk_beg[blockIdx.x], cl = chunk_len[blockIdx.x];
+        const uint32_t cb = a.chunk_beg[tile], cl = a.chunk_len[tile];
         uint32_t idx[kHubChunk / kPrThreads];
 #pragma unroll
         for (int j = 0; j < kHubChunk / kPrThreads; j++) {
             uint32_t k = (uint32_t)j * kPrThreads + tid;
-            idx[j] = k < cl ? __builtin_nontemporal_load(&hub_src[cb + k]) : 0u;
+            idx[j] = k < cl ? __builtin_nontemporal_load(&a.hub_src[cb + k]) : 0u;
         }
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < kHubChunk / kPrThreads; j++) {
             uint32_t k = (uint32_t)j * kPrThreads + tid;
-            float v = k < cl ? x_in[idx[j]] : 0.0f;
+            float v = k < cl ? a.x_in[idx[j]] : 0.0f;
             acc += v;
         }
         acc = wave_sum(acc);
@@ -140,29 +149,30 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(
         if (tid == 0) {
             float t = 0.0f;
             for (int w = 0; w < kPrWaves; w++) t += s_redf[w];
-            chunk_part[blockIdx.x] = t;
+            a.chunk_part[tile] = t;
         }
         return;
     }
 
     // ---- regular block ----
-    const uint32_t b = blockIdx.x - nchunk;
-    const uint32_t rb = blk_row[b], re = blk_row[b + 1];
-    const uint32_t e0 = reg_off[rb], e1 = reg_off[re];
+    const uint32_t b = tile - a.nchunk;
+    const uint32_t rb = a.blk_row[b], re = a.blk_row[b + 1];
+    const uint32_t g = a.blk_graph[b];
+    const uint32_t e0 = a.reg_off[rb], e1 = a.reg_off[re];
     const uint32_t span = e1 - e0;
-    if (tid == 0) s_nlong = 0;
+    if (tid == 0) *s_nlong = 0;
     for (uint32_t base = 0; base < span; base += 8 * kPrThreads) {
         uint32_t idx[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             uint32_t k = base + (uint32_t)j * kPrThreads + tid;
-            idx[j] = k < span ? __builtin_nontemporal_load(&reg_src[e0 + k]) : 0u;
+            idx[j] = k < span ? __builtin_nontemporal_load(&a.reg_src[e0 + k]) : 0u;
         }
         float val[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             uint32_t k = base + (uint32_t)j * kPrThreads + tid;
-            val[j] = k < span ? x_in[idx[j]] : 0.0f;
+            val[j] = k < span ? a.x_in[idx[j]] : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -174,26 +184,26 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(
 
     double res = 0.0, sum = 0.0;
     for (uint32_t v = rb + tid; v < re; v += kPrThreads) {
-        if ((hub_bits[v >> 5] >> (v & 31)) & 1u) continue;
-        uint32_t beg = reg_off[v] - e0, end = reg_off[v + 1] - e0;
-        if ((int)(end - beg) <= short_max) {
+        if ((a.hub_bits[v >> 5] >> (v & 31)) & 1u) continue;
+        uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
+        if ((int)(end - beg) <= a.short_max) {
             float s = 0.0f;
             for (uint32_t k = beg; k < end; k++) s += s_val[k];
-            pr_finish_row(v, s, c, r_in, outdeg, r_out, x_out, res, sum);
+            pr_finish_row(v, g, s, c, a.r_in, a.outdeg, a.r_out, a.x_out, res, sum);
         } else {
-            uint32_t slot = atomicAdd(&s_nlong, 1u);
+            uint32_t slot = atomicAdd(s_nlong, 1u);
             s_long[slot] = v;
         }
     }
     __syncthreads();
-    const uint32_t nlong = s_nlong;
+    const uint32_t nlong = *s_nlong;
     for (uint32_t li = wave; li < nlong; li += kPrWaves) {
         uint32_t v = s_long[li];
-        uint32_t beg = reg_off[v] - e0, end = reg_off[v + 1] - e0;
+        uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
         float s = 0.0f;
         for (uint32_t k = beg + lane; k < end; k += kWave) s += s_val[k];
         s = wave_sum(s);
-        if (lane == 0) pr_finish_row(v, s, c, r_in, outdeg, r_out, x_out, res, sum);
+        if (lane == 0) pr_finish_row(v, g, s, c, a.r_in, a.outdeg, a.r_out, a.x_out, res, sum);
     }
     res = wave_sum(res);
     sum = wave_sum(sum);
@@ -203,23 +213,53 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(
     }
     __syncthreads();
     if (tid == 0) {
-        double a = 0.0, s2 = 0.0;
+        double r0 = 0.0, s0 = 0.0;
         for (int w = 0; w < kPrWaves; w++) {
-            a += s_red[2 * w];
-            s2 += s_red[2 * w + 1];
+            r0 += s_red[2 * w];
+            s0 += s_red[2 * w + 1];
         }
-        blk_stat[2 * b] = a;
-        blk_stat[2 * b + 1] = s2;
+        a.blk_stat[2 * b] = r0;
+        a.blk_stat[2 * b + 1] = s0;
     }
 }
 
-// Hub rows (sum of chunk partials, apply) + deterministic reduction of the
-// per-block statistics; the last finalize workgroup writes d_stats.
+__global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(PrArgs a, PrConsts c) {
+    extern __shared__ float s_val[];  // 2W floats
+    __shared__ uint32_t s_long[kMaxLong];
+    __shared__ uint32_t s_nlong;
+    __shared__ double s_red[2 * kPrWaves];
+    __shared__ float s_redf[kPrWaves];
+    pr_tile(a, c, blockIdx.x, s_val, s_long, &s_nlong, s_red, s_redf);
+}
+
+// Persistent form: workgroups pull tiles from the work counter until the queue
+// is drained (hub chunks first, then regular blocks).  Every workgroup exits
+// once the counter passes ntiles; the finalize kernel resets the counter.
+__global__ __launch_bounds__(kPrThreads) void pr_sweep_persistent_kernel(PrArgs a, PrConsts c) {
+    extern __shared__ float s_val[];
+    __shared__ uint32_t s_long[kMaxLong];
+    __shared__ uint32_t s_nlong;
+    __shared__ double s_red[2 * kPrWaves];
+    __shared__ float s_redf[kPrWaves];
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= a.ntiles) break;
+        pr_tile(a, c, tile, s_val, s_long, &s_nlong, s_red, s_redf);
+        __syncthreads();
+    }
+}
+
+// Single-graph finalize: hub rows (sum of chunk partials, apply) + the
+// deterministic reduction of per-block statistics; the last workgroup writes
+// d_stats and resets the counters for the next sweep / graph replay.
 __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     const uint32_t *__restrict__ hub_v, const uint32_t *__restrict__ hub_chunk_off,
     const float *__restrict__ chunk_part, const uint32_t *__restrict__ outdeg, const float *__restrict__ r_in,
     float *__restrict__ r_out, float *__restrict__ x_out, const double *__restrict__ blk_stat, uint32_t nblk,
-    uint32_t nhub, double *fin_stat, uint32_t *fin_counter, double *d_stats, PrConsts c) {
+    uint32_t nhub, double *fin_stat, uint32_t *counters, double *d_stats, PrConsts c) {
     __shared__ double s_red[2 * (kFinThreads / kWave)];
     __shared__ bool s_last;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -231,7 +271,7 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
         float s = 0.0f;
         for (uint32_t k = c0 + lane; k < c1; k += kWave) s += chunk_part[k];
         s = wave_sum(s);
-        if (lane == 0) pr_finish_row(hub_v[h], s, c, r_in, outdeg, r_out, x_out, res, sum);
+        if (lane == 0) pr_finish_row(hub_v[h], 0, s, c, r_in, outdeg, r_out, x_out, res, sum);
     }
     const uint32_t gt = blockIdx.x * kFinThreads + tid;
     const uint32_t nthreads_total = gridDim.x * kFinThreads;
@@ -255,7 +295,7 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
         st_relaxed_agent(&fin_stat[2 * blockIdx.x], a);
         st_relaxed_agent(&fin_stat[2 * blockIdx.x + 1], s2);
         __threadfence();
-        uint32_t ticket = atomicAdd(fin_counter, 1u);
+        uint32_t ticket = atomicAdd(&counters[0], 1u);
         s_last = (ticket == gridDim.x - 1);
     }
     __syncthreads();
@@ -268,19 +308,96 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
         }
         d_stats[0] = a;
         d_stats[1] = s2;
-        *fin_counter = 0u;  // ready for the next sweep / graph replay
+        counters[0] = 0u;
+        counters[1] = 0u;
     }
 }
 
-__global__ void pr_init_kernel(const uint32_t *outdeg, uint32_t n, float init, float *x, float *r) {
+// Multi-graph finalize, step 1: hub rows (wave per hub), per-hub statistics.
+__global__ __launch_bounds__(kFinThreads) void pr_hub_apply_kernel(
+    const uint32_t *__restrict__ hub_v, const uint32_t *__restrict__ hub_graph,
+    const uint32_t *__restrict__ hub_chunk_off, const float *__restrict__ chunk_part,
+    const uint32_t *__restrict__ outdeg, const float *__restrict__ r_in, float *__restrict__ r_out,
+    float *__restrict__ x_out, uint32_t nhub, double *hub_stat, PrConsts c) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t nw = gridDim.x * (kFinThreads / kWave);
+    for (uint32_t h = blockIdx.x * (kFinThreads / kWave) + wave; h < nhub; h += nw) {
+        uint32_t c0 = hub_chunk_off[h], c1 = hub_chunk_off[h + 1];
+        float s = 0.0f;
+        for (uint32_t k = c0 + lane; k < c1; k += kWave) s += chunk_part[k];
+        s = wave_sum(s);
+        if (lane == 0) {
+            double res = 0.0, sum = 0.0;
+            pr_finish_row(hub_v[h], hub_graph[h], s, c, r_in, outdeg, r_out, x_out, res, sum);
+            hub_stat[2 * h] = res;
+            hub_stat[2 * h + 1] = sum;
+        }
+    }
+}
+
+// Multi-graph finalize, step 2: one workgroup per graph reduces its blocks'
+// and hubs' statistics in a fixed order into d_stats[2g], d_stats[2g+1].
+__global__ __launch_bounds__(kFinThreads) void pr_graph_reduce_kernel(const uint32_t *__restrict__ gblk,
+                                                                      const uint32_t *__restrict__ ghub,
+                                                                      const double *__restrict__ blk_stat,
+                                                                      const double *__restrict__ hub_stat,
+                                                                      uint32_t G, uint32_t *counters,
+                                                                      double *d_stats) {
+    __shared__ double s_red[2 * (kFinThreads / kWave)];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+        double res = 0.0, sum = 0.0;
+        for (uint32_t i = gblk[g] + tid; i < gblk[g + 1]; i += kFinThreads) {
+            res += blk_stat[2 * i];
+            sum += blk_stat[2 * i + 1];
+        }
+        for (uint32_t h = ghub[g] + tid; h < ghub[g + 1]; h += kFinThreads) {
+            res += hub_stat[2 * h];
+            sum += hub_stat[2 * h + 1];
+        }
+        res = wave_sum(res);
+        sum = wave_sum(sum);
+        if (lane == 0) {
+            s_red[2 * wave] = res;
+            s_red[2 * wave + 1] = sum;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, s2 = 0.0;
+            for (int w = 0; w < kFinThreads / kWave; w++) {
+                a += s_red[2 * w];
+                s2 += s_red[2 * w + 1];
+            }
+            d_stats[2 * g] = a;
+            d_stats[2 * g + 1] = s2;
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && tid == 0) counters[1] = 0u;
+}
+
+__global__ void pr_init_kernel(const uint32_t *outdeg, uint32_t n, float init, const uint32_t *seed_bits,
+                               float seed_value, float *x, float *r) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        r[v] = init;
-        x[v] = pr_scatter(init, outdeg[v]);
+        float r0 = init;
+        if (seed_bits) r0 = ((seed_bits[v >> 5] >> (v & 31)) & 1u) ? seed_value : 0.0f;
+        r[v] = r0;
+        x[v] = pr_scatter(r0, outdeg[v]);
     }
 }
 
 // ---------------------------------------------------------------- plan build
+GCHI_DEV uint32_t graph_of(const uint32_t *vbase, uint32_t G, uint32_t v) {
+    uint32_t lo = 0, hi = G - 1;  // largest g with vbase[g] <= v
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (vbase[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ void plan_rows(const uint32_t *in_off, uint32_t N, int W, uint32_t *reg_deg, uint32_t *hub_flag,
                           uint32_t *hub_bits) {
     uint32_t stride = gridDim.x * blockDim.x;
@@ -335,26 +452,56 @@ __global__ void plan_chunks(const uint32_t *hub_off, const uint32_t *hub_chunk_o
     }
 }
 
-__global__ void plan_blocks(const uint32_t *reg_off, uint32_t N, int W, uint32_t nblk, uint32_t *blk_row) {
+// blocks per graph: the last row's work position within its graph / W + 1
+__global__ void plan_graph_blocks(const uint32_t *reg_off, const uint32_t *vbase, uint32_t G, int W,
+                                  uint32_t *nblk_g) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
+        uint32_t v0 = vbase[g], v1 = vbase[g + 1];
+        uint64_t p_last = (uint64_t)(reg_off[v1 - 1] - reg_off[v0]) + (v1 - 1 - v0);
+        nblk_g[g] = (uint32_t)(p_last / (uint64_t)W + 1);
+    }
+}
+
+__global__ void plan_blocks(const uint32_t *reg_off, const uint32_t *vbase, uint32_t G, const uint32_t *gblk,
+                            uint32_t N, int W, uint32_t nblk, uint32_t *blk_row, uint32_t *blk_graph) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) {
-        uint64_t bv = ((uint64_t)reg_off[v] + v) / (uint64_t)W;
-        int64_t bp = v == 0 ? -1 : (int64_t)(((uint64_t)reg_off[v - 1] + v - 1) / (uint64_t)W);
-        for (int64_t bb = bp + 1; bb <= (int64_t)bv; bb++) blk_row[bb] = v;
+        const uint32_t g = graph_of(vbase, G, v);
+        const uint32_t v0 = vbase[g];
+        const uint64_t base = gblk[g];
+        const uint64_t bv = base + ((uint64_t)(reg_off[v] - reg_off[v0]) + (v - v0)) / (uint64_t)W;
+        int64_t bp = (int64_t)base - 1;
+        if (v > v0) bp = (int64_t)(base + ((uint64_t)(reg_off[v - 1] - reg_off[v0]) + (v - 1 - v0)) / (uint64_t)W);
+        for (int64_t bb = bp + 1; bb <= (int64_t)bv; bb++) {
+            blk_row[bb] = v;
+            blk_graph[bb] = g;
+        }
         if (v == N - 1) blk_row[nblk] = N;
     }
 }
 
-static int pr_default_W(const gchi_pr_tune *t) {
-    int W = (t && t->edges_per_block > 0) ? t->edges_per_block : 2048;
-    return W;
+__global__ void plan_hub_graphs(const uint32_t *hub_v, uint32_t nhub, const uint32_t *vbase, uint32_t G,
+                                uint32_t *hub_graph) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhub; h += stride)
+        hub_graph[h] = graph_of(vbase, G, hub_v[h]);
 }
 
-}  // namespace gchi
-
-using namespace gchi;
-
-namespace gchi {
+// ghub[g] = first hub with vertex >= vbase[g]
+__global__ void plan_graph_hubs(const uint32_t *hub_v, uint32_t nhub, const uint32_t *vbase, uint32_t G,
+                                uint32_t *ghub) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += stride) {
+        uint32_t target = vbase[g], lo = 0, hi = nhub;
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (hub_v[mid] >= target) hi = mid;
+            else lo = mid + 1;
+        }
+        ghub[g] = g == G ? nhub : lo;
+    }
+}
 
 // Temporary device allocations of the plan build, freed on every exit path.
 struct TmpPool {
@@ -377,9 +524,10 @@ static int read_u32(const uint32_t *d, uint32_t *h, hipStream_t s) {
     return GCHI_OK;
 }
 
-static int plan_build(const gchi_dgraph *g, int W, int short_max, gchi_pr_plan *p, hipStream_t s) {
+static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi_pr_plan *p, hipStream_t s) {
     const uint32_t N = g->nvertices;
     const uint64_t E = g->nedges;
+    const uint32_t G = p->G;
     TmpPool tp;
     const size_t nwords = (N + 31) / 32;
     uint32_t *reg_deg = tp.alloc<uint32_t>(N);
@@ -387,23 +535,31 @@ static int plan_build(const gchi_dgraph *g, int W, int short_max, gchi_pr_plan *
     uint32_t *hub_idx = tp.alloc<uint32_t>(N + 1);
     uint32_t *reg_off = tp.alloc<uint32_t>(N + 1);
     uint32_t *bits = tp.alloc<uint32_t>(nwords);
-    const size_t swb = scan_workspace((size_t)N + 1);
+    uint32_t *vbase = tp.alloc<uint32_t>(G + 1);
+    uint32_t *nblk_g = tp.alloc<uint32_t>(G + 1);
+    uint32_t *gblk = tp.alloc<uint32_t>(G + 1);
+    const size_t swb = scan_workspace((size_t)std::max(N, G) + 1);
     void *sw = tp.alloc<char>(swb);
-    GCHI_CHECK(reg_deg && hub_flag && hub_idx && reg_off && bits && sw, GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
+    GCHI_CHECK(reg_deg && hub_flag && hub_idx && reg_off && bits && vbase && nblk_g && gblk && sw, GCHI_ERR_HIP,
+               "pr_plan: hipMalloc failed");
+    GCHI_HIP(hipMemcpyAsync(vbase, h_vbase, 4 * (size_t)(G + 1), hipMemcpyHostToDevice, s));
     launch_fill_u32(bits, 0u, nwords, s);
     hipLaunchKernelGGL(plan_rows, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, g->in_off, N, W, reg_deg, hub_flag,
                        bits);
     GCHI_LAUNCH_CHECK();
     GCHI_TRY(exclusive_scan_u32(reg_deg, reg_off, N, sw, swb, s));
     GCHI_TRY(exclusive_scan_u32(hub_flag, hub_idx, N, sw, swb, s));
-    uint32_t e_reg = 0, nhub = 0, reg_last = 0;
+    hipLaunchKernelGGL(plan_graph_blocks, dim3(grid_for(G, 256, 4096)), dim3(256), 0, s, reg_off, vbase, G, W, nblk_g);
+    GCHI_LAUNCH_CHECK();
+    GCHI_TRY(exclusive_scan_u32(nblk_g, gblk, G, sw, swb, s));
+    uint32_t e_reg = 0, nhub = 0, nblk = 0;
     GCHI_TRY(read_u32(reg_off + N, &e_reg, s));
     GCHI_TRY(read_u32(hub_idx + N, &nhub, s));
-    GCHI_TRY(read_u32(reg_off + (N - 1), &reg_last, s));
+    GCHI_TRY(read_u32(gblk + G, &nblk, s));
     p->e_reg = e_reg;
     p->e_hub = E - e_reg;
     p->nhub = nhub;
-    p->nblk = (uint32_t)(((uint64_t)reg_last + (N - 1)) / (uint64_t)W + 1);
+    p->nblk = nblk;
 
     uint32_t *hub_v = tp.alloc<uint32_t>(nhub + 1);
     uint32_t *hub_deg = tp.alloc<uint32_t>(nhub + 1);
@@ -430,57 +586,79 @@ static int plan_build(const gchi_dgraph *g, int W, int short_max, gchi_pr_plan *
     zz.take<uint32_t>(std::max<uint64_t>(p->e_reg, 1));
     zz.take<uint32_t>(nwords);
     zz.take<uint32_t>(p->nblk + 1);
+    zz.take<uint32_t>(p->nblk + 1);
+    zz.take<uint32_t>(nhub + 1);
     zz.take<uint32_t>(nhub + 1);
     zz.take<uint32_t>(nhub + 1);
     zz.take<uint32_t>(nchunk + 1);
     zz.take<uint32_t>(nchunk + 1);
     zz.take<uint32_t>(std::max<uint64_t>(p->e_hub, 1));
+    zz.take<uint32_t>(G + 1);
+    zz.take<uint32_t>(G + 1);
+    zz.take<uint32_t>(G + 1);
     zz.take<float>(nchunk + 1);
     zz.take<double>(2 * (size_t)p->nblk);
+    zz.take<double>(2 * (size_t)nhub + 2);
     zz.take<double>(2 * (size_t)p->nfin);
-    zz.take<uint32_t>(1);
+    zz.take<uint32_t>(2);
     GCHI_HIP(hipMalloc(&p->mem, zz.off));
     Carver c(p->mem, zz.off);
     p->reg_off = c.take<uint32_t>(N + 1);
     p->reg_src = c.take<uint32_t>(std::max<uint64_t>(p->e_reg, 1));
     p->hub_bits = c.take<uint32_t>(nwords);
     p->blk_row = c.take<uint32_t>(p->nblk + 1);
+    p->blk_graph = c.take<uint32_t>(p->nblk + 1);
     p->hub_v = c.take<uint32_t>(nhub + 1);
+    p->hub_graph = c.take<uint32_t>(nhub + 1);
     p->hub_chunk_off = c.take<uint32_t>(nhub + 1);
     p->chunk_beg = c.take<uint32_t>(nchunk + 1);
     p->chunk_len = c.take<uint32_t>(nchunk + 1);
     p->hub_src = c.take<uint32_t>(std::max<uint64_t>(p->e_hub, 1));
+    p->gblk = c.take<uint32_t>(G + 1);
+    p->ghub = c.take<uint32_t>(G + 1);
+    p->vbase = c.take<uint32_t>(G + 1);
     p->chunk_part = c.take<float>(nchunk + 1);
     p->blk_stat = c.take<double>(2 * (size_t)p->nblk);
+    p->hub_stat = c.take<double>(2 * (size_t)nhub + 2);
     p->fin_stat = c.take<double>(2 * (size_t)p->nfin);
-    p->fin_counter = c.take<uint32_t>(1);
+    p->counters = c.take<uint32_t>(2);
     GCHI_HIP(hipMemcpyAsync(p->reg_off, reg_off, (size_t)(N + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_bits, bits, nwords * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_v, hub_v, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_chunk_off, hub_choff, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
-    launch_fill_u32(p->fin_counter, 0u, 1, s);
+    GCHI_HIP(hipMemcpyAsync(p->gblk, gblk, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, s));
+    GCHI_HIP(hipMemcpyAsync(p->vbase, vbase, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, s));
+    launch_fill_u32(p->counters, 0u, 2, s);
     if (E)
         hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, g->in_off,
                            g->in_src, N, hub_flag, hub_idx, reg_off, hub_off, p->reg_src, p->hub_src);
-    if (nhub)
+    if (nhub) {
         hipLaunchKernelGGL(plan_chunks, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, hub_off, hub_choff, nhub,
                            p->chunk_beg, p->chunk_len);
-    hipLaunchKernelGGL(plan_blocks, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->reg_off, N, W, p->nblk,
-                       p->blk_row);
+        hipLaunchKernelGGL(plan_hub_graphs, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, p->hub_v, nhub,
+                           p->vbase, G, p->hub_graph);
+    }
+    hipLaunchKernelGGL(plan_graph_hubs, dim3(grid_for(G + 1, 256, 4096)), dim3(256), 0, s, p->hub_v, nhub, p->vbase, G,
+                       p->ghub);
+    hipLaunchKernelGGL(plan_blocks, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->reg_off, p->vbase, G, p->gblk, N,
+                       W, p->nblk, p->blk_row, p->blk_graph);
     GCHI_LAUNCH_CHECK();
     GCHI_HIP(hipStreamSynchronize(s));
     return GCHI_OK;
 }
 
-}  // namespace gchi
-
-extern "C" int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream) {
-    GCHI_CHECK(g && out, GCHI_ERR_INVALID, "pr_plan_create: null argument");
-    GCHI_CHECK(g->nvertices > 0, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
+static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G, const gchi_pr_tune *tune,
+                       gchi_pr_plan **out, hipStream_t s) {
+    GCHI_CHECK(g && out && h_vbase, GCHI_ERR_INVALID, "pr_plan_create: null argument");
+    GCHI_CHECK(g->nvertices > 0 && G >= 1, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
     GCHI_CHECK(g->in_off && g->outdeg && (g->nedges == 0 || g->in_src), GCHI_ERR_INVALID,
                "pr_plan_create: graph arrays missing");
     GCHI_CHECK(g->nedges < (1ull << 32), GCHI_ERR_UNSUPPORTED, "pr_plan_create: E must be < 2^32");
-    const int W = pr_default_W(tune);
+    GCHI_CHECK(h_vbase[0] == 0 && h_vbase[G] == g->nvertices, GCHI_ERR_INVALID,
+               "pr_plan_create: vertex bases must start at 0 and end at N");
+    for (uint32_t i = 0; i < G; i++)
+        GCHI_CHECK(h_vbase[i + 1] > h_vbase[i], GCHI_ERR_INVALID, "pr_plan_create: every graph needs a vertex");
+    const int W = (tune && tune->edges_per_block > 0) ? tune->edges_per_block : 2048;
     GCHI_CHECK(W >= 256 && W <= kMaxW && W % 256 == 0, GCHI_ERR_INVALID,
                "pr_plan_create: edges_per_block must be a multiple of 256 in [256, 4096]");
     const int short_max = (tune && tune->short_row_max > 0) ? tune->short_row_max : 32;
@@ -490,10 +668,19 @@ extern "C" int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tun
     gchi_pr_plan *p = new gchi_pr_plan();
     p->N = g->nvertices;
     p->E = g->nedges;
+    p->G = G;
     p->W = W;
     p->short_max = short_max;
+    p->persistent = tune ? (tune->persistent != 0) : 0;
     p->outdeg = g->outdeg;
-    int st = plan_build(g, W, short_max, p, as_stream(stream));
+    int st = plan_build(g, h_vbase, W, p, s);
+    if (st == GCHI_OK) {
+        int dev = 0, ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            ncu = prop.multiProcessorCount;
+        p->npersist = (uint32_t)std::min<uint64_t>((uint64_t)ncu * 8, (uint64_t)p->nblk + p->nchunk);
+    }
     if (st != GCHI_OK) {
         if (p->mem) (void)hipFree(p->mem);
         delete p;
@@ -501,6 +688,69 @@ extern "C" int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tun
     }
     *out = p;
     return GCHI_OK;
+}
+
+int pr_init(const gchi_pr_plan *p, float init, const uint32_t *seed_bits, float seed_value, float *x, float *r,
+            hipStream_t s) {
+    hipLaunchKernelGGL(pr_init_kernel, dim3(grid_for(p->N, 256, 65536)), dim3(256), 0, s, p->outdeg, p->N, init,
+                       seed_bits, seed_value, x, r);
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
+int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
+             float *r_out, double *d_stats, hipStream_t s) {
+    PrConsts c{op->teleport, op->damping, op->d_graph_teleport, op->d_seed_bits, op->seed_teleport};
+    PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->hub_bits, p->chunk_beg, p->chunk_len, p->hub_src,
+             p->outdeg, x_in, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + 1, p->nchunk,
+             p->nchunk + p->nblk, p->short_max};
+    const size_t shmem = (size_t)2 * p->W * sizeof(float);
+    if (p->persistent)
+        hipLaunchKernelGGL(pr_sweep_persistent_kernel, dim3(p->npersist), dim3(kPrThreads), shmem, s, a, c);
+    else
+        hipLaunchKernelGGL(pr_sweep_kernel, dim3(p->nchunk + p->nblk), dim3(kPrThreads), shmem, s, a, c);
+    if (p->G == 1) {
+        hipLaunchKernelGGL(pr_finalize_kernel, dim3(p->nfin), dim3(kFinThreads), 0, s, p->hub_v, p->hub_chunk_off,
+                           p->chunk_part, p->outdeg, r_in, r_out, x_out, p->blk_stat, p->nblk, p->nhub, p->fin_stat,
+                           p->counters, d_stats, c);
+    } else {
+        if (p->nhub)
+            hipLaunchKernelGGL(pr_hub_apply_kernel, dim3(grid_for((uint64_t)p->nhub * kWave, kFinThreads, 1024)),
+                               dim3(kFinThreads), 0, s, p->hub_v, p->hub_graph, p->hub_chunk_off, p->chunk_part,
+                               p->outdeg, r_in, r_out, x_out, p->nhub, p->hub_stat, c);
+        hipLaunchKernelGGL(pr_graph_reduce_kernel, dim3(std::min<uint32_t>(p->G, 4096)), dim3(kFinThreads), 0, s,
+                           p->gblk, p->ghub, p->blk_stat, p->hub_stat, p->G, p->counters, d_stats);
+    }
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
+struct PlanView {
+    uint32_t N, nblk, nhub, nchunk, npersist;
+    const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off, *chunk_beg, *chunk_len, *hub_src,
+        *outdeg;
+};
+
+int pr_plan_view(const gchi_pr_plan *p, PlanView *v) {
+    GCHI_CHECK(p && v, GCHI_ERR_INVALID, "plan view: null argument");
+    *v = PlanView{p->N, p->nblk, p->nhub, p->nchunk, p->npersist, p->reg_off, p->reg_src, p->blk_row, p->hub_bits,
+                  p->hub_v, p->hub_chunk_off, p->chunk_beg, p->chunk_len, p->hub_src, p->outdeg};
+    return GCHI_OK;
+}
+
+}  // namespace gchi
+
+using namespace gchi;
+
+extern "C" int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream) {
+    GCHI_CHECK(g, GCHI_ERR_INVALID, "pr_plan_create: null argument");
+    uint32_t vb[2] = {0, g->nvertices};
+    return plan_create(g, vb, 1, tune, out, as_stream(stream));
+}
+
+extern "C" int gchi_pr_plan_create_multi(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t ngraphs,
+                                         const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream) {
+    return plan_create(g, h_vbase, ngraphs, tune, out, as_stream(stream));
 }
 
 extern "C" int gchi_pr_plan_destroy(gchi_pr_plan *p) {
@@ -521,37 +771,21 @@ extern "C" int gchi_pr_plan_info_get(const gchi_pr_plan *p, gchi_pr_plan_info *i
     info->hub_chunks = p->nchunk;
     info->regular_edges = p->e_reg;
     info->hub_edges = p->e_hub;
+    info->ngraphs = p->G;
+    info->persistent = p->persistent;
+    info->persistent_workgroups = p->npersist;
     return GCHI_OK;
 }
-
-namespace gchi {
-
-int pr_init(const gchi_pr_plan *p, float init, float *x, float *r, hipStream_t s) {
-    hipLaunchKernelGGL(pr_init_kernel, dim3(grid_for(p->N, 256, 65536)), dim3(256), 0, s, p->outdeg, p->N, init, x, r);
-    GCHI_LAUNCH_CHECK();
-    return GCHI_OK;
-}
-
-int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
-             float *r_out, double *d_stats, hipStream_t s) {
-    PrConsts c{op->teleport, op->damping};
-    const unsigned grid = p->nchunk + p->nblk;
-    const size_t shmem = (size_t)2 * p->W * sizeof(float);
-    hipLaunchKernelGGL(pr_sweep_kernel, dim3(grid), dim3(kPrThreads), shmem, s, p->reg_off, p->reg_src, p->blk_row,
-                       p->hub_bits, p->chunk_beg, p->chunk_len, p->hub_src, p->outdeg, x_in, r_in, x_out, r_out,
-                       p->chunk_part, p->blk_stat, p->nchunk, p->short_max, c);
-    hipLaunchKernelGGL(pr_finalize_kernel, dim3(p->nfin), dim3(kFinThreads), 0, s, p->hub_v, p->hub_chunk_off,
-                       p->chunk_part, p->outdeg, r_in, r_out, x_out, p->blk_stat, p->nblk, p->nhub, p->fin_stat,
-                       p->fin_counter, d_stats, c);
-    GCHI_LAUNCH_CHECK();
-    return GCHI_OK;
-}
-
-}  // namespace gchi
 
 extern "C" int gchi_pr_init(const gchi_pr_plan *p, float init, float *d_x, float *d_r, void *stream) {
     GCHI_CHECK(p && d_x && d_r, GCHI_ERR_INVALID, "pr_init: null argument");
-    return pr_init(p, init, d_x, d_r, as_stream(stream));
+    return pr_init(p, init, nullptr, 0.0f, d_x, d_r, as_stream(stream));
+}
+
+extern "C" int gchi_pr_init_seeds(const gchi_pr_plan *p, const uint32_t *d_seed_bits, float seed_value, float *d_x,
+                                  float *d_r, void *stream) {
+    GCHI_CHECK(p && d_seed_bits && d_x && d_r, GCHI_ERR_INVALID, "pr_init_seeds: null argument");
+    return pr_init(p, 0.0f, d_seed_bits, seed_value, d_x, d_r, as_stream(stream));
 }
 
 extern "C" int gchi_pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *d_x_in, const float *d_r_in,
@@ -559,6 +793,7 @@ extern "C" int gchi_pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const 
     GCHI_CHECK(p && op && d_x_in && d_r_in && d_x_out && d_r_out && d_stats, GCHI_ERR_INVALID,
                "pr_sweep: null argument");
     GCHI_CHECK(d_x_in != d_x_out && d_r_in != d_r_out, GCHI_ERR_INVALID, "pr_sweep: in and out buffers must differ");
+    GCHI_CHECK(p->G == 1 || op->d_graph_teleport || op->teleport >= 0, GCHI_ERR_INVALID, "pr_sweep: bad operator");
     return pr_sweep(p, op, d_x_in, d_r_in, d_x_out, d_r_out, d_stats, as_stream(stream));
 }
 
@@ -569,8 +804,9 @@ struct gchi_pr_exec {
     int nsweeps = 0;
 };
 
-extern "C" int gchi_pr_exec_create(const gchi_pr_plan *p, const gchi_pr_op *op, float init, int nsweeps, float *d_x0,
-                                   float *d_x1, float *d_r0, float *d_r1, double *d_stats_all, gchi_pr_exec **out) {
+extern "C" int gchi_pr_exec_create_ex(const gchi_pr_plan *p, const gchi_pr_op *op, float init, int do_init,
+                                      int nsweeps, float *d_x0, float *d_x1, float *d_r0, float *d_r1,
+                                      double *d_stats_all, gchi_pr_exec **out) {
     GCHI_CHECK(p && op && d_x0 && d_x1 && d_r0 && d_r1 && d_stats_all && out, GCHI_ERR_INVALID,
                "pr_exec_create: null argument");
     GCHI_CHECK(nsweeps >= 0, GCHI_ERR_INVALID, "pr_exec_create: nsweeps < 0");
@@ -584,9 +820,16 @@ extern "C" int gchi_pr_exec_create(const gchi_pr_plan *p, const gchi_pr_op *op, 
     if (st == GCHI_OK) {
         float *xs[2] = {d_x0, d_x1};
         float *rs[2] = {d_r0, d_r1};
-        int s1 = pr_init(p, init, xs[0], rs[0], cs);
+        int s1 = GCHI_OK;
+        if (do_init) {
+            if (op->d_seed_bits)
+                s1 = pr_init(p, 0.0f, op->d_seed_bits, (float)op->seed_teleport, xs[0], rs[0], cs);
+            else
+                s1 = pr_init(p, init, nullptr, 0.0f, xs[0], rs[0], cs);
+        }
+        const size_t stride = 2 * (size_t)p->G;
         for (int k = 0; k < nsweeps && s1 == GCHI_OK; k++)
-            s1 = pr_sweep(p, op, xs[k & 1], rs[k & 1], xs[(k + 1) & 1], rs[(k + 1) & 1], d_stats_all + 2 * k, cs);
+            s1 = pr_sweep(p, op, xs[k & 1], rs[k & 1], xs[(k + 1) & 1], rs[(k + 1) & 1], d_stats_all + stride * k, cs);
         e = hipStreamEndCapture(cs, &ex->graph);
         if (s1 != GCHI_OK) st = s1;
         else if (e != hipSuccess) st = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
@@ -603,6 +846,11 @@ extern "C" int gchi_pr_exec_create(const gchi_pr_plan *p, const gchi_pr_op *op, 
     }
     *out = ex;
     return GCHI_OK;
+}
+
+extern "C" int gchi_pr_exec_create(const gchi_pr_plan *p, const gchi_pr_op *op, float init, int nsweeps, float *d_x0,
+                                   float *d_x1, float *d_r0, float *d_r1, double *d_stats_all, gchi_pr_exec **out) {
+    return gchi_pr_exec_create_ex(p, op, init, 1, nsweeps, d_x0, d_x1, d_r0, d_r1, d_stats_all, out);
 }
 
 extern "C" int gchi_pr_exec_launch(gchi_pr_exec *ex, void *stream) {

@@ -33,18 +33,30 @@ class DGraph(ctypes.Structure):
 
 
 class PrOp(ctypes.Structure):
-    _fields_ = [("teleport", ctypes.c_double), ("damping", ctypes.c_double)]
+    _fields_ = [("teleport", ctypes.c_double), ("damping", ctypes.c_double), ("d_graph_teleport", c_vp),
+                ("d_seed_bits", c_vp), ("seed_teleport", ctypes.c_double)]
 
 
 class PrTune(ctypes.Structure):
     _fields_ = [("edges_per_block", c_i32), ("short_row_max", c_i32),
-                ("reserved0", c_i32), ("reserved1", c_i32)]
+                ("persistent", c_i32), ("reserved", c_i32)]
 
 
 class PrPlanInfo(ctypes.Structure):
     _fields_ = [("nvertices", c_u32), ("edges_per_block", c_u32), ("nedges", c_u64),
                 ("short_row_max", c_u32), ("regular_blocks", c_u32), ("hub_rows", c_u32),
-                ("hub_chunks", c_u32), ("regular_edges", c_u64), ("hub_edges", c_u64)]
+                ("hub_chunks", c_u32), ("regular_edges", c_u64), ("hub_edges", c_u64),
+                ("ngraphs", c_u32), ("persistent", c_u32), ("persistent_workgroups", c_u32), ("reserved", c_u32)]
+
+
+class PprConfig(ctypes.Structure):
+    _fields_ = [("damping", ctypes.c_double), ("tol", ctypes.c_double), ("bf16_switch_residual", ctypes.c_double),
+                ("bf16_sweeps", c_i32), ("max_sweeps", c_i32), ("check_every", c_i32), ("reserved", c_i32)]
+
+
+class PprResult(ctypes.Structure):
+    _fields_ = [("sweeps_bf16", c_i32), ("sweeps_fp32", c_i32), ("converged", c_i32), ("reserved", c_i32),
+                ("max_residual", ctypes.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -72,8 +84,18 @@ _SIGS = {
     "gchi_pr_sweep": (c_i32, [c_vp, ctypes.POINTER(PrOp), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "gchi_pr_exec_create": (c_i32, [c_vp, ctypes.POINTER(PrOp), ctypes.c_float, c_i32, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, ctypes.POINTER(c_vp)]),
+    "gchi_pr_exec_create_ex": (c_i32, [c_vp, ctypes.POINTER(PrOp), ctypes.c_float, c_i32, c_i32, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    "gchi_pr_plan_create_multi": (c_i32, [ctypes.POINTER(DGraph), c_vp, c_u32, ctypes.POINTER(PrTune),
+                                          ctypes.POINTER(c_vp), c_vp]),
+    "gchi_pr_init_seeds": (c_i32, [c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp]),
     "gchi_pr_exec_launch": (c_i32, [c_vp, c_vp]),
     "gchi_pr_exec_destroy": (c_i32, [c_vp]),
+    "gchi_ppr_batch_create": (c_i32, [c_vp, c_u32, ctypes.POINTER(c_vp)]),
+    "gchi_ppr_batch_destroy": (c_i32, [c_vp]),
+    "gchi_ppr_batch_run": (c_i32, [c_vp, c_vp, c_vp, ctypes.POINTER(PprConfig), ctypes.POINTER(PprResult), c_vp,
+                                   c_vp]),
+    "gchi_ppr_batch_extract": (c_i32, [c_vp, c_u32, c_vp, c_vp]),
     "gchi_cc_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
     "gchi_cc_minlabel": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_sz, c_vp]),
     "gchi_cc_labelprop_init": (c_i32, [c_vp, c_u32, c_vp]),
@@ -252,13 +274,38 @@ class DeviceGraph:
                          self.indeg.data_ptr())
 
 
+def pr_op(teleport, damping=0.85, graph_teleport=None, seed_bits=None, seed_teleport=0.0):
+    """gchi_pr_op; graph_teleport / seed_bits are device tensors (kept alive by the caller)."""
+    op = PrOp(teleport, damping, graph_teleport.data_ptr() if graph_teleport is not None else None,
+              seed_bits.data_ptr() if seed_bits is not None else None, seed_teleport)
+    op._keep = (graph_teleport, seed_bits)
+    return op
+
+
+def seed_bitmap(seeds, n):
+    """Device bitmap (uint32 words) with the seed vertices set."""
+    words = np.zeros((n + 31) // 32, np.uint32)
+    seeds = np.asarray(seeds, dtype=np.uint64)
+    np.bitwise_or.at(words, (seeds >> np.uint64(5)).astype(np.int64),
+                     (np.uint32(1) << (seeds & np.uint64(31)).astype(np.uint32)))
+    return to_dev_u32(words)
+
+
 class PageRankPlan:
-    def __init__(self, graph, edges_per_block=0, short_row_max=0):
+    def __init__(self, graph, edges_per_block=0, short_row_max=0, persistent=0, vbase=None):
         self.graph = graph
         self.handle = c_vp()
-        tune = PrTune(edges_per_block, short_row_max, 0, 0)
-        check(lib().gchi_pr_plan_create(ctypes.byref(graph.dg), ctypes.byref(tune), ctypes.byref(self.handle),
-                                        stream()), "pr_plan_create")
+        tune = PrTune(edges_per_block, short_row_max, persistent, 0)
+        if vbase is None:
+            self.ngraphs = 1
+            check(lib().gchi_pr_plan_create(ctypes.byref(graph.dg), ctypes.byref(tune), ctypes.byref(self.handle),
+                                            stream()), "pr_plan_create")
+        else:
+            vb = np.ascontiguousarray(vbase, dtype=np.uint32)
+            self.ngraphs = len(vb) - 1
+            check(lib().gchi_pr_plan_create_multi(ctypes.byref(graph.dg), c_vp(vb.ctypes.data), self.ngraphs,
+                                                  ctypes.byref(tune), ctypes.byref(self.handle), stream()),
+                  "pr_plan_create_multi")
 
     def info(self):
         inf = PrPlanInfo()
@@ -279,6 +326,10 @@ class PageRankPlan:
     def init(self, init, x, r):
         check(lib().gchi_pr_init(self.handle, init, dptr(x), dptr(r), stream()), "pr_init")
 
+    def init_seeds(self, seed_bits, seed_value, x, r):
+        check(lib().gchi_pr_init_seeds(self.handle, dptr(seed_bits), seed_value, dptr(x), dptr(r), stream()),
+              "pr_init_seeds")
+
     def sweep(self, op, x_in, r_in, x_out, r_out, stats):
         check(lib().gchi_pr_sweep(self.handle, ctypes.byref(op), dptr(x_in), dptr(r_in), dptr(x_out), dptr(r_out),
                                   dptr(stats), stream()), "pr_sweep")
@@ -287,18 +338,18 @@ class PageRankPlan:
 class PageRankExec:
     """init + nsweeps sweeps captured once into a HIP graph."""
 
-    def __init__(self, plan, op, init, nsweeps):
+    def __init__(self, plan, op, init, nsweeps, do_init=True, x=None, r=None):
         torch = _torch()
         N = plan.graph.N
         self.plan, self.nsweeps = plan, nsweeps
-        self.x = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
-        self.r = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
-        self.stats = torch.zeros(max(2 * nsweeps, 2), dtype=torch.float64, device="cuda")
+        self.x = x if x is not None else [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.r = r if r is not None else [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.stats = torch.zeros(max(2 * plan.ngraphs * nsweeps, 2), dtype=torch.float64, device="cuda")
         self.op = op
         self.handle = c_vp()
-        check(lib().gchi_pr_exec_create(plan.handle, ctypes.byref(op), init, nsweeps, dptr(self.x[0]),
-                                        dptr(self.x[1]), dptr(self.r[0]), dptr(self.r[1]), dptr(self.stats),
-                                        ctypes.byref(self.handle)), "pr_exec_create")
+        check(lib().gchi_pr_exec_create_ex(plan.handle, ctypes.byref(op), init, 1 if do_init else 0, nsweeps,
+                                           dptr(self.x[0]), dptr(self.x[1]), dptr(self.r[0]), dptr(self.r[1]),
+                                           dptr(self.stats), ctypes.byref(self.handle)), "pr_exec_create")
 
     def launch(self):
         check(lib().gchi_pr_exec_launch(self.handle, stream()), "pr_exec_launch")
@@ -438,3 +489,92 @@ class Shards:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- personalized PageRank
+def seed_csr(queries):
+    off = np.zeros(len(queries) + 1, np.uint32)
+    for i, q in enumerate(queries):
+        off[i + 1] = off[i] + len(q)
+    flat = np.concatenate([np.asarray(q, np.uint32) for q in queries]) if queries else np.zeros(0, np.uint32)
+    return off, np.ascontiguousarray(flat, dtype=np.uint32)
+
+
+class PprBatch:
+    """Batched personalized PageRank over a PageRankPlan (64 queries per block)."""
+
+    def __init__(self, plan, nqueries):
+        self.plan, self.nq = plan, nqueries
+        self.handle = c_vp()
+        check(lib().gchi_ppr_batch_create(plan.handle, nqueries, ctypes.byref(self.handle)), "ppr_batch_create")
+
+    def run(self, queries, damping=0.85, tol=1e-6, bf16_sweeps=60, bf16_switch_residual=2e-3, max_sweeps=1000,
+            check_every=4):
+        assert len(queries) == self.nq
+        off, flat = seed_csr(queries)
+        cfg = PprConfig(damping, tol, bf16_switch_residual, bf16_sweeps, max_sweeps, check_every, 0)
+        res = PprResult()
+        resid = np.zeros(self.nq, np.float64)
+        check(lib().gchi_ppr_batch_run(self.handle, c_vp(off.ctypes.data), c_vp(flat.ctypes.data), ctypes.byref(cfg),
+                                       ctypes.byref(res), c_vp(resid.ctypes.data), stream()), "ppr_batch_run")
+        return {k: getattr(res, k) for k, _ in PprResult._fields_}, resid
+
+    def ranks(self, q):
+        torch = _torch()
+        out = torch.empty(self.plan.graph.N, dtype=torch.float32, device="cuda")
+        check(lib().gchi_ppr_batch_extract(self.handle, q, dptr(out), stream()), "ppr_batch_extract")
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().gchi_ppr_batch_destroy(self.handle)
+            self.handle = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PlainPpr:
+    """One query at a time with the FP32 PageRank sweep (seed bitmap apply):
+    sweeps launched one per launch, residual read every check_every sweeps."""
+
+    def __init__(self, plan, damping=0.85):
+        torch = _torch()
+        self.plan, self.d = plan, damping
+        N = plan.graph.N
+        self.x = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.r = [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
+        self.stats = torch.zeros(2, dtype=torch.float64, device="cuda")
+
+    def run(self, seeds, tol=1e-6, max_sweeps=1000, check_every=4):
+        torch = _torch()
+        n = self.plan.graph.N
+        bits = seed_bitmap(seeds, n)
+        mass = (1.0 - self.d) / len(seeds)
+        op = pr_op(0.0, self.d, seed_bits=bits, seed_teleport=mass)
+        self.plan.init_seeds(bits, mass, self.x[0], self.r[0])
+        stop = tol * (1.0 - self.d) / self.d
+        k = 0
+        while k < max_sweeps:
+            self.plan.sweep(op, self.x[k & 1], self.r[k & 1], self.x[(k + 1) & 1], self.r[(k + 1) & 1], self.stats)
+            k += 1
+            if k % check_every == 0 and float(self.stats[0].item()) <= stop:
+                break
+        torch.cuda.synchronize()
+        return self.r[k & 1], k
+
+
+# ---------------------------------------------------------------- many graphs
+def union_of(graphs_list):
+    """Disjoint union of (src, dst, n) host graphs: concatenated edges with
+    vertex offsets, and the vertex bases [0, n0, n0+n1, ...]."""
+    vbase = np.zeros(len(graphs_list) + 1, np.uint64)
+    for i, (_, _, n) in enumerate(graphs_list):
+        vbase[i + 1] = vbase[i] + n
+    assert vbase[-1] < 2 ** 32
+    src = np.concatenate([s.astype(np.uint64) + vbase[i] for i, (s, _, _) in enumerate(graphs_list)])
+    dst = np.concatenate([d.astype(np.uint64) + vbase[i] for i, (_, d, _) in enumerate(graphs_list)])
+    return src.astype(np.uint32), dst.astype(np.uint32), vbase.astype(np.uint32)

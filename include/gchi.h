@@ -118,11 +118,17 @@ int gchi_csc_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges
 /* Affine functional kernel of example_apps/pagerank_functional.cpp:48-83:
  *   value_to_neighbor : x_u = r_u / outdeg_u            (FP32, :78-81)
  *   op_neighborval/plus: s_v = sum of x_u over in-edges  (FP32, :62-69)
- *   compute_vertexvalue: r_v = (float)(teleport + damping*(double)s_v)  (:72-75)
- * teleport = 0.15/N reproduces pagerank_functional, 0.15 the pagerank.cpp form. */
+ *   compute_vertexvalue: r_v = (float)(c_v + damping*(double)s_v)  (:72-75)
+ * with c_v = teleport (0.15/N reproduces pagerank_functional, 0.15 the
+ * pagerank.cpp form), or d_graph_teleport[g] for the graph g of v in a
+ * multi-graph plan, plus seed_teleport when v is set in d_seed_bits
+ * (personalized PageRank: restart mass on a seed set). */
 typedef struct gchi_pr_op {
     double teleport;
     double damping;
+    const double *d_graph_teleport;  /* optional, [ngraphs] (device) */
+    const uint32_t *d_seed_bits;     /* optional, bitmap over N (device) */
+    double seed_teleport;
 } gchi_pr_op;
 
 /* Launch shape of the pull sweep (filled by gchi_autotune_pr or defaults). */
@@ -130,8 +136,9 @@ typedef struct gchi_pr_tune {
     int edges_per_block;   /* W: multiple of 256 in [256,4096]; 0 = 2048.  Rows with
                               in-degree >= W are hub rows, cut into 4096-edge chunks */
     int short_row_max;     /* rows up to this length use one lane per row; 0 = 32 */
-    int reserved0;
-    int reserved1;
+    int persistent;        /* 1: persistent workgroups pull tiles from a device work
+                              counter; 0: one workgroup per tile */
+    int reserved;
 } gchi_pr_tune;
 
 /* Sweep plan: the graph's in-edges regrouped into regular rows (LDS-staged
@@ -150,15 +157,29 @@ typedef struct gchi_pr_plan_info {
     uint32_t hub_chunks;
     uint64_t regular_edges;
     uint64_t hub_edges;
+    uint32_t ngraphs;
+    uint32_t persistent;
+    uint32_t persistent_workgroups;
+    uint32_t reserved;
 } gchi_pr_plan_info;
 
 int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream);
+/* Plan over a disjoint union of ngraphs graphs: graph k owns vertices
+ * [h_vbase[k], h_vbase[k+1]) of g (h_vbase[0] = 0, h_vbase[ngraphs] = N, no edge
+ * crosses graphs).  Blocks never straddle graphs; sweeps write per-graph
+ * statistics (d_stats[2k], d_stats[2k+1]).  Many independent PageRank problems
+ * run in one launch per sweep. */
+int gchi_pr_plan_create_multi(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t ngraphs,
+                              const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream);
 int gchi_pr_plan_destroy(gchi_pr_plan *plan);
 int gchi_pr_plan_info_get(const gchi_pr_plan *plan, gchi_pr_plan_info *info);
 
 /* Iteration 0 of the synchronous functional engine (functional_bulksync.hpp:84-87,
  * 107-113): r = init, x = r/outdeg (0 where outdeg == 0). */
 int gchi_pr_init(const gchi_pr_plan *plan, float init, float *d_x, float *d_r, void *stream);
+/* Personalized start: r = seed_value on seed vertices, 0 elsewhere. */
+int gchi_pr_init_seeds(const gchi_pr_plan *plan, const uint32_t *d_seed_bits, float seed_value, float *d_x,
+                       float *d_r, void *stream);
 
 /* One synchronous gather->apply->scatter sweep (functional_bulksync.hpp:91-113,
  * functional_engine.hpp:50-121): two kernels (sweep, finalize).
@@ -171,13 +192,56 @@ int gchi_pr_sweep(const gchi_pr_plan *plan, const gchi_pr_op *op, const float *d
 /* init + nsweeps sweeps captured into one HIP graph (instantiated once,
  * replayed by gchi_pr_exec_launch).  Sweep k reads buffers [k&1] and writes
  * [(k+1)&1]; the final ranks are in d_r{nsweeps&1}.  d_stats_all receives
- * 2*nsweeps doubles.  Replaces the iteration loop of graphchi_engine::run
+ * 2*ngraphs*nsweeps doubles.  Replaces the iteration loop of graphchi_engine::run
  * (src/engine/graphchi_engine.hpp:779-956) for functional kernels. */
 typedef struct gchi_pr_exec gchi_pr_exec;
 int gchi_pr_exec_create(const gchi_pr_plan *plan, const gchi_pr_op *op, float init, int nsweeps, float *d_x0,
                         float *d_x1, float *d_r0, float *d_r1, double *d_stats_all, gchi_pr_exec **out);
+/* As gchi_pr_exec_create; do_init = 0 captures sweeps only (the caller's
+ * buffers carry the state), so replays continue an iteration. */
+int gchi_pr_exec_create_ex(const gchi_pr_plan *plan, const gchi_pr_op *op, float init, int do_init, int nsweeps,
+                           float *d_x0, float *d_x1, float *d_r0, float *d_r1, double *d_stats_all,
+                           gchi_pr_exec **out);
 int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
 int gchi_pr_exec_destroy(gchi_pr_exec *exec);
+
+/* ------------------------------------------------- batched personalized PageRank */
+/* Many personalized-PageRank queries (restart sets of 1..64 seeds; kernel
+ * r_v = (v in S ? (1-d)/|S| : 0) + d * sum r_u/outdeg_u, from r = restart vector)
+ * swept together: 64 queries per block, ranks stored vertex-major (64 per
+ * vertex), one wavefront per destination row with one lane per query, tiles
+ * (query block, row block / hub chunk) pulled from a device work queue by a
+ * persistent kernel.  Phase 1 stores ranks in BF16 (FP32 accumulation) for up to
+ * bf16_sweeps sweeps or until every query's L1 residual <= bf16_switch_residual
+ * or stops falling; phase 2 refines in FP32 until every query satisfies
+ * residual * d/(1-d) <= tol (a bound on the L1 distance to the fixed point,
+ * iterates being monotone).  Residuals are FP64. */
+typedef struct gchi_ppr_config {
+    double damping;
+    double tol;
+    double bf16_switch_residual;
+    int bf16_sweeps;     /* 0: FP32 only */
+    int max_sweeps;
+    int check_every;     /* residual checked every k sweeps (default 4) */
+    int reserved;
+} gchi_ppr_config;
+typedef struct gchi_ppr_result {
+    int sweeps_bf16;
+    int sweeps_fp32;
+    int converged;
+    int reserved;
+    double max_residual;
+} gchi_ppr_result;
+typedef struct gchi_ppr_batch gchi_ppr_batch;
+int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries, gchi_ppr_batch **out);
+int gchi_ppr_batch_destroy(gchi_ppr_batch *batch);
+/* h_seed_off[nqueries+1], h_seeds: restart set of query q is
+ * h_seeds[h_seed_off[q] .. h_seed_off[q+1]) (distinct ids).  h_residuals
+ * (optional) receives the last L1 residual of every query. */
+int gchi_ppr_batch_run(gchi_ppr_batch *batch, const uint32_t *h_seed_off, const uint32_t *h_seeds,
+                       const gchi_ppr_config *cfg, gchi_ppr_result *result, double *h_residuals, void *stream);
+/* Copies query q's rank vector (N floats) to d_out. */
+int gchi_ppr_batch_extract(const gchi_ppr_batch *batch, uint32_t query, float *d_out, void *stream);
 
 /* --------------------------------------------------------- connected components */
 /* Labels = smallest vertex id of each weakly connected component: the fixed

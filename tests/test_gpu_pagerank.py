@@ -22,7 +22,7 @@ TUNES = [(0, 0), (256, 4), (1024, 8)]
 def run_pr(dev, src, dst, n, teleport, sweeps, tune=(0, 0), init=1.0):
     g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)
     plan = dev.PageRankPlan(g, *tune)
-    ex = dev.PageRankExec(plan, dev.PrOp(teleport, DAMPING), init, sweeps)
+    ex = dev.PageRankExec(plan, dev.pr_op(teleport, DAMPING), init, sweeps)
     ex.launch()
     r = ex.ranks().cpu().numpy().copy()
     stats = ex.stats.cpu().numpy()[:2 * sweeps].reshape(-1, 2).copy()
@@ -76,7 +76,7 @@ def test_pagerank_hub_path_and_reproducible(dev):
     st = torch.zeros(2, dtype=torch.float64, device="cuda")
     plan.init(1.0, x[0], r[0])
     for k in range(9):
-        plan.sweep(dev.PrOp(0.15, DAMPING), x[k & 1], r[k & 1], x[(k + 1) & 1], r[(k + 1) & 1], st)
+        plan.sweep(dev.pr_op(0.15, DAMPING), x[k & 1], r[k & 1], x[(k + 1) & 1], r[(k + 1) & 1], st)
     np.testing.assert_array_equal(r[1].cpu().numpy(), r1)
 
 
@@ -102,7 +102,7 @@ def test_pagerank_scale22_vs_fp64(dev):
     hs, hd = dev.to_host_u32(s, cnt), dev.to_host_u32(d, cnt)
     g = dev.DeviceGraph(s, d, n)
     plan = dev.PageRankPlan(g)
-    ex = dev.PageRankExec(plan, dev.PrOp(0.15, DAMPING), 1.0, 20)
+    ex = dev.PageRankExec(plan, dev.pr_op(0.15, DAMPING), 1.0, 20)
     ex.launch()
     r = ex.ranks().cpu().numpy().astype(np.float64)
     t = time.time()
