@@ -1,13 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the GraphChi hot path on one MI355X.
 
-Workload (one JSON line): synchronous FP32 PageRank, r = 0.15 + 0.85 * sum(x),
-x = r / outdeg, on a directed R-MAT graph of scale 22 (edge factor 16,
-a/b/c/d = 0.57/0.19/0.19/0.05, seed 2, self-loops dropped, duplicates kept),
-resident in HBM.  A step is one sweep (gather -> apply -> scatter over every
-edge); `value` = edges per second per sweep over the timed sweeps.
+Primary workload (the JSON line's `value`): synchronous FP32 PageRank,
+r = 0.15 + 0.85 * sum(r_u / outdeg_u), on a directed R-MAT graph of scale 22
+(edge factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, seed 2, self-loops dropped,
+duplicates kept) resident in HBM.  A step is one sweep (gather -> apply ->
+scatter over every edge); value = edges per second per sweep.
 
-  python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--cpu-sweeps 2]
+The same line carries the other rows of the hot path under "extras":
+batched personalized PageRank (1,024 queries, BF16 -> FP32) against plain FP32
+one-query-at-a-time PPR, both checked against the FP64 oracle; the (dst,src)
+radix sort of the 67M pairs; the device sharder; connected components on an
+undirected R-MAT scale 24; triangle counting on R-MAT scale 21; many graphs of
+uneven size in one launch per sweep.
+
+  python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--only pagerank]
 """
 import argparse
 import json
@@ -22,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "graphchi-cpp_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "PageRank edges/s/iter & % HBM peak; batched PPR queries/s at L1<=1e-6 vs FP64"
+ABCD = (0.57, 0.19, 0.19, 0.05)
 
 
 def log(msg):
@@ -38,48 +46,36 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(hs, hd, n, sweeps, nshards):
+def timed(fn, torch):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t
+
+
+def oracle_mod():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, os.cpu_count() or 1)
-    t = time.time()
-    _, secs = oracle.pr_psw_omp(n, hs, hd, nshards, 0.15, 0.85, 1.0, sweeps, threads)
-    log(f"cpu baseline: {sweeps} sweeps in {secs:.2f}s (+{time.time() - t - secs:.1f}s preprocessing)")
-    return {"value": len(hs) * sweeps / secs, "unit": "edges/s/iter", "cores": threads, "kind": "port",
-            "sample": f"same scale-22 graph, {sweeps} sweeps of the oracle's OpenMP restatement of the "
-                      f"reference PSW sweep (shards preloaded, {nshards} dst-interval shards)"}
+    return oracle
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--cpu-sweeps", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--edges-per-block", type=int, default=0)
-    ap.add_argument("--short-row-max", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    import gchi
-    torch.cuda.set_device(0)
-    gchi.open_device(0)
-    scale, ef, seed = args.scale, 16, 2
-    log(f"generating R-MAT scale {scale} on device")
-    src, dst, E, maxid = gchi.rmat_device(scale, ef << scale, seed)
-    N = maxid + 1
+# ------------------------------------------------------------------ PageRank
+def bench_pagerank(gchi, torch, args, src, dst, E, N):
     g = gchi.DeviceGraph(src, dst, N)
-    plan = gchi.PageRankPlan(g, args.edges_per_block, args.short_row_max)
+    tune = {"edges_per_block": args.edges_per_block, "short_row_max": args.short_row_max,
+            "persistent": args.persistent}
+    tuned = None
+    if args.autotune and not args.edges_per_block:
+        best, best_ms, tried = gchi.autotune_pr(g, sweeps=4, report=True)
+        tune = best
+        tuned = {"best": best, "best_ms": round(best_ms, 4), "candidates": len(tried),
+                 "worst_ms": round(max(t[1] for t in tried), 4)}
+        log(f"autotune: best {best} {best_ms:.4f} ms/sweep over {len(tried)} candidates")
+    plan = gchi.PageRankPlan(g, **tune)
     info = plan.info()
     op = gchi.pr_op(0.15, 0.85)
     log(f"graph N={N} E={E} plan={info}")
-
     if args.warmup > 0:
         w = gchi.PageRankExec(plan, op, 1.0, args.warmup)
         w.launch()
@@ -99,12 +95,241 @@ def main():
     stats = ex.stats.cpu().numpy()[:2 * args.steps].reshape(-1, 2)
     log(f"timed {args.steps} sweeps: {ms:.4f} ms/sweep (wall {wall * 1e3:.2f} ms); last residual "
         f"{stats[-1, 0]:.3e}, rank sum {stats[-1, 1]:.6e}")
+    return g, plan, info, ms, tuned, stats
+
+
+# ------------------------------------------------------------------ PPR
+def bench_ppr(gchi, torch, args, g, plan, N):
+    oracle = oracle_mod()
+    rng = np.random.default_rng(5)
+    queries = [rng.choice(N, size=int(rng.integers(1, 65)), replace=False).astype(np.uint32)
+               for _ in range(args.ppr_queries)]
+    b = gchi.PprBatch(plan, len(queries))
+    b.run(queries[:1] * len(queries), max_sweeps=8)  # warm-up (seed tables, caches)
+    (info, resid), t_batch = timed(lambda: b.run(queries), torch)
+    log(f"ppr batched: {len(queries)} queries in {t_batch:.3f}s, sweeps bf16={info['sweeps_bf16']} "
+        f"fp32={info['sweeps_fp32']} max residual {info['max_residual']:.2e}")
+    plain = gchi.PlainPpr(plan)
+    plain.run(queries[0], max_sweeps=8)
+    nplain = min(args.ppr_plain_queries, len(queries))
+    sweeps_plain = []
+
+    def run_plain():
+        for q in queries[:nplain]:
+            _, k = plain.run(q)
+            sweeps_plain.append(k)
+
+    _, t_plain = timed(run_plain, torch)
+    log(f"ppr plain: {nplain} queries in {t_plain:.3f}s (sweeps {min(sweeps_plain)}..{max(sweeps_plain)})")
+    # accuracy vs the FP64 oracle on a sample of queries (CPU)
+    hs, hd = gchi.to_host_u32(g.in_src, g.E), None
+    in_off = gchi.to_host_u32(g.in_off, N + 1)
+    outdeg = gchi.to_host_u32(g.outdeg, N)
+    errs_b, errs_p = [], []
+    for qi in range(min(args.ppr_check, len(queries))):
+        q = len(queries) - 1 - qi if qi % 2 else qi
+        r64, _ = oracle.ppr_f64(N, in_off, hs, outdeg, queries[q], 0.85, tol=1e-13)
+        rb = b.ranks(q).cpu().numpy().astype(np.float64)
+        rp, _ = plain.run(queries[q])
+        rp = rp.cpu().numpy().astype(np.float64)
+        errs_b.append((float(np.abs(rb - r64).sum()), float(np.abs(rb - r64).max())))
+        errs_p.append((float(np.abs(rp - r64).sum()), float(np.abs(rp - r64).max())))
+    b.close()
+    qps_b, qps_p = len(queries) / t_batch, nplain / t_plain
+    return {"workload": f"{len(queries)} PPR queries, restart sets of 1-64 uniform seeds, damping 0.85, "
+                        "scale-22 graph; tol L1<=1e-6 (residual bound)",
+            "batched_queries_per_s": round(qps_b, 2), "plain_fp32_queries_per_s": round(qps_p, 2),
+            "speedup": round(qps_b / qps_p, 2), "batched_sweeps_bf16": info["sweeps_bf16"],
+            "batched_sweeps_fp32": info["sweeps_fp32"], "plain_sweeps_mean": float(np.mean(sweeps_plain)),
+            "plain_queries_timed": nplain,
+            "l1_vs_fp64_batched_max": max(e[0] for e in errs_b) if errs_b else None,
+            "maxabs_vs_fp64_batched_max": max(e[1] for e in errs_b) if errs_b else None,
+            "l1_vs_fp64_plain_max": max(e[0] for e in errs_p) if errs_p else None,
+            "maxabs_vs_fp64_plain_max": max(e[1] for e in errs_p) if errs_p else None,
+            "queries_checked_vs_fp64": len(errs_b)}
+
+
+# ------------------------------------------------------------------ sort / sharder
+def bench_sort_and_shards(gchi, torch, src, dst, E, maxid):
+    idb = max(1, int(maxid).bit_length())
+    gchi.sort_pairs_dst_src(src, dst, idb)  # warm-up
+    reps = 5
+    _, t = timed(lambda: [gchi.sort_pairs_dst_src(src, dst, idb) for _ in range(reps)], torch)
+    t /= reps
+    passes = -(-2 * idb // 8)
+    algo = 16 * E * passes
+    nsh = gchi.shard_auto_count(E, 8)
+    sh, t_sh = timed(lambda: gchi.Shards(src, dst, maxid, nsh), torch)
+    out = {"sort_workload": f"{E} (src,dst) uint32 pairs by (dst,src), {2 * idb}-bit keys, {passes} onesweep passes",
+           "sort_pairs_per_s": round(E / t, 1), "sort_ms": round(t * 1e3, 3),
+           "sort_algorithmic_GBps": round(algo / t / 1e9, 1), "sort_hbm_frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4),
+           "shard_build_ms": round(t_sh * 1e3, 2), "shards": int(sh.nshards)}
+    sh.close()
+    return out
+
+
+# ------------------------------------------------------------------ CC
+def bench_cc(gchi, torch, scale):
+    s, d, cnt, mx = gchi.rmat_device(scale, 8 << scale, 3)
+    s2 = torch.cat([s, d])
+    d2 = torch.cat([d, s])
+    del s, d
+    n = mx + 1
+    e = int(s2.numel())
+    gchi.cc_minlabel(s2, d2, n)
+    lab, t = timed(lambda: gchi.cc_minlabel(s2, d2, n), torch)
+    ncomp = int(torch.unique(lab[:n]).numel())
+    _, t_lp = timed(lambda: gchi.cc_labelprop(s2, d2, n), torch)
+    _, sweeps = gchi.cc_labelprop(s2, d2, n)
+    del s2, d2
+    return {"workload": f"undirected R-MAT scale {scale}, {e} directed edges (both directions), seed 3",
+            "time_to_fixed_point_ms": round(t * 1e3, 3), "edges_per_s": round(e / t, 1), "components": ncomp,
+            "labelprop_sweeps": sweeps, "labelprop_time_ms": round(t_lp * 1e3, 2),
+            "labelprop_edges_sweeps_per_s": round(e * sweeps / t_lp, 1)}
+
+
+# ------------------------------------------------------------------ TC
+def bench_tc(gchi, torch, scale):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    hs, hd = gchi.rmat_host(scale, 16 << scale, 4)
+    keep = hs != hd
+    a, b = np.minimum(hs[keep], hd[keep]), np.maximum(hs[keep], hd[keep])
+    key = np.unique((a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64))
+    s = gchi.to_dev_u32((key >> np.uint64(32)).astype(np.uint32))
+    d = gchi.to_dev_u32((key & np.uint64(0xffffffff)).astype(np.uint32))
+    n = int(max(int(key.max() >> np.uint64(32)), int((key & np.uint64(0xffffffff)).max()))) + 1
+    prep, t_prep = timed(lambda: gchi.TrianglePrep(s, d, n), torch)
+    prep.count()
+    (pv, tot, ovf), t = timed(prep.count, torch)
+    off = gchi.to_host_u32(prep.off, n + 1).astype(np.int64)
+    adj = gchi.to_host_u32(prep.adj, prep.noriented).astype(np.int64)
+    deg = np.diff(off)
+    src_of = np.repeat(np.arange(n), deg)
+    algo = int((4 * (deg[src_of] + deg[adj])).sum() + 4 * prep.noriented)
+    total = int(tot.item())
+    return {"workload": f"simple undirected R-MAT scale {scale}, {len(key)} edges, seed 4",
+            "triangles": total, "count_ms": round(t * 1e3, 3), "prepare_ms": round(t_prep * 1e3, 2),
+            "triangles_per_s": round(total / t, 1), "oriented_edges_per_s": round(prep.noriented / t, 1),
+            "algorithmic_GBps": round(algo / t / 1e9, 1), "overflow": int(ovf.item())}
+
+
+# ------------------------------------------------------------------ many graphs
+def bench_multigraph(gchi, torch, ngraphs, sweeps, seed=9):
+    """Many graphs of uneven size: each graph's CSC is built on its own, the CSCs
+    are concatenated into disjoint unions (a union holds < 2^32 edges), and
+    every union is swept by one persistent launch per sweep."""
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 21, ngraphs)
+    efs = rng.integers(8, 33, ngraphs)
+    unions, cur = [], {"off": [], "src": [], "od": [], "ind": [], "ns": [], "E": 0, "N": 0}
+
+    def close_union():
+        if not cur["ns"]:
+            return
+        offs = torch.cat(cur["off"] + [torch.tensor([cur["E"]], dtype=torch.int64, device="cuda")]).to(torch.int32)
+        g = gchi.graph_from_csc(offs, torch.cat(cur["src"]), torch.cat(cur["od"]), torch.cat(cur["ind"]),
+                                cur["N"], cur["E"])
+        vbase = np.concatenate([[0], np.cumsum(cur["ns"])]).astype(np.uint32)
+        unions.append((g, vbase, list(cur["ns"])))
+        for k in ("off", "src", "od", "ind", "ns"):
+            cur[k] = []
+        cur["E"] = cur["N"] = 0
+
+    for i in range(ngraphs):
+        s, d, cnt, mx = gchi.rmat_device(int(scales[i]), int(efs[i]) << int(scales[i]), 10000 + i)
+        gi = gchi.DeviceGraph(s, d, mx + 1)
+        if cur["E"] + cnt >= (1 << 32) - 1:
+            close_union()
+        cur["off"].append(gi.in_off[:mx + 1].to(torch.int64) + cur["E"])
+        cur["src"].append(gi.in_src[:cnt] + cur["N"])
+        cur["od"].append(gi.outdeg[:mx + 1])
+        cur["ind"].append(gi.indeg[:mx + 1])
+        cur["ns"].append(mx + 1)
+        cur["E"] += cnt
+        cur["N"] += mx + 1
+        del s, d, gi
+    close_union()
+    execs, E_total = [], 0
+    for g, vbase, ns in unions:
+        tele = torch.tensor([0.15 / n for n in ns], dtype=torch.float64, device="cuda")
+        op = gchi.pr_op(0.15, 0.85, graph_teleport=tele)
+        plan = gchi.PageRankPlan(g, persistent=1, vbase=vbase)
+        ex = gchi.PageRankExec(plan, op, 1.0, sweeps)
+        execs.append((ex, plan, op, tele, g))
+        E_total += g.E
+    for ex in execs:
+        ex[0].launch()
+    _, t = timed(lambda: [ex[0].launch() for ex in execs], torch)
+    return {"workload": f"{ngraphs} directed R-MAT graphs, scales uniform 10-20, edge factor uniform 8-32, "
+                        f"{E_total} edges total in {len(unions)} union(s); kernel 0.15/N_g, {sweeps} sweeps",
+            "launches_per_sweep": 2 * len(unions),
+            "ms_per_sweep_all_graphs": round(t * 1e3 / sweeps, 3), "edges_per_s": round(E_total * sweeps / t, 1),
+            "graphs_per_s": round(ngraphs * sweeps / t, 1)}
+
+
+def cpu_baseline(hs, hd, n, sweeps, nshards):
+    oracle = oracle_mod()
+    threads = min(16, os.cpu_count() or 1)
+    t = time.time()
+    _, secs = oracle.pr_psw_omp(n, hs, hd, nshards, 0.15, 0.85, 1.0, sweeps, threads)
+    log(f"cpu baseline: {sweeps} sweeps in {secs:.2f}s (+{time.time() - t - secs:.1f}s preprocessing)")
+    return {"value": len(hs) * sweeps / secs, "unit": "edges/s/iter", "cores": threads, "kind": "port",
+            "sample": f"same scale-22 graph, {sweeps} sweeps of the oracle's OpenMP restatement of the "
+                      f"reference PSW sweep (shards preloaded, {nshards} dst-interval shards)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--edges-per-block", type=int, default=0)
+    ap.add_argument("--short-row-max", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--only", default="", help="comma list of sections: pagerank,ppr,sort,cc,tc,multigraph")
+    ap.add_argument("--ppr-queries", type=int, default=1024)
+    ap.add_argument("--ppr-plain-queries", type=int, default=64)
+    ap.add_argument("--ppr-check", type=int, default=1)
+    ap.add_argument("--cc-scale", type=int, default=24)
+    ap.add_argument("--tc-scale", type=int, default=21)
+    ap.add_argument("--graphs", type=int, default=256)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration for HBM bytes)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    only = set(x for x in args.only.split(",") if x)
+    want = (lambda k: not only or k in only)
+
+    import torch
+    import gchi
+    torch.cuda.set_device(0)
+    gchi.open_device(0)
+    if args.calibrate_copy:
+        a = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + written per launch, 3 launches")
+        del a, b
+    scale, ef, seed = args.scale, 16, 2
+    log(f"generating R-MAT scale {scale} on device")
+    src, dst, E, maxid = gchi.rmat_device(scale, ef << scale, seed)
+    N = maxid + 1
+    g, plan, info, ms, tuned, stats = bench_pagerank(gchi, torch, args, src, dst, E, N)
 
     algo_bytes = 8 * E + 20 * N + 4
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("pr_sweep_rmat22_traffic"),
-                "kernel": "pr_sweep_kernel + pr_finalize_kernel (one sweep)", "algorithmic_bytes": algo_bytes}
+                "kernel": "pr_sweep_kernel (+ pr_finalize_kernel), one sweep", "algorithmic_bytes": algo_bytes}
     out = {
         "metric": METRIC, "value": E / (ms * 1e-3), "unit": "edges/s/iter", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
@@ -113,12 +338,28 @@ def main():
                 "self-loops dropped, duplicates kept (device generator)",
         "config": {"workload": f"pagerank_sync_fp32_rmat{scale}", "scale": scale, "edge_factor": ef, "seed": seed,
                    "nvertices": N, "nedges": E, "formula": "0.15 + 0.85*sum(r/outdeg)", "parallelism": "none (1 GPU)",
-                   "edges_per_block": info["edges_per_block"], "short_row_max": info["short_row_max"]},
+                   "edges_per_block": info["edges_per_block"], "short_row_max": info["short_row_max"],
+                   "persistent": info["persistent"], "autotune": tuned},
         "roofline": roofline,
     }
-    if not args.no_cpu_baseline:
+    extras = {}
+    sections = [("ppr", lambda: bench_ppr(gchi, torch, args, g, plan, N)),
+                ("sort", lambda: bench_sort_and_shards(gchi, torch, src, dst, E, maxid)),
+                ("cc", lambda: bench_cc(gchi, torch, args.cc_scale)),
+                ("tc", lambda: bench_tc(gchi, torch, args.tc_scale)),
+                ("multigraph", lambda: bench_multigraph(gchi, torch, args.graphs, 10))]
+    for name, fn in sections:
+        if not want(name):
+            continue
+        log(f"== {name}")
+        t = time.time()
+        extras[name] = fn()
+        log(f"   {extras[name]} ({time.time() - t:.1f}s)")
+        torch.cuda.empty_cache()
+    out["extras"] = extras
+    if not args.no_cpu_baseline and want("pagerank"):
         hs, hd = gchi.to_host_u32(src, E), gchi.to_host_u32(dst, E)
-        nshards = 6  # auto rule of the reference sharder for E*8 bytes at membudget 800 MB
+        nshards = gchi.shard_auto_count(E, 8)  # the reference's auto rule, PairContainer<float> edges
         out["cpu_baseline"] = cpu_baseline(hs, hd, N, args.cpu_sweeps, nshards)
     else:
         out["cpu_baseline"] = None

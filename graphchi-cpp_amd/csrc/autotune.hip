@@ -1,0 +1,111 @@
+// Autotuner for the PageRank sweep launch shape.
+//
+// Candidates: edges per regular block W (which also sets the hub threshold),
+// the one-lane-per-row threshold, and persistent vs one-workgroup-per-tile
+// launches.  Each candidate gets its own plan on the caller's graph and is
+// timed with HIP events over a few sweeps after a warm-up sweep; the fastest
+// wins.  Results are cached per (N, E) in the process (a plan depends on the
+// graph, a tune only on its shape and degree distribution).
+#include "common.hpp"
+#include "internal.hpp"
+
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+namespace gchi {
+int pr_init(const gchi_pr_plan *p, float init, const uint32_t *seed_bits, float seed_value, float *x, float *r,
+            hipStream_t s);
+int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
+             float *r_out, double *d_stats, hipStream_t s);
+
+static std::mutex g_mu;
+static std::map<std::pair<uint32_t, uint64_t>, std::pair<gchi_pr_tune, double>> g_cache;
+}  // namespace gchi
+
+using namespace gchi;
+
+extern "C" int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, gchi_pr_tune *best, double *best_ms,
+                                gchi_pr_tune *h_tried, double *h_ms, int max_tried, int *ntried, void *stream) {
+    GCHI_CHECK(g && best, GCHI_ERR_INVALID, "autotune_pr: null argument");
+    const auto key = std::make_pair(g->nvertices, g->nedges);
+    if (!h_tried) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) {
+            *best = it->second.first;
+            if (best_ms) *best_ms = it->second.second;
+            if (ntried) *ntried = 0;
+            return GCHI_OK;
+        }
+    }
+    hipStream_t s = as_stream(stream);
+    const int reps = sweeps_per_candidate > 0 ? sweeps_per_candidate : 4;
+    std::vector<gchi_pr_tune> cand;
+    for (int W : {512, 1024, 2048, 4096})
+        for (int sm : {8, 16, 32, 64})
+            for (int pers : {0, 1}) {
+                if (2 * W / (sm + 1) > 512) continue;
+                cand.push_back(gchi_pr_tune{W, sm, pers, 0});
+            }
+    const size_t N = g->nvertices;
+    float *buf = nullptr;
+    double *st = nullptr;
+    GCHI_HIP(hipMalloc(&buf, 4 * N * sizeof(float)));
+    if (hipMalloc(&st, 2 * sizeof(double)) != hipSuccess) {
+        (void)hipFree(buf);
+        return fail(GCHI_ERR_HIP, "autotune_pr: hipMalloc failed");
+    }
+    float *x[2] = {buf, buf + N}, *r[2] = {buf + 2 * N, buf + 3 * N};
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    gchi_pr_op op{0.15, 0.85, nullptr, nullptr, 0.0};
+    double best_t = 1e300;
+    gchi_pr_tune best_c = cand[0];
+    int n = 0, status = GCHI_OK;
+    for (const auto &c : cand) {
+        gchi_pr_plan *p = nullptr;
+        int rc = gchi_pr_plan_create(g, &c, &p, stream);
+        if (rc != GCHI_OK) continue;  // e.g. too little memory for this shape: skip it
+        rc = pr_init(p, 1.0f, nullptr, 0.0f, x[0], r[0], s);
+        if (rc == GCHI_OK) rc = pr_sweep(p, &op, x[0], r[0], x[1], r[1], st, s);  // warm-up
+        float ms = 0.0f;
+        if (rc == GCHI_OK) {
+            (void)hipEventRecord(e0, s);
+            for (int k = 0; k < reps && rc == GCHI_OK; k++)
+                rc = pr_sweep(p, &op, x[(k + 1) & 1], r[(k + 1) & 1], x[k & 1], r[k & 1], st, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(GCHI_ERR_HIP, "autotune_pr: event sync failed");
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        gchi_pr_plan_destroy(p);
+        if (rc != GCHI_OK) {
+            status = rc;
+            break;
+        }
+        const double per = ms / reps;
+        if (h_tried && n < max_tried) {
+            h_tried[n] = c;
+            if (h_ms) h_ms[n] = per;
+        }
+        n++;
+        if (per < best_t) {
+            best_t = per;
+            best_c = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    (void)hipFree(st);
+    if (status != GCHI_OK) return status;
+    GCHI_CHECK(n > 0, GCHI_ERR_UNSUPPORTED, "autotune_pr: no candidate could be planned");
+    *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    if (ntried) *ntried = n;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = {best_c, best_t};
+    return GCHI_OK;
+}

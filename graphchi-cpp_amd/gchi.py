@@ -91,6 +91,8 @@ _SIGS = {
     "gchi_pr_init_seeds": (c_i32, [c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp]),
     "gchi_pr_exec_launch": (c_i32, [c_vp, c_vp]),
     "gchi_pr_exec_destroy": (c_i32, [c_vp]),
+    "gchi_autotune_pr": (c_i32, [ctypes.POINTER(DGraph), c_i32, ctypes.POINTER(PrTune), ctypes.POINTER(ctypes.c_double),
+                                 c_vp, c_vp, c_i32, ctypes.POINTER(c_i32), c_vp]),
     "gchi_ppr_batch_create": (c_i32, [c_vp, c_u32, ctypes.POINTER(c_vp)]),
     "gchi_ppr_batch_destroy": (c_i32, [c_vp]),
     "gchi_ppr_batch_run": (c_i32, [c_vp, c_vp, c_vp, ctypes.POINTER(PprConfig), ctypes.POINTER(PprResult), c_vp,
@@ -289,6 +291,15 @@ def seed_bitmap(seeds, n):
     np.bitwise_or.at(words, (seeds >> np.uint64(5)).astype(np.int64),
                      (np.uint32(1) << (seeds & np.uint64(31)).astype(np.uint32)))
     return to_dev_u32(words)
+
+
+def graph_from_csc(in_off, in_src, outdeg, indeg, n, e):
+    """DeviceGraph over existing device CSC tensors (no build)."""
+    g = DeviceGraph.__new__(DeviceGraph)
+    g.N, g.E = int(n), int(e)
+    g.in_off, g.in_src, g.outdeg, g.indeg = in_off, in_src, outdeg, indeg
+    g.dg = DGraph(g.N, 0, g.E, in_off.data_ptr(), in_src.data_ptr(), outdeg.data_ptr(), indeg.data_ptr())
+    return g
 
 
 class PageRankPlan:
@@ -578,3 +589,17 @@ def union_of(graphs_list):
     src = np.concatenate([s.astype(np.uint64) + vbase[i] for i, (s, _, _) in enumerate(graphs_list)])
     dst = np.concatenate([d.astype(np.uint64) + vbase[i] for i, (_, d, _) in enumerate(graphs_list)])
     return src.astype(np.uint32), dst.astype(np.uint32), vbase.astype(np.uint32)
+
+
+def autotune_pr(graph, sweeps=4, report=False):
+    """Best gchi_pr_tune for the graph -> (dict, ms per sweep, [(dict, ms)...])."""
+    best, ms, n = PrTune(), ctypes.c_double(), c_i32()
+    tried = (PrTune * 64)()
+    tms = (ctypes.c_double * 64)()
+    check(lib().gchi_autotune_pr(ctypes.byref(graph.dg), sweeps, ctypes.byref(best), ctypes.byref(ms),
+                                 ctypes.cast(tried, c_vp) if report else None,
+                                 ctypes.cast(tms, c_vp) if report else None, 64, ctypes.byref(n), stream()),
+          "autotune_pr")
+    asd = lambda t: {"edges_per_block": t.edges_per_block, "short_row_max": t.short_row_max,  # noqa: E731
+                     "persistent": t.persistent}
+    return asd(best), ms.value, [(asd(tried[i]), tms[i]) for i in range(n.value)] if report else []

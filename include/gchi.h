@@ -205,6 +205,14 @@ int gchi_pr_exec_create_ex(const gchi_pr_plan *plan, const gchi_pr_op *op, float
 int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
 int gchi_pr_exec_destroy(gchi_pr_exec *exec);
 
+/* Autotuner: times every candidate launch shape (edges_per_block in
+ * {512..4096}, short_row_max in {8..64}, persistent or not) on the caller's
+ * graph with HIP events (sweeps_per_candidate sweeps after a warm-up) and
+ * returns the fastest; cached per (N, E) for the process unless h_tried is
+ * given (then every candidate is re-timed and reported). */
+int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, gchi_pr_tune *best, double *best_ms,
+                     gchi_pr_tune *h_tried, double *h_ms, int max_tried, int *ntried, void *stream);
+
 /* ------------------------------------------------- batched personalized PageRank */
 /* Many personalized-PageRank queries (restart sets of 1..64 seeds; kernel
  * r_v = (v in S ? (1-d)/|S| : 0) + d * sum r_u/outdeg_u, from r = restart vector)
