@@ -222,3 +222,20 @@ extern "C" int gchi_memcpy(void *dst, const void *src, size_t bytes, void *strea
     GCHI_HIP(hipStreamSynchronize(s));
     return GCHI_OK;
 }
+
+extern "C" int gchi_malloc(void **d_ptr, size_t bytes) {
+    GCHI_CHECK(d_ptr, GCHI_ERR_INVALID, "malloc: null pointer");
+    *d_ptr = nullptr;
+    GCHI_HIP(hipMalloc(d_ptr, bytes ? bytes : 1));
+    return GCHI_OK;
+}
+
+extern "C" int gchi_free(void *d_ptr) {
+    if (d_ptr) GCHI_HIP(hipFree(d_ptr));
+    return GCHI_OK;
+}
+
+extern "C" int gchi_synchronize(void *stream) {
+    GCHI_HIP(hipStreamSynchronize(as_stream(stream)));
+    return GCHI_OK;
+}

@@ -314,3 +314,76 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
 }
+
+__global__ void relabel_kernel(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, const uint32_t *map,
+                               uint32_t *so, uint32_t *dout, uint32_t *err) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        uint32_t s = src[i], d = dst[i];
+        if (s >= n || d >= n) {
+            atomicOr(err, 1u);
+            so[i] = dout[i] = 0;
+            continue;
+        }
+        so[i] = map[s];
+        dout[i] = map[d];
+    }
+}
+
+extern "C" int gchi_degree_order_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes) {
+    return gchi_tc_prepare_workspace(nedges, nvertices, bytes);
+}
+
+// order_by_degree's translation table only (orderbydegree.hpp:64-100)
+extern "C" int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                                 uint32_t *d_vertexmap, void *d_work, size_t work_bytes, void *stream) {
+    GCHI_CHECK(d_vertexmap && nvertices > 0, GCHI_ERR_INVALID, "degree_order: null argument");
+    GCHI_CHECK(nedges == 0 || (d_src && d_dst), GCHI_ERR_INVALID, "degree_order: null edges");
+    GCHI_CHECK(nedges < (1ull << 31), GCHI_ERR_UNSUPPORTED, "degree_order: E must be < 2^31");
+    hipStream_t s = as_stream(stream);
+    Carver c(d_work, work_bytes);
+    uint32_t *deg, *flags, *pos, *err;
+    uint64_t *vkeys, *ekeys;
+    void *sw;
+    size_t swb;
+    tc_prep_layout(c, nedges, nvertices, &deg, &vkeys, &ekeys, &flags, &pos, &err, &sw, &swb);
+    GCHI_CHECK(c.ok(), GCHI_ERR_WORKSPACE, "degree_order: workspace too small");
+    const uint32_t N = nvertices;
+    int idb = bits_for(N - 1);
+    if (idb == 0) idb = 1;
+    const int degb = bits_for(2 * nedges);
+    launch_fill_u32(deg, 0u, N, s);
+    launch_fill_u32(err, 0u, 1, s);
+    if (nedges)
+        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(nedges, 256, 65536)), dim3(256), 0, s, d_src, d_dst, nedges, N,
+                           deg, err);
+    hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
+    GCHI_LAUNCH_CHECK();
+    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
+    hipLaunchKernelGGL(tc_vertexmap, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, vkeys, N, idb, d_vertexmap);
+    uint32_t h_err = 0;
+    GCHI_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(h_err == 0, GCHI_ERR_INVALID, "degree_order: vertex id >= nvertices");
+    return GCHI_OK;
+}
+
+extern "C" int gchi_relabel_edges(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                                  const uint32_t *d_map, uint32_t *d_src_out, uint32_t *d_dst_out, void *stream) {
+    GCHI_CHECK(d_map && (nedges == 0 || (d_src && d_dst && d_src_out && d_dst_out)), GCHI_ERR_INVALID,
+               "relabel_edges: null argument");
+    if (!nedges) return GCHI_OK;
+    hipStream_t s = as_stream(stream);
+    uint32_t *err = nullptr;
+    GCHI_HIP(hipMalloc(&err, 4));
+    launch_fill_u32(err, 0u, 1, s);
+    hipLaunchKernelGGL(relabel_kernel, dim3(grid_for(nedges, 256, 65536)), dim3(256), 0, s, d_src, d_dst, nedges,
+                       nvertices, d_map, d_src_out, d_dst_out, err);
+    uint32_t h = 0;
+    hipError_t e = hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(err);
+    if (e != hipSuccess) return hip_fail(e, "relabel_edges", __FILE__, __LINE__);
+    GCHI_CHECK(h == 0, GCHI_ERR_INVALID, "relabel_edges: vertex id >= nvertices");
+    return GCHI_OK;
+}

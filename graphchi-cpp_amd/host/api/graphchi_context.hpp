@@ -1,0 +1,38 @@
+// Per-run information handed to programs (src/api/graphchi_context.hpp:45-105)
+// and the scheduler interface (src/api/ischeduler.hpp).
+#pragma once
+#include <stddef.h>
+
+#include <vector>
+
+#include "graphchi_types.hpp"
+
+namespace graphchi {
+
+class ischeduler {
+  public:
+    virtual ~ischeduler() {}
+    virtual void add_task(vid_t, bool = false) {}
+    virtual void remove_tasks(vid_t, vid_t) {}
+    virtual void add_task_to_all() {}
+    virtual bool is_scheduled(vid_t) { return true; }
+};
+
+struct graphchi_context {
+    size_t nvertices = 0;
+    size_t nedges = 0;
+    int iteration = 0;
+    int num_iterations = 0;
+    int last_iteration = -1;
+    int execthreads = 1;
+    ischeduler *scheduler = nullptr;
+    std::vector<double> deltas = std::vector<double>(1, 0.0);
+
+    void set_last_iteration(int it) { last_iteration = it; }
+    bool isfinal() const { return last_iteration >= 0 && iteration >= last_iteration; }
+    void log_change(double delta) { deltas[0] += delta; }
+    double get_delta() const { return deltas[0]; }
+    void reset_deltas(int) { deltas.assign(1, 0.0); }
+};
+
+}  // namespace graphchi

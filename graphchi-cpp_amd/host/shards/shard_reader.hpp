@@ -1,0 +1,76 @@
+// Reads the adjacency stream of a shard back into (src, dst) pairs: the format
+// finish_shard writes (src/preprocessing/sharder.hpp:773-819) and
+// memory_shard::load_vertices decodes (src/shards/memoryshard.hpp:448-467):
+//   count byte (0xff -> uint32 count follows), then count uint32 dst ids for
+//   the current src; a 0 byte followed by k means 1 + k vertices without
+//   out-edges in this shard.
+#pragma once
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "api/chifilenames.hpp"
+#include "logger/logger.hpp"
+
+namespace graphchi {
+
+inline std::vector<uint8_t> read_file_bytes(const std::string &path) {
+    std::vector<uint8_t> buf;
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) logstream(LOG_FATAL) << "could not open " << path << std::endl;
+    fseek(f, 0, SEEK_END);
+    long n = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    buf.resize(n > 0 ? (size_t)n : 0);
+    if (n > 0 && fread(buf.data(), 1, (size_t)n, f) != (size_t)n) {
+        fclose(f);
+        logstream(LOG_FATAL) << "short read from " << path << std::endl;
+    }
+    fclose(f);
+    return buf;
+}
+
+// Appends the shard's edges; returns false on a malformed stream.
+inline bool decode_adjacency(const std::vector<uint8_t> &adj, std::vector<vid_t> &src, std::vector<vid_t> &dst) {
+    size_t p = 0, n = adj.size();
+    uint64_t vid = 0;
+    while (p < n) {
+        uint8_t ns = adj[p++];
+        if (ns == 0) {
+            if (p >= n) return false;
+            vid += 1 + adj[p++];
+            continue;
+        }
+        uint32_t cnt = ns;
+        if (ns == 0xff) {
+            if (p + 4 > n) return false;
+            memcpy(&cnt, &adj[p], 4);
+            p += 4;
+        }
+        if (p + 4ull * cnt > n) return false;
+        for (uint32_t k = 0; k < cnt; k++) {
+            uint32_t d;
+            memcpy(&d, &adj[p], 4);
+            p += 4;
+            src.push_back((vid_t)vid);
+            dst.push_back(d);
+        }
+        vid++;
+    }
+    return true;
+}
+
+// All edges of a sharded graph, shard by shard.
+inline void load_shard_edges(const std::string &base, int nshards, std::vector<vid_t> &src, std::vector<vid_t> &dst) {
+    std::vector<vid_t> ends = load_interval_ends(base, nshards);
+    for (size_t p = 0; p < ends.size(); p++) {
+        std::vector<uint8_t> adj = read_file_bytes(filename_shard_adj(base, (int)p, nshards));
+        if (!decode_adjacency(adj, src, dst))
+            logstream(LOG_FATAL) << "malformed shard " << filename_shard_adj(base, (int)p, nshards) << std::endl;
+    }
+}
+
+}  // namespace graphchi

@@ -52,6 +52,10 @@ int gchi_device_open(int device);
 /* Number of compute units and GPU arch name of the current device. */
 int gchi_device_info(int *num_cus, char *arch, size_t arch_len);
 
+/* Device memory for host-side callers (the engine's RAII buffers). */
+int gchi_malloc(void **d_ptr, size_t bytes);
+int gchi_free(void *d_ptr);
+int gchi_synchronize(void *stream);
 /* Copies bytes between any host/device pointers on the stream and waits for it
  * (the pinned one-time upload/readback of the engine). */
 int gchi_memcpy(void *dst, const void *src, size_t bytes, void *stream);
@@ -276,6 +280,14 @@ int gchi_tc_prepare_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes
 int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                     uint32_t *d_vertexmap, uint32_t *d_off, uint32_t *d_adj, uint64_t *h_noriented,
                     void *d_work, size_t work_bytes, void *stream);
+/* The translation table of order_by_degree alone, and an edge relabel
+ * (src' = map[src], dst' = map[dst]) -- the re-sharding input of
+ * orderbydegree.hpp:112-142. */
+int gchi_degree_order_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
+int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                      uint32_t *d_vertexmap, void *d_work, size_t work_bytes, void *stream);
+int gchi_relabel_edges(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                       const uint32_t *d_map, uint32_t *d_src_out, uint32_t *d_dst_out, void *stream);
 /* Per-vertex incident-triangle counts in the new id space (uint32, as
  * <G>_degord.4B.vout of example_apps/trianglecounting.cpp:72) and the uint64
  * total (:476-477).  *d_overflow != 0 when a count does not fit 32 bits. */
