@@ -104,7 +104,11 @@ def bench_ppr(gchi, torch, args, g, plan, N):
     rng = np.random.default_rng(5)
     queries = [rng.choice(N, size=int(rng.integers(1, 65)), replace=False).astype(np.uint32)
                for _ in range(args.ppr_queries)]
-    b = gchi.PprBatch(plan, len(queries))
+    info = plan.info()
+    # the batched kernel reads vertex ids from the plan's edges: same block shape, no gather layout
+    vplan = plan if not info["gather_sources"] else gchi.PageRankPlan(
+        g, edges_per_block=info["edges_per_block"], short_row_max=info["short_row_max"], persistent=1)
+    b = gchi.PprBatch(vplan, len(queries))
     b.run(queries[:1] * len(queries), max_sweeps=8)  # warm-up (seed tables, caches)
     (info, resid), t_batch = timed(lambda: b.run(queries), torch)
     log(f"ppr batched: {len(queries)} queries in {t_batch:.3f}s, sweeps bf16={info['sweeps_bf16']} "

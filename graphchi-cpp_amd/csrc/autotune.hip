@@ -1,8 +1,10 @@
 // Autotuner for the PageRank sweep launch shape.
 //
 // Candidates: edges per regular block W (which also sets the hub threshold),
-// the one-lane-per-row threshold, and persistent vs one-workgroup-per-tile
-// launches.  Each candidate gets its own plan on the caller's graph and is
+// the one-lane-per-row threshold, the launch form (one workgroup per tile,
+// persistent workgroups, or persistent wavefronts), and (N < 2^30) the
+// out-degree gather layout with and without an LDS cache of the most
+// referenced source values.  Each candidate gets its own plan on the caller's graph and is
 // timed with HIP events over a few sweeps after a warm-up sweep; the fastest
 // wins.  Results are cached per (N, E) in the process (a plan depends on the
 // graph, a tune only on its shape and degree distribution).
@@ -49,6 +51,15 @@ extern "C" int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, 
                 if (2 * W / (sm + 1) > 512) continue;
                 cand.push_back(gchi_pr_tune{W, sm, pers, 0});
             }
+    const bool layout_ok = g->nvertices < (1u << 30) && g->nedges > 0;
+    for (int sm : {8, 16, 32, 64})
+        for (int hot : {0, -1, 16384, 1 << 20}) {
+            if (hot && !layout_ok) continue;
+            cand.push_back(gchi_pr_tune{256, sm, 2, hot});
+        }
+    if (layout_ok)
+        for (int W : {1024, 2048})
+            for (int sm : {32, 64}) cand.push_back(gchi_pr_tune{W, sm, 3, 1 << 20});
     const size_t N = g->nvertices;
     float *buf = nullptr;
     double *st = nullptr;

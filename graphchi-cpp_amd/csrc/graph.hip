@@ -10,8 +10,22 @@
 
 namespace gchi {
 
-__global__ void degree_hist(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, uint32_t *outdeg,
-                            uint32_t *indeg, uint32_t *err) {
+// Degree histograms.  Counts of the ids below kPriv are kept per workgroup in
+// LDS and flushed once: on skewed graphs a few vertices own a large share of
+// the edges, and one global counter per such vertex would serialise every
+// atomic on it.  Higher ids count with global atomics directly.
+constexpr uint32_t kPriv = 4096;
+constexpr int kHistThreads = 256;
+
+__global__ __launch_bounds__(kHistThreads) void degree_hist(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                                                            uint32_t nv, uint32_t *outdeg, uint32_t *indeg,
+                                                            uint32_t *err) {
+    __shared__ uint32_t s_out[kPriv], s_in[kPriv];
+    for (uint32_t i = threadIdx.x; i < kPriv; i += kHistThreads) {
+        s_out[i] = 0u;
+        s_in[i] = 0u;
+    }
+    __syncthreads();
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t s = src[i], d = dst[i];
@@ -19,8 +33,20 @@ __global__ void degree_hist(const uint32_t *src, const uint32_t *dst, uint64_t n
             atomicOr(err, 1u);
             continue;
         }
-        if (outdeg) atomicAdd(&outdeg[s], 1u);
-        if (indeg) atomicAdd(&indeg[d], 1u);
+        if (outdeg) {
+            if (s < kPriv) atomicAdd(&s_out[s], 1u);
+            else atomicAdd(&outdeg[s], 1u);
+        }
+        if (indeg) {
+            if (d < kPriv) atomicAdd(&s_in[d], 1u);
+            else atomicAdd(&indeg[d], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t lim = nv < kPriv ? nv : kPriv;
+    for (uint32_t v = threadIdx.x; v < lim; v += kHistThreads) {
+        if (outdeg && s_out[v]) atomicAdd(&outdeg[v], s_out[v]);
+        if (indeg && s_in[v]) atomicAdd(&indeg[v], s_in[v]);
     }
 }
 
@@ -50,8 +76,8 @@ int degrees(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, u
     if (outdeg) launch_fill_u32(outdeg, 0u, nv, s);
     if (indeg) launch_fill_u32(indeg, 0u, nv, s);
     if (n)
-        hipLaunchKernelGGL(degree_hist, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, src, dst, n, nv, outdeg,
-                           indeg, err);
+        hipLaunchKernelGGL(degree_hist, dim3(grid_for(n, kHistThreads * 64, 2048)), dim3(kHistThreads), 0, s, src,
+                           dst, n, nv, outdeg, indeg, err);
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
 }

@@ -18,14 +18,20 @@
 //     (non-temporal), x[src] gathered into LDS, then rows are reduced: one lane
 //     per row up to short_row_max edges (sequential adds in src order, as the
 //     reference's gather), one wavefront per longer row.
-//   hub rows (in-degree >= W): their in-edges are cut into chunks of 4096; one
-//     workgroup reduces a chunk; a finalize kernel sums each hub's chunk
+//   hub rows (in-degree >= W): their in-edges are cut into chunks of 512; one
+//     wavefront reduces a chunk; a finalize kernel sums each hub's chunk
 //     partials in chunk order and applies.
 // Many graphs (a disjoint union with vertex bases) share one plan: blocks are
 // aligned to graph boundaries, so every block and hub belongs to one graph,
 // the apply can use a per-graph constant and FP64 statistics are per graph.
 // Tiles (hub chunks first, then regular blocks) are either one workgroup each
 // or pulled by persistent workgroups from a device work counter.
+// Gather layout (optional): sources are ranked by out-degree, descending, and
+// the plan's edges store that rank; each sweep first copies x into rank order
+// (xg), so the most referenced values share cache lines, and persistent
+// workgroups of the LDS-cached kernel keep xg[0, H) in LDS for the whole sweep
+// (on R-MAT graphs the top 32K sources cover about half of all edges).  The
+// gathered values, and therefore every sum, are the same as without it.
 // Every sum is in a fixed order, so a sweep is bit-reproducible run to run.
 #include "common.hpp"
 #include "internal.hpp"
@@ -52,6 +58,7 @@ struct gchi_pr_plan {
     uint32_t *hub_bits = nullptr;
     uint32_t *blk_row = nullptr;
     uint32_t *blk_graph = nullptr;
+    uint32_t *tile_e0 = nullptr;   // [nblk+1] first regular edge of each block
     uint32_t *hub_v = nullptr;
     uint32_t *hub_graph = nullptr;
     uint32_t *hub_chunk_off = nullptr;
@@ -66,6 +73,15 @@ struct gchi_pr_plan {
     double *hub_stat = nullptr;
     double *fin_stat = nullptr;
     uint32_t *counters = nullptr;  // [0] finalize ticket, [1] tile counter
+    int hot = 0;           // tune value (0: vertex ids, -1: gather layout, > 0: layout + LDS cache)
+    uint32_t A = 0;        // sources in the gather layout (out-degree > 0)
+    uint32_t H = 0;        // layout values cached in LDS by the wavefront-tiled kernel
+    uint32_t nwave_wg = 0; // workgroups of the wavefront-tiled kernel (persistent == 2)
+    size_t wave_lds = 0;   // its dynamic LDS bytes
+    uint32_t nhot_wg = 0;  // workgroups of the cached workgroup-tiled kernel (persistent == 3)
+    size_t hotwg_lds = 0;  // its dynamic LDS bytes
+    uint32_t *gorder = nullptr;  // [A] vertex at each layout position
+    float *xg = nullptr;         // [A] x in layout order, refreshed by every sweep
     void *mem = nullptr;
 };
 
@@ -73,10 +89,13 @@ namespace gchi {
 
 constexpr int kPrThreads = 256;
 constexpr int kPrWaves = kPrThreads / kWave;
-constexpr int kHubChunk = 4096;
+constexpr int kHubChunk = 512;
 constexpr int kMaxW = 4096;
 constexpr int kMaxLong = 512;  // long rows per regular block
 constexpr int kFinThreads = 256;
+constexpr size_t kStaticLds = 4096;  // slack for static LDS beside the dynamic allocation
+constexpr int kWaveThreads = 1024;   // workgroup of the wavefront-tiled sweep (one per CU)
+constexpr int kWaveWaves = kWaveThreads / kWave;
 
 struct PrConsts {
     double teleport;
@@ -87,12 +106,14 @@ struct PrConsts {
 };
 
 struct PrArgs {
-    const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *hub_bits, *chunk_beg, *chunk_len, *hub_src, *outdeg;
-    const float *x_in, *r_in;
+    const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *tile_e0, *hub_bits, *chunk_beg, *chunk_len, *hub_src,
+        *outdeg;
+    const float *xg;  // gather source: x by vertex id, or the gather layout
+    const float *r_in;
     float *x_out, *r_out, *chunk_part;
     double *blk_stat;
     uint32_t *tile_counter;
-    uint32_t nchunk, ntiles;
+    uint32_t nchunk, nhubt, nblk, ntiles;  // hub chunks, hub tiles (NW chunks each), regular blocks, all tiles
     int short_max;
 };
 
@@ -123,67 +144,97 @@ GCHI_DEV void pr_finish_row(uint32_t v, uint32_t g, float s, const PrConsts &c, 
     sum += (double)r;
 }
 
-// One tile: a hub chunk (tile < nchunk) or a regular block.
+// Makes a wavefront's LDS writes visible to its own later LDS reads (and
+// orders its reads before later overwrites); no other wavefront takes part.
+GCHI_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Partial sum of one hub chunk (<= kHubChunk in-edges) by one wavefront, in
+// the one order every kernel uses: lane l adds edges l, l+64, l+128, ... in
+// turn, then the 64 lane sums are combined by the butterfly of wave_sum.
+// Returns the sum in every lane.  With HOT, layout positions below H are read
+// from the LDS cache.
+template <bool HOT>
+GCHI_DEV float hub_chunk_sum(const PrArgs &a, uint32_t c, const float *hot, uint32_t H, int lane) {
+    constexpr int PER = kHubChunk / kWave;
+    const uint32_t cb = a.chunk_beg[c], cl = a.chunk_len[c];
+    // Every load is issued unconditionally at an in-bounds address (lanes past
+    // the chunk read its first edge and discard it), so the compiler emits no
+    // branch around loads and its wait counts stay exact.
+    uint32_t idx[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        uint32_t k = (uint32_t)j * kWave + lane;
+        idx[j] = __builtin_nontemporal_load(&a.hub_src[cb + (k < cl ? k : 0u)]);
+    }
+    float val[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t u = idx[j];
+        const bool in_lds = HOT && u < H;
+        float g = a.xg[in_lds ? 0u : u];
+        if (HOT) {
+            float h = hot[in_lds ? u : 0u];
+            g = in_lds ? h : g;
+        }
+        val[j] = (uint32_t)j * kWave + lane < cl ? g : 0.0f;
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < PER; j++) acc += val[j];
+    return wave_sum(acc);
+}
+
+// One tile of a T-thread workgroup: NW = T/64 hub chunks (one per wavefront,
+// tile < nhubt) or one regular block.
+template <int T, bool HOT = false>
 GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *s_val, uint32_t *s_long,
-                      uint32_t *s_nlong, double *s_red, float *s_redf) {
+                      uint32_t *s_nlong, double *s_red, const float *hot = nullptr, uint32_t H = 0) {
+    constexpr int NW = T / kWave;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    if (tile < a.nchunk) {
-        // ---- hub chunk: partial sum of <= 4096 in-edges of one hub row ----
-        const uint32_t cb = a.chunk_beg[tile], cl = a.chunk_len[tile];
-        uint32_t idx[kHubChunk / kPrThreads];
-#pragma unroll
-        for (int j = 0; j < kHubChunk / kPrThreads; j++) {
-            uint32_t k = (uint32_t)j * kPrThreads + tid;
-            idx[j] = k < cl ? __builtin_nontemporal_load(&a.hub_src[cb + k]) : 0u;
-        }
-        float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kHubChunk / kPrThreads; j++) {
-            uint32_t k = (uint32_t)j * kPrThreads + tid;
-            float v = k < cl ? a.x_in[idx[j]] : 0.0f;
-            acc += v;
-        }
-        acc = wave_sum(acc);
-        if (lane == 0) s_redf[wave] = acc;
-        __syncthreads();
-        if (tid == 0) {
-            float t = 0.0f;
-            for (int w = 0; w < kPrWaves; w++) t += s_redf[w];
-            a.chunk_part[tile] = t;
+    if (tile < a.nhubt) {
+        const uint32_t ch = tile * NW + wave;
+        if (ch < a.nchunk) {
+            float t = hub_chunk_sum<HOT>(a, ch, hot, H, lane);
+            if (lane == 0) a.chunk_part[ch] = t;
         }
         return;
     }
 
     // ---- regular block ----
-    const uint32_t b = tile - a.nchunk;
+    const uint32_t b = tile - a.nhubt;
     const uint32_t rb = a.blk_row[b], re = a.blk_row[b + 1];
     const uint32_t g = a.blk_graph[b];
     const uint32_t e0 = a.reg_off[rb], e1 = a.reg_off[re];
     const uint32_t span = e1 - e0;
     if (tid == 0) *s_nlong = 0;
-    for (uint32_t base = 0; base < span; base += 8 * kPrThreads) {
+    for (uint32_t base = 0; base < span; base += 8 * T) {
         uint32_t idx[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            uint32_t k = base + (uint32_t)j * kPrThreads + tid;
+            uint32_t k = base + (uint32_t)j * T + tid;
             idx[j] = k < span ? __builtin_nontemporal_load(&a.reg_src[e0 + k]) : 0u;
         }
         float val[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            uint32_t k = base + (uint32_t)j * kPrThreads + tid;
-            val[j] = k < span ? a.x_in[idx[j]] : 0.0f;
+            uint32_t k = base + (uint32_t)j * T + tid;
+            val[j] = 0.0f;
+            if (k < span) val[j] = (HOT && idx[j] < H) ? hot[idx[j]] : a.xg[idx[j]];
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            uint32_t k = base + (uint32_t)j * kPrThreads + tid;
+            uint32_t k = base + (uint32_t)j * T + tid;
             if (k < span) s_val[k] = val[j];
         }
     }
     __syncthreads();
 
     double res = 0.0, sum = 0.0;
-    for (uint32_t v = rb + tid; v < re; v += kPrThreads) {
+    for (uint32_t v = rb + tid; v < re; v += T) {
         if ((a.hub_bits[v >> 5] >> (v & 31)) & 1u) continue;
         uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
         if ((int)(end - beg) <= a.short_max) {
@@ -197,7 +248,7 @@ GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *
     }
     __syncthreads();
     const uint32_t nlong = *s_nlong;
-    for (uint32_t li = wave; li < nlong; li += kPrWaves) {
+    for (uint32_t li = wave; li < nlong; li += NW) {
         uint32_t v = s_long[li];
         uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
         float s = 0.0f;
@@ -214,7 +265,7 @@ GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *
     __syncthreads();
     if (tid == 0) {
         double r0 = 0.0, s0 = 0.0;
-        for (int w = 0; w < kPrWaves; w++) {
+        for (int w = 0; w < NW; w++) {
             r0 += s_red[2 * w];
             s0 += s_red[2 * w + 1];
         }
@@ -228,8 +279,7 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(PrArgs a, PrConsts
     __shared__ uint32_t s_long[kMaxLong];
     __shared__ uint32_t s_nlong;
     __shared__ double s_red[2 * kPrWaves];
-    __shared__ float s_redf[kPrWaves];
-    pr_tile(a, c, blockIdx.x, s_val, s_long, &s_nlong, s_red, s_redf);
+    pr_tile<kPrThreads>(a, c, blockIdx.x, s_val, s_long, &s_nlong, s_red);
 }
 
 // Persistent form: workgroups pull tiles from the work counter until the queue
@@ -240,16 +290,397 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_persistent_kernel(PrArgs 
     __shared__ uint32_t s_long[kMaxLong];
     __shared__ uint32_t s_nlong;
     __shared__ double s_red[2 * kPrWaves];
-    __shared__ float s_redf[kPrWaves];
     __shared__ uint32_t s_tile;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= a.ntiles) break;
-        pr_tile(a, c, tile, s_val, s_long, &s_nlong, s_red, s_redf);
+        pr_tile<kPrThreads>(a, c, tile, s_val, s_long, &s_nlong, s_red);
         __syncthreads();
     }
+}
+
+// Persistent workgroups with a static tile list (tile = blockIdx.x, then every
+// gridDim.x-th) and an LDS cache of layout positions [0, H), loaded once per
+// workgroup; several such workgroups share a CU, each with its own copy.
+__global__ __launch_bounds__(kPrThreads) void pr_sweep_hotwg_kernel(PrArgs a, PrConsts c, uint32_t H) {
+    extern __shared__ float lds[];  // H + 2W floats
+    __shared__ uint32_t s_long[kMaxLong];
+    __shared__ uint32_t s_nlong;
+    __shared__ double s_red[2 * kPrWaves];
+    float *hot = lds;
+    float *s_val = lds + H;
+    for (uint32_t i = threadIdx.x; i < H; i += kPrThreads) hot[i] = a.xg[i];
+    __syncthreads();
+    for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        pr_tile<kPrThreads, true>(a, c, tile, s_val, s_long, &s_nlong, s_red, hot, H);
+        __syncthreads();
+    }
+}
+
+// ---- wavefront-tiled persistent sweep ----
+// Every wavefront works alone through a fixed list of tiles: hub chunks
+// round-robin, then a contiguous range of regular blocks, whose gathered values
+// it stages in its own LDS slice (2W floats), so no workgroup barrier separates
+// one wavefront's gathers from another's reductions.  The workgroup first loads
+// layout positions [0, H) into LDS (the most referenced sources of the gather
+// layout) and serves those gathers from there for the whole sweep.
+// Blocks are software-pipelined: while block b is reduced, the gathers of
+// block b+1 and the edge ids of block b+2 are in flight.  Memory operations
+// complete in issue order as far as the wait counters are concerned, so the
+// ids of b+2 and the row metadata of b+1 are issued before the gathers of b+1,
+// and everything the reduction of b reads was issued before the gathers of b.
+// Per row, the summation order is the one of pr_tile (one lane per short row
+// in src order, one wavefront per long row, strided), so ranks are
+// bit-identical to the workgroup-tiled kernels.
+
+struct TileDesc {
+    uint32_t rb, re, e0, e1, g;
+    double tele;  // the graph's constant term
+};
+
+// Row metadata of row v = rb + lane of a block (zero when v >= re).
+struct RowMeta {
+    uint32_t beg, end, hubw, seedw, od;
+    float ro;
+};
+
+// Descriptors of 64 consecutive blocks, lane j holding block base + j.
+struct DescBatch {
+    uint32_t base;
+    uint32_t rb, re, e0, e1, g;
+    uint32_t tlo, thi;
+};
+
+GCHI_DEV void load_desc_batch(const PrArgs &a, const PrConsts &c, uint32_t base, uint32_t bend, int lane,
+                              DescBatch &d) {
+    d.base = base;
+    const uint32_t b = min(base + (uint32_t)lane, bend - 1);  // bend > base
+    d.rb = a.blk_row[b];
+    d.re = a.blk_row[b + 1];
+    d.e0 = a.tile_e0[b];
+    d.e1 = a.tile_e0[b + 1];
+    d.g = a.blk_graph[b];
+    double t = c.teleport;
+    if (c.g_tele) t = c.g_tele[d.g];
+    d.tlo = (uint32_t)__double_as_longlong(t);
+    d.thi = (uint32_t)(__double_as_longlong(t) >> 32);
+}
+
+GCHI_DEV TileDesc get_desc(const DescBatch &d, uint32_t b) {
+    const int j = (int)(b - d.base);
+    TileDesc t;
+    t.rb = __builtin_amdgcn_readlane(d.rb, j);
+    t.re = __builtin_amdgcn_readlane(d.re, j);
+    t.e0 = __builtin_amdgcn_readlane(d.e0, j);
+    t.e1 = __builtin_amdgcn_readlane(d.e1, j);
+    t.g = __builtin_amdgcn_readlane(d.g, j);
+    // readlane returns int: widen through uint32_t so the low word is not sign-extended
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane(d.tlo, j);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane(d.thi, j);
+    t.tele = __longlong_as_double((long long)(lo | (hi << 32)));
+    return t;
+}
+
+// The loads below are issued unconditionally at in-bounds addresses (a lane
+// with nothing to load reads a valid element), and nothing is computed from a
+// loaded value until it is consumed one pipeline step later: a load under a
+// branch, or an instruction using a loaded register, makes the wavefront wait
+// for every earlier load, which would serialise the pipeline.
+
+// Edge ids of a block, raw (lanes past the span hold a discarded value).
+template <int PER>
+GCHI_DEV void load_ids(const PrArgs &a, const TileDesc &t, int lane, uint32_t (&idx)[PER]) {
+    const uint32_t span = t.e1 - t.e0;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t k = (uint32_t)j * kWave + lane;
+        idx[j] = __builtin_nontemporal_load(&a.reg_src[k < span ? t.e0 + k : 0u]);
+    }
+}
+
+// Gathered values of a block, raw: gv from global memory, hv from the LDS
+// cache; cls bit j = value j comes from LDS, bit 16+j = no edge (value 0).
+template <int PER, bool HOT>
+struct Gathered {
+    float gv[PER];
+    float hv[HOT ? PER : 1];
+    uint32_t cls;
+};
+
+template <int PER, bool HOT>
+GCHI_DEV void gather_ids(const PrArgs &a, const TileDesc &t, const float *hot, uint32_t H, int lane,
+                         const uint32_t (&idx)[PER], Gathered<PER, HOT> &gd) {
+    const uint32_t span = t.e1 - t.e0;
+    uint32_t cls = 0;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t u = idx[j];
+        const bool none = (uint32_t)j * kWave + lane >= span;
+        const bool in_lds = HOT && !none && u < H;
+        gd.gv[j] = a.xg[(none || in_lds) ? 0u : u];
+        if (HOT) gd.hv[j] = hot[in_lds ? u : 0u];
+        cls |= (in_lds ? 1u : 0u) << j;
+        cls |= (none ? 1u : 0u) << (16 + j);
+    }
+    gd.cls = cls;
+}
+
+// Row metadata of row v = rb + lane, raw (meaningful only when v < re).
+GCHI_DEV void load_meta(const PrArgs &a, const PrConsts &c, const TileDesc &t, int lane, RowMeta &m) {
+    const uint32_t v = t.rb + lane;
+    const uint32_t w = v < t.re ? v : t.rb;  // rb < re: every block has a row
+    m.beg = a.reg_off[w];
+    m.end = a.reg_off[w + 1];
+    m.hubw = a.hub_bits[w >> 5];
+    m.seedw = (c.seed_bits ? c.seed_bits : a.hub_bits)[w >> 5];
+    m.od = a.outdeg[w];
+    m.ro = a.r_in[w];
+}
+
+// pr_finish_row with the row's inputs already in registers
+GCHI_DEV void finish_row_pf(uint32_t v, double tele, uint32_t seedw, const PrConsts &c, float s, float ro,
+                            uint32_t od, float *r_out, float *x_out, double &res, double &sum) {
+    double t = tele;
+    if (c.seed_bits && ((seedw >> (v & 31)) & 1u)) t += c.seed_tele;
+    float r = pr_apply(t, c.damping, s);
+    r_out[v] = r;
+    x_out[v] = pr_scatter(r, od);
+    res += fabs((double)r - (double)ro);
+    sum += (double)r;
+}
+
+// ---- hub chunks of the wavefront-tiled sweep, software-pipelined ----
+// Same order of additions as hub_chunk_sum (lane j*64+l values in turn, then
+// the butterfly), with the ids of chunk i+2 and the gathers of chunk i+1 in
+// flight while chunk i is summed.
+constexpr int kChunkPer = kHubChunk / kWave;
+
+GCHI_DEV void chunk_ids(const PrArgs &a, uint32_t ch, int lane, uint32_t (&idx)[kChunkPer]) {
+    const uint32_t cb = a.chunk_beg[ch], cl = a.chunk_len[ch];
+#pragma unroll
+    for (int j = 0; j < kChunkPer; j++) {
+        const uint32_t k = (uint32_t)j * kWave + lane;
+        idx[j] = __builtin_nontemporal_load(&a.hub_src[cb + (k < cl ? k : 0u)]);
+    }
+}
+
+template <bool HOT>
+struct ChunkVals {
+    float gv[kChunkPer];
+    float hv[HOT ? kChunkPer : 1];
+    uint32_t lds_bits, cl;
+};
+
+template <bool HOT>
+GCHI_DEV void chunk_gather(const PrArgs &a, uint32_t ch, const float *hot, uint32_t H,
+                           const uint32_t (&idx)[kChunkPer], ChunkVals<HOT> &cv) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < kChunkPer; j++) {
+        const uint32_t u = idx[j];
+        const bool in_lds = HOT && u < H;
+        cv.gv[j] = a.xg[in_lds ? 0u : u];
+        if (HOT) cv.hv[j] = hot[in_lds ? u : 0u];
+        bits |= (in_lds ? 1u : 0u) << j;
+    }
+    cv.lds_bits = bits;
+    cv.cl = a.chunk_len[ch];
+}
+
+template <bool HOT>
+GCHI_DEV float chunk_sum(const ChunkVals<HOT> &cv, int lane) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kChunkPer; j++) {
+        float v = cv.gv[j];
+        asm volatile("" : "+v"(v));  // no "select of loads" folding (see wave_step)
+        if (HOT) {
+            float h = cv.hv[j];
+            asm volatile("" : "+v"(h));
+            v = ((cv.lds_bits >> j) & 1u) ? h : v;
+        }
+        acc += (uint32_t)j * kWave + lane < cv.cl ? v : 0.0f;
+    }
+    return wave_sum(acc);
+}
+
+template <bool HOT, int S>
+GCHI_DEV void chunk_step(const PrArgs &a, const float *hot, uint32_t H, int lane, uint32_t ch, uint32_t step,
+                         uint32_t (&idx)[2][kChunkPer], ChunkVals<HOT> (&cv)[2]) {
+    if (ch + step < a.nchunk) chunk_gather<HOT>(a, ch + step, hot, H, idx[1 - S], cv[1 - S]);
+    if (ch + 2 * step < a.nchunk) chunk_ids(a, ch + 2 * step, lane, idx[S]);
+    const float t = chunk_sum<HOT>(cv[S], lane);
+    if (lane == 0) a.chunk_part[ch] = t;
+}
+
+template <bool HOT>
+GCHI_DEV void wave_chunks(const PrArgs &a, const float *hot, uint32_t H, int lane, uint32_t ch0, uint32_t step) {
+    if (ch0 >= a.nchunk) return;
+    uint32_t idx[2][kChunkPer];
+    ChunkVals<HOT> cv[2];
+    chunk_ids(a, ch0, lane, idx[0]);
+    if (ch0 + step < a.nchunk) chunk_ids(a, ch0 + step, lane, idx[1]);
+    chunk_gather<HOT>(a, ch0, hot, H, idx[0], cv[0]);
+    uint32_t ch = ch0;
+    for (; ch + step < a.nchunk; ch += 2 * step) {
+        chunk_step<HOT, 0>(a, hot, H, lane, ch, step, idx, cv);
+        chunk_step<HOT, 1>(a, hot, H, lane, ch + step, step, idx, cv);
+    }
+    if (ch < a.nchunk) chunk_step<HOT, 0>(a, hot, H, lane, ch, step, idx, cv);
+}
+
+// Reduces block b (values staged in st, metadata of its first 64 rows in m).
+GCHI_DEV void wave_rows(const PrArgs &a, const PrConsts &c, uint32_t b, const TileDesc &t, const RowMeta &m,
+                        const float *st, int lane) {
+    double res = 0.0, sum = 0.0;
+    {
+        const uint32_t v = t.rb + lane;
+        bool is_long = false;
+        if (v < t.re && !((m.hubw >> (v & 31)) & 1u)) {
+            const uint32_t beg = m.beg - t.e0, end = m.end - t.e0;
+            if ((int)(end - beg) <= a.short_max) {
+                float s = 0.0f;
+                for (uint32_t k = beg; k < end; k++) s += st[k];
+                finish_row_pf(v, t.tele, m.seedw, c, s, m.ro, m.od, a.r_out, a.x_out, res, sum);
+            } else {
+                is_long = true;
+            }
+        }
+        uint64_t mask = __ballot(is_long);
+        while (mask) {
+            const int i = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t u = t.rb + (uint32_t)i;
+            const uint32_t beg = (uint32_t)__builtin_amdgcn_readlane(m.beg, i) - t.e0;
+            const uint32_t end = (uint32_t)__builtin_amdgcn_readlane(m.end, i) - t.e0;
+            const uint32_t seedw = __builtin_amdgcn_readlane(m.seedw, i);
+            const uint32_t od = __builtin_amdgcn_readlane(m.od, i);
+            const float ro = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.ro), i));
+            float s = 0.0f;
+            for (uint32_t k = beg + lane; k < end; k += kWave) s += st[k];
+            s = wave_sum(s);
+            if (lane == 0) finish_row_pf(u, t.tele, seedw, c, s, ro, od, a.r_out, a.x_out, res, sum);
+        }
+    }
+    // rows past the first 64 (blocks of many short rows): metadata read here
+    for (uint32_t v0 = t.rb + kWave; v0 < t.re; v0 += kWave) {
+        const uint32_t v = v0 + lane;
+        bool is_long = false;
+        if (v < t.re && !((a.hub_bits[v >> 5] >> (v & 31)) & 1u)) {
+            const uint32_t beg = a.reg_off[v] - t.e0, end = a.reg_off[v + 1] - t.e0;
+            if ((int)(end - beg) <= a.short_max) {
+                float s = 0.0f;
+                for (uint32_t k = beg; k < end; k++) s += st[k];
+                pr_finish_row(v, t.g, s, c, a.r_in, a.outdeg, a.r_out, a.x_out, res, sum);
+            } else {
+                is_long = true;
+            }
+        }
+        uint64_t mask = __ballot(is_long);
+        while (mask) {
+            const uint32_t u = v0 + (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t beg = a.reg_off[u] - t.e0, end = a.reg_off[u + 1] - t.e0;
+            float s = 0.0f;
+            for (uint32_t k = beg + lane; k < end; k += kWave) s += st[k];
+            s = wave_sum(s);
+            if (lane == 0) pr_finish_row(u, t.g, s, c, a.r_in, a.outdeg, a.r_out, a.x_out, res, sum);
+        }
+    }
+    res = wave_sum(res);
+    sum = wave_sum(sum);
+    if (lane == 0) {
+        a.blk_stat[2 * b] = res;
+        a.blk_stat[2 * b + 1] = sum;
+    }
+}
+
+// One pipeline step for block b (slot S holds b's gathered values and
+// metadata, slot 1-S receives b+1's; idx[S] receives b+2's edge ids).
+template <int PER, bool HOT, int S>
+GCHI_DEV void wave_step(const PrArgs &a, const PrConsts &c, const float *hot, uint32_t H, float *st, int lane,
+                        uint32_t b, uint32_t b1, const DescBatch &db, uint32_t (&idx)[2][PER],
+                        Gathered<PER, HOT> (&gd)[2], RowMeta (&meta)[2]) {
+    const TileDesc t = get_desc(db, b);
+    if (b + 2 < b1) load_ids<PER>(a, get_desc(db, b + 2), lane, idx[S]);
+    if (b + 1 < b1) {
+        const TileDesc t1 = get_desc(db, b + 1);
+        load_meta(a, c, t1, lane, meta[1 - S]);
+        gather_ids<PER, HOT>(a, t1, hot, H, lane, idx[1 - S], gd[1 - S]);
+    }
+    // stage block b: the whole 2W slice, zeros past the span
+    const uint32_t cls = gd[S].cls;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        // the empty asm keeps the compiler from folding "select of two loaded
+        // values" into a load through a selected address (a stack array)
+        float v = gd[S].gv[j];
+        asm volatile("" : "+v"(v));
+        if (HOT) {
+            float h = gd[S].hv[j];
+            asm volatile("" : "+v"(h));
+            v = ((cls >> j) & 1u) ? h : v;
+        }
+        v = ((cls >> (16 + j)) & 1u) ? 0.0f : v;
+        st[(uint32_t)j * kWave + lane] = v;
+    }
+    wave_lds_sync();
+    wave_rows(a, c, b, t, meta[S], st, lane);
+    wave_lds_sync();
+}
+
+// Blocks [b0, b1) of one wavefront, b1 - b0 <= 64: descriptors in one batch,
+// then the pipeline (prologue, steps in slot pairs).
+template <int PER, bool HOT>
+GCHI_DEV void wave_blocks(const PrArgs &a, const PrConsts &c, const float *hot, uint32_t H, float *st, int lane,
+                          uint32_t b0, uint32_t b1) {
+    DescBatch db;
+    load_desc_batch(a, c, b0, b1, lane, db);
+    uint32_t idx[2][PER];
+    Gathered<PER, HOT> gd[2];
+    RowMeta meta[2];
+    const TileDesc t0 = get_desc(db, b0);
+    load_ids<PER>(a, t0, lane, idx[0]);
+    load_meta(a, c, t0, lane, meta[0]);
+    if (b0 + 1 < b1) load_ids<PER>(a, get_desc(db, b0 + 1), lane, idx[1]);
+    gather_ids<PER, HOT>(a, t0, hot, H, lane, idx[0], gd[0]);
+    uint32_t b = b0;
+    for (; b + 1 < b1; b += 2) {
+        wave_step<PER, HOT, 0>(a, c, hot, H, st, lane, b, b1, db, idx, gd, meta);
+        wave_step<PER, HOT, 1>(a, c, hot, H, st, lane, b + 1, b1, db, idx, gd, meta);
+    }
+    if (b < b1) wave_step<PER, HOT, 0>(a, c, hot, H, st, lane, b, b1, db, idx, gd, meta);
+}
+
+template <int T, int PER, bool HOT>
+__global__ __launch_bounds__(T) void pr_sweep_wave_kernel(PrArgs a, PrConsts c, uint32_t H) {
+    constexpr int NW = T / kWave;
+    extern __shared__ float lds[];  // H + NW * PER*64 floats
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    float *hot = lds;
+    float *st = lds + H + (size_t)wave * PER * kWave;
+    if (HOT) {
+        for (uint32_t i = tid; i < H; i += T) hot[i] = a.xg[i];
+    }
+    __syncthreads();
+    const uint32_t nwv = gridDim.x * NW;
+    const uint32_t gw = blockIdx.x * NW + wave;
+    wave_chunks<HOT>(a, hot, H, lane, gw, nwv);
+    const uint64_t per = ((uint64_t)a.nblk + nwv - 1) / nwv;
+    const uint32_t b0 = (uint32_t)min<uint64_t>(a.nblk, per * gw);
+    const uint32_t b1 = (uint32_t)min<uint64_t>(a.nblk, per * gw + per);
+    for (uint32_t p = b0; p < b1; p += kWave)
+        wave_blocks<PER, HOT>(a, c, hot, H, st, lane, p, min(b1, p + (uint32_t)kWave));
+}
+
+// Gather-layout refresh: xg[i] = x[order[i]] for the A layout positions.
+__global__ void pr_gather_layout_kernel(const uint32_t *__restrict__ order, uint32_t A, const float *__restrict__ x,
+                                        float *__restrict__ xg) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += stride) xg[i] = x[order[i]];
 }
 
 // Single-graph finalize: hub rows (sum of chunk partials, apply) + the
@@ -299,17 +730,31 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
         s_last = (ticket == gridDim.x - 1);
     }
     __syncthreads();
-    if (s_last && tid == 0) {
+    if (s_last) {  // block-uniform
         __threadfence();
         double a = 0.0, s2 = 0.0;
-        for (uint32_t j = 0; j < gridDim.x; j++) {
+        for (uint32_t j = tid; j < gridDim.x; j += kFinThreads) {
             a += ld_relaxed_agent(&fin_stat[2 * j]);
             s2 += ld_relaxed_agent(&fin_stat[2 * j + 1]);
         }
-        d_stats[0] = a;
-        d_stats[1] = s2;
-        counters[0] = 0u;
-        counters[1] = 0u;
+        a = wave_sum(a);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            s_red[2 * wave] = a;
+            s_red[2 * wave + 1] = s2;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double ta = 0.0, ts = 0.0;
+            for (int w = 0; w < kFinThreads / kWave; w++) {
+                ta += s_red[2 * w];
+                ts += s_red[2 * w + 1];
+            }
+            d_stats[0] = ta;
+            d_stats[1] = ts;
+            counters[0] = 0u;
+            counters[1] = 0u;
+        }
     }
 }
 
@@ -424,17 +869,52 @@ __global__ void plan_hubs(const uint32_t *in_off, uint32_t N, const uint32_t *hu
     }
 }
 
-// one wavefront per row: copy the row's in-edges into the regular or hub array
+// one wavefront per row: copy the row's in-edges into the regular or hub array,
+// mapped to gather-layout positions when grank is given.  A source id >= N, or
+// one without a layout position (its out-degree says 0), sets *bad and is
+// stored as 0, so no sweep ever gathers out of bounds; the plan is refused.
 __global__ void plan_copy(const uint32_t *in_off, const uint32_t *in_src, uint32_t N, const uint32_t *hub_flag,
                           const uint32_t *hub_idx, const uint32_t *reg_off, const uint32_t *hub_off,
-                          uint32_t *reg_src, uint32_t *hub_src) {
+                          const uint32_t *grank, uint32_t *reg_src, uint32_t *hub_src, uint32_t *bad) {
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
     for (uint64_t v = wid; v < N; v += nw) {
         uint32_t b = in_off[v], e = in_off[v + 1];
         uint32_t *dst = hub_flag[v] ? hub_src + hub_off[hub_idx[v]] : reg_src + reg_off[v];
-        for (uint32_t k = b + lane; k < e; k += kWave) dst[k - b] = in_src[k];
+        for (uint32_t k = b + lane; k < e; k += kWave) {
+            uint32_t u = in_src[k];
+            bool ok = u < N;
+            if (ok && grank) {
+                u = grank[u];
+                ok = u != 0xffffffffu;
+            }
+            if (!ok) {
+                atomicOr(bad, 1u);
+                u = 0;
+            }
+            dst[k - b] = u;
+        }
+    }
+}
+
+// gather-layout keys: ascending order = out-degree descending, then id
+__global__ void plan_order_keys(const uint32_t *outdeg, uint32_t N, uint64_t *keys) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride)
+        keys[v] = ((uint64_t)(~outdeg[v]) << 32) | v;
+}
+
+// layout position of every vertex with out-degree > 0; *A = their count
+__global__ void plan_gather_rank(const uint64_t *keys, uint32_t N, uint32_t *order, uint32_t *grank, uint32_t *A) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        const uint32_t v = (uint32_t)keys[i];
+        const uint32_t od = ~(uint32_t)(keys[i] >> 32);
+        if (od == 0) continue;
+        order[i] = v;
+        grank[v] = i;
+        if (i + 1 == N || ~(uint32_t)(keys[i + 1] >> 32) == 0u) *A = i + 1;
     }
 }
 
@@ -479,6 +959,11 @@ __global__ void plan_blocks(const uint32_t *reg_off, const uint32_t *vbase, uint
         }
         if (v == N - 1) blk_row[nblk] = N;
     }
+}
+
+__global__ void plan_tile_e0(const uint32_t *reg_off, const uint32_t *blk_row, uint32_t nblk, uint32_t *tile_e0) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nblk; b += stride) tile_e0[b] = reg_off[blk_row[b]];
 }
 
 __global__ void plan_hub_graphs(const uint32_t *hub_v, uint32_t nhub, const uint32_t *vbase, uint32_t G,
@@ -579,7 +1064,34 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     uint32_t nchunk = 0;
     GCHI_TRY(read_u32(hub_choff + nhub, &nchunk, s));
     p->nchunk = nchunk;
-    p->nfin = 64;
+    // one wavefront per hub row; at least 64 workgroups for the statistics
+    p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(64, (nhub + kFinThreads / kWave - 1) / (kFinThreads / kWave)));
+
+    // gather layout: sources ranked by out-degree (descending, ties by id)
+    uint32_t *grank = nullptr, *gorder_tmp = nullptr, *bad = tp.alloc<uint32_t>(1);
+    GCHI_CHECK(bad, GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
+    launch_fill_u32(bad, 0u, 1, s);
+    uint32_t A = 0;
+    if (p->hot) {
+        uint64_t *keys = tp.alloc<uint64_t>(N);
+        grank = tp.alloc<uint32_t>(N);
+        gorder_tmp = tp.alloc<uint32_t>(N);
+        uint32_t *dA = tp.alloc<uint32_t>(1);
+        size_t sb = 0;
+        GCHI_TRY(gchi_sort_workspace(N, 64, 0, &sb));
+        void *sws = tp.alloc<char>(sb);
+        GCHI_CHECK(keys && grank && gorder_tmp && dA && sws, GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
+        launch_fill_u32(grank, 0xffffffffu, N, s);
+        launch_fill_u32(dA, 0u, 1, s);
+        hipLaunchKernelGGL(plan_order_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, g->outdeg, N, keys);
+        GCHI_LAUNCH_CHECK();
+        GCHI_TRY(gchi_sort_u64(keys, nullptr, N, 64, sws, sb, s));
+        hipLaunchKernelGGL(plan_gather_rank, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, keys, N, gorder_tmp,
+                           grank, dA);
+        GCHI_LAUNCH_CHECK();
+        GCHI_TRY(read_u32(dA, &A, s));
+    }
+    p->A = A;
 
     Sizer zz;
     zz.take<uint32_t>(N + 1);
@@ -601,6 +1113,9 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     zz.take<double>(2 * (size_t)nhub + 2);
     zz.take<double>(2 * (size_t)p->nfin);
     zz.take<uint32_t>(2);
+    zz.take<uint32_t>(std::max<uint32_t>(A, 1));
+    zz.take<float>(std::max<uint32_t>(A, 1));
+    zz.take<uint32_t>(p->nblk + 1);
     GCHI_HIP(hipMalloc(&p->mem, zz.off));
     Carver c(p->mem, zz.off);
     p->reg_off = c.take<uint32_t>(N + 1);
@@ -622,6 +1137,10 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     p->hub_stat = c.take<double>(2 * (size_t)nhub + 2);
     p->fin_stat = c.take<double>(2 * (size_t)p->nfin);
     p->counters = c.take<uint32_t>(2);
+    p->gorder = c.take<uint32_t>(std::max<uint32_t>(A, 1));
+    p->xg = c.take<float>(std::max<uint32_t>(A, 1));
+    p->tile_e0 = c.take<uint32_t>(p->nblk + 1);
+    if (A) GCHI_HIP(hipMemcpyAsync(p->gorder, gorder_tmp, (size_t)A * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->reg_off, reg_off, (size_t)(N + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_bits, bits, nwords * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_v, hub_v, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
@@ -631,7 +1150,7 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     launch_fill_u32(p->counters, 0u, 2, s);
     if (E)
         hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, g->in_off,
-                           g->in_src, N, hub_flag, hub_idx, reg_off, hub_off, p->reg_src, p->hub_src);
+                           g->in_src, N, hub_flag, hub_idx, reg_off, hub_off, grank, p->reg_src, p->hub_src, bad);
     if (nhub) {
         hipLaunchKernelGGL(plan_chunks, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, hub_off, hub_choff, nhub,
                            p->chunk_beg, p->chunk_len);
@@ -643,7 +1162,13 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     hipLaunchKernelGGL(plan_blocks, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->reg_off, p->vbase, G, p->gblk, N,
                        W, p->nblk, p->blk_row, p->blk_graph);
     GCHI_LAUNCH_CHECK();
-    GCHI_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(plan_tile_e0, dim3(grid_for(p->nblk + 1, 256, 65536)), dim3(256), 0, s, p->reg_off, p->blk_row,
+                       p->nblk, p->tile_e0);
+    GCHI_LAUNCH_CHECK();
+    uint32_t h_bad = 0;
+    GCHI_TRY(read_u32(bad, &h_bad, s));
+    GCHI_CHECK(!h_bad, GCHI_ERR_INVALID,
+               "pr_plan_create: an in-edge source is >= N or has out-degree 0 (in_src and outdeg disagree)");
     return GCHI_OK;
 }
 
@@ -671,15 +1196,74 @@ static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G
     p->G = G;
     p->W = W;
     p->short_max = short_max;
-    p->persistent = tune ? (tune->persistent != 0) : 0;
+    p->persistent = tune ? tune->persistent : 0;
+    p->hot = tune ? tune->hot_vertices : 0;
+    if (p->persistent < 0 || p->persistent > 3) {
+        delete p;
+        return fail(GCHI_ERR_INVALID, "pr_plan_create: persistent must be 0, 1, 2 or 3");
+    }
+    // the LDS cache lives in the persistent kernels 2 and 3 (default 2)
+    if (p->hot > 0 && p->persistent != 3) p->persistent = 2;
+    if (p->hot) {
+        if (p->hot < -1) {
+            delete p;
+            return fail(GCHI_ERR_INVALID, "pr_plan_create: hot_vertices must be -1, 0 or positive");
+        }
+        if (g->nvertices >= (1u << 30)) {
+            delete p;
+            return fail(GCHI_ERR_UNSUPPORTED, "pr_plan_create: the gather layout needs N < 2^30");
+        }
+    }
     p->outdeg = g->outdeg;
     int st = plan_build(g, h_vbase, W, p, s);
     if (st == GCHI_OK) {
         int dev = 0, ncu = 256;
+        size_t lds_cu = 160 * 1024, lds_wg = 160 * 1024;
         hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
             ncu = prop.multiProcessorCount;
-        p->npersist = (uint32_t)std::min<uint64_t>((uint64_t)ncu * 8, (uint64_t)p->nblk + p->nchunk);
+            lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+            lds_wg = prop.sharedMemPerBlock;
+        }
+        const uint32_t nhubt = (p->nchunk + kPrWaves - 1) / kPrWaves;
+        p->npersist = (uint32_t)std::min<uint64_t>((uint64_t)ncu * 8, (uint64_t)p->nblk + nhubt);
+        if (p->persistent == 3) {
+            // 256-thread workgroups, four per CU: each holds H cached values and
+            // its 2W staging floats within a quarter of the CU's LDS
+            const size_t cap = std::min(lds_cu / 4, lds_wg);
+            const size_t stage = (size_t)8 * W;
+            uint32_t H = p->hot > 0 ? std::min<uint32_t>((uint32_t)p->hot, p->A) / 1024 * 1024 : 0;
+            const size_t hmax = cap > stage + kStaticLds ? (cap - stage - kStaticLds) / 4 / 1024 * 1024 : 0;
+            H = std::min<uint32_t>(H, (uint32_t)hmax);
+            p->H = H;
+            p->hotwg_lds = stage + 4 * (size_t)H;
+            p->nhot_wg = (uint32_t)std::min<uint64_t>((uint64_t)ncu * 4, (uint64_t)p->nblk + nhubt);
+            hipError_t e = hipFuncSetAttribute((const void *)pr_sweep_hotwg_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->hotwg_lds);
+            if (e != hipSuccess) st = hip_fail(e, "hipFuncSetAttribute", __FILE__, __LINE__);
+        }
+        if (p->persistent == 2) {
+            // one 1024-thread workgroup per CU: H cached layout values + a 2W-float
+            // staging slice per wavefront + static LDS (bounded by kStaticLds)
+            const size_t cap = std::min(lds_cu, lds_wg);
+            const size_t stage = (size_t)kWaveWaves * 8 * W;
+            uint32_t H = p->hot > 0 ? std::min<uint32_t>((uint32_t)p->hot, p->A) / 1024 * 1024 : 0;
+            if (W != 256) {
+                st = fail(GCHI_ERR_INVALID, "pr_plan_create: the wavefront-tiled sweep needs edges_per_block 256");
+            } else if (stage + kStaticLds > cap) {
+                st = fail(GCHI_ERR_INVALID, "pr_plan_create: edges_per_block too large for the wavefront-tiled sweep");
+            } else {
+                const size_t hmax = (cap - stage - kStaticLds) / 4 / 1024 * 1024;
+                H = std::min<uint32_t>(H, (uint32_t)hmax);
+                p->H = H;
+                p->wave_lds = stage + 4 * (size_t)H;
+                p->nwave_wg = (uint32_t)ncu;
+                const void *fn = H ? (const void *)pr_sweep_wave_kernel<kWaveThreads, 8, true>
+                                   : (const void *)pr_sweep_wave_kernel<kWaveThreads, 8, false>;
+                hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->wave_lds);
+                if (e != hipSuccess) st = hip_fail(e, "hipFuncSetAttribute", __FILE__, __LINE__);
+            }
+        }
     }
     if (st != GCHI_OK) {
         if (p->mem) (void)hipFree(p->mem);
@@ -701,14 +1285,32 @@ int pr_init(const gchi_pr_plan *p, float init, const uint32_t *seed_bits, float 
 int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
              float *r_out, double *d_stats, hipStream_t s) {
     PrConsts c{op->teleport, op->damping, op->d_graph_teleport, op->d_seed_bits, op->seed_teleport};
-    PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->hub_bits, p->chunk_beg, p->chunk_len, p->hub_src,
-             p->outdeg, x_in, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + 1, p->nchunk,
-             p->nchunk + p->nblk, p->short_max};
+    const float *xg = x_in;
+    if (p->hot) {
+        if (p->A)
+            hipLaunchKernelGGL(pr_gather_layout_kernel, dim3(grid_for(p->A, 256, 65536)), dim3(256), 0, s, p->gorder,
+                               p->A, x_in, p->xg);
+        xg = p->xg;
+    }
+    const uint32_t nhubt = (p->nchunk + kPrWaves - 1) / kPrWaves;
+    PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->tile_e0, p->hub_bits, p->chunk_beg, p->chunk_len,
+             p->hub_src,
+             p->outdeg, xg, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + 1, p->nchunk, nhubt,
+             p->nblk, nhubt + p->nblk, p->short_max};
     const size_t shmem = (size_t)2 * p->W * sizeof(float);
-    if (p->persistent)
+    if (p->persistent == 3) {
+        hipLaunchKernelGGL(pr_sweep_hotwg_kernel, dim3(p->nhot_wg), dim3(kPrThreads), p->hotwg_lds, s, a, c, p->H);
+    } else if (p->persistent == 2) {
+        const dim3 grid(p->nwave_wg), blk(kWaveThreads);
+        if (p->H)
+            hipLaunchKernelGGL((pr_sweep_wave_kernel<kWaveThreads, 8, true>), grid, blk, p->wave_lds, s, a, c, p->H);
+        else
+            hipLaunchKernelGGL((pr_sweep_wave_kernel<kWaveThreads, 8, false>), grid, blk, p->wave_lds, s, a, c, 0u);
+    } else if (p->persistent == 1) {
         hipLaunchKernelGGL(pr_sweep_persistent_kernel, dim3(p->npersist), dim3(kPrThreads), shmem, s, a, c);
-    else
-        hipLaunchKernelGGL(pr_sweep_kernel, dim3(p->nchunk + p->nblk), dim3(kPrThreads), shmem, s, a, c);
+    } else {
+        hipLaunchKernelGGL(pr_sweep_kernel, dim3(nhubt + p->nblk), dim3(kPrThreads), shmem, s, a, c);
+    }
     if (p->G == 1) {
         hipLaunchKernelGGL(pr_finalize_kernel, dim3(p->nfin), dim3(kFinThreads), 0, s, p->hub_v, p->hub_chunk_off,
                            p->chunk_part, p->outdeg, r_in, r_out, x_out, p->blk_stat, p->nblk, p->nhub, p->fin_stat,
@@ -731,8 +1333,11 @@ struct PlanView {
         *outdeg;
 };
 
+// Row structure of a plan whose edges hold vertex ids (no gather layout).
 int pr_plan_view(const gchi_pr_plan *p, PlanView *v) {
     GCHI_CHECK(p && v, GCHI_ERR_INVALID, "plan view: null argument");
+    GCHI_CHECK(!p->hot, GCHI_ERR_UNSUPPORTED, "plan view: the plan's edges hold gather-layout positions "
+                                              "(hot_vertices != 0); build it with hot_vertices = 0");
     *v = PlanView{p->N, p->nblk, p->nhub, p->nchunk, p->npersist, p->reg_off, p->reg_src, p->blk_row, p->hub_bits,
                   p->hub_v, p->hub_chunk_off, p->chunk_beg, p->chunk_len, p->hub_src, p->outdeg};
     return GCHI_OK;
@@ -773,7 +1378,13 @@ extern "C" int gchi_pr_plan_info_get(const gchi_pr_plan *p, gchi_pr_plan_info *i
     info->hub_edges = p->e_hub;
     info->ngraphs = p->G;
     info->persistent = p->persistent;
-    info->persistent_workgroups = p->npersist;
+    info->persistent_workgroups = p->persistent == 3   ? p->nhot_wg
+                                  : p->persistent == 2 ? p->nwave_wg
+                                  : p->persistent      ? p->npersist
+                                                       : 0u;
+    info->hot_vertices = p->H;
+    info->gather_sources = p->A;
+    info->hot_threads = p->persistent == 2 ? (uint32_t)kWaveThreads : p->persistent == 3 ? (uint32_t)kPrThreads : 0u;
     return GCHI_OK;
 }
 

@@ -16,7 +16,9 @@
 //       global memory when longer than 1024), the (b, w in N+(b)) pairs of 64
 //       consecutive b are flattened over the 64 lanes, and each lane
 //       binary-searches its w in N+(a).  Counts: a, b and w each get +1 per
-//       triangle (uint64 atomics), converted to uint32 with an overflow flag.
+//       triangle, gathered per wavefront (a: one sum per vertex, b: one per
+//       (a,b) pair, w: an LDS counter per position of N+(a)) and flushed with
+//       uint64 atomics, converted to uint32 with an overflow flag.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -25,8 +27,15 @@ namespace gchi {
 constexpr int kTcWaves = 4;
 constexpr int kTcLds = 1024;
 
-__global__ void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, uint32_t *deg,
-                           uint32_t *err) {
+// in+out degree; ids below kTcPriv count in LDS per workgroup first (skewed
+// graphs would otherwise serialise on a few global counters)
+constexpr uint32_t kTcPriv = 8192;
+
+__global__ __launch_bounds__(256) void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n,
+                                                  uint32_t *deg, uint32_t *err) {
+    __shared__ uint32_t s_deg[kTcPriv];
+    for (uint32_t i = threadIdx.x; i < kTcPriv; i += 256) s_deg[i] = 0u;
+    __syncthreads();
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
         uint32_t s = src[i], d = dst[i];
@@ -35,9 +44,15 @@ __global__ void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e,
             continue;
         }
         if (s == d) continue;
-        atomicAdd(&deg[s], 1u);
-        atomicAdd(&deg[d], 1u);
+        if (s < kTcPriv) atomicAdd(&s_deg[s], 1u);
+        else atomicAdd(&deg[s], 1u);
+        if (d < kTcPriv) atomicAdd(&s_deg[d], 1u);
+        else atomicAdd(&deg[d], 1u);
     }
+    __syncthreads();
+    const uint32_t lim = n < kTcPriv ? n : kTcPriv;
+    for (uint32_t v = threadIdx.x; v < lim; v += 256)
+        if (s_deg[v]) atomicAdd(&deg[v], s_deg[v]);
 }
 
 __global__ void tc_degree_keys(const uint32_t *deg, uint32_t n, int idb, uint64_t *keys) {
@@ -87,17 +102,18 @@ __global__ void tc_scatter_unique(const uint64_t *keys, const uint32_t *flags, c
     }
 }
 
-GCHI_DEV bool bsearch_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t x) {
+// Position of x in the ascending a[lo, hi), or -1.
+GCHI_DEV int bsearch_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t x) {
     while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
         uint32_t v = a[mid];
-        if (v == x) return true;
+        if (v == x) return (int)mid;
         if (v < x)
             lo = mid + 1;
         else
             hi = mid;
     }
-    return false;
+    return -1;
 }
 
 __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_t *__restrict__ off,
@@ -105,6 +121,7 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
                                                                    unsigned long long *cnt,
                                                                    unsigned long long *total, uint32_t *work) {
     __shared__ uint32_t sA[kTcWaves][kTcLds];
+    __shared__ uint32_t sW[kTcWaves][kTcLds];  // triangles found per position of N+(a) (the w credit)
     __shared__ uint32_t sPre[kTcWaves][kWave];
     __shared__ uint32_t sBeg[kTcWaves][kWave];
     __shared__ uint32_t sCnt[kTcWaves][kWave];
@@ -119,7 +136,10 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
         const uint32_t *A = adj + ab;
         const bool lds = da <= (uint32_t)kTcLds;
         if (lds)
-            for (uint32_t k = lane; k < da; k += kWave) sA[wave][k] = A[k];
+            for (uint32_t k = lane; k < da; k += kWave) {
+                sA[wave][k] = A[k];
+                sW[wave][k] = 0u;
+            }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const uint32_t *As = lds ? sA[wave] : A;
@@ -155,8 +175,12 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
                 const uint32_t excl = j ? sPre[wave][j - 1] : 0u;
                 const uint32_t w = adj[sBeg[wave][j] + (p - excl)];
                 // w > b_j > a; N+(a) is ascending, b_j sits at index bc + j
-                if (bsearch_u32(As, bc + j + 1, da, w)) {
-                    atomicAdd(&cnt[w], 1ull);
+                const int kw = bsearch_u32(As, bc + j + 1, da, w);
+                if (kw >= 0) {
+                    if (lds)
+                        atomicAdd(&sW[wave][kw], 1u);
+                    else
+                        atomicAdd(&cnt[w], 1ull);
                     atomicAdd(&sCnt[wave][j], 1u);
                 }
             }
@@ -167,11 +191,20 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
             acc_a += c;
             __builtin_amdgcn_wave_barrier();
         }
+        if (lds) {  // one global atomic per w of N+(a) that closed a triangle
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t k = lane; k < da; k += kWave) {
+                const uint32_t cw = sW[wave][k];
+                if (cw) atomicAdd(&cnt[sA[wave][k]], (unsigned long long)cw);
+            }
+        }
         acc_a = wave_sum(acc_a);
         if (lane == 0 && acc_a) {
             atomicAdd(&cnt[a], acc_a);
             atomicAdd(total, acc_a);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -242,7 +275,7 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
     launch_fill_u32(deg, 0u, N, s);
     launch_fill_u32(err, 0u, 1, s);
     if (E)
-        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N, deg, err);
+        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(E, 256 * 64, 2048)), dim3(256), 0, s, d_src, d_dst, E, N, deg, err);
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();
     GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
@@ -355,7 +388,7 @@ extern "C" int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, u
     launch_fill_u32(deg, 0u, N, s);
     launch_fill_u32(err, 0u, 1, s);
     if (nedges)
-        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(nedges, 256, 65536)), dim3(256), 0, s, d_src, d_dst, nedges, N,
+        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(nedges, 256 * 64, 2048)), dim3(256), 0, s, d_src, d_dst, nedges, N,
                            deg, err);
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();

@@ -39,14 +39,15 @@ class PrOp(ctypes.Structure):
 
 class PrTune(ctypes.Structure):
     _fields_ = [("edges_per_block", c_i32), ("short_row_max", c_i32),
-                ("persistent", c_i32), ("reserved", c_i32)]
+                ("persistent", c_i32), ("hot_vertices", c_i32)]
 
 
 class PrPlanInfo(ctypes.Structure):
     _fields_ = [("nvertices", c_u32), ("edges_per_block", c_u32), ("nedges", c_u64),
                 ("short_row_max", c_u32), ("regular_blocks", c_u32), ("hub_rows", c_u32),
                 ("hub_chunks", c_u32), ("regular_edges", c_u64), ("hub_edges", c_u64),
-                ("ngraphs", c_u32), ("persistent", c_u32), ("persistent_workgroups", c_u32), ("reserved", c_u32)]
+                ("ngraphs", c_u32), ("persistent", c_u32), ("persistent_workgroups", c_u32), ("hot_vertices", c_u32),
+                ("gather_sources", c_u32), ("hot_threads", c_u32)]
 
 
 class PprConfig(ctypes.Structure):
@@ -303,10 +304,10 @@ def graph_from_csc(in_off, in_src, outdeg, indeg, n, e):
 
 
 class PageRankPlan:
-    def __init__(self, graph, edges_per_block=0, short_row_max=0, persistent=0, vbase=None):
+    def __init__(self, graph, edges_per_block=0, short_row_max=0, persistent=0, vbase=None, hot_vertices=0):
         self.graph = graph
         self.handle = c_vp()
-        tune = PrTune(edges_per_block, short_row_max, persistent, 0)
+        tune = PrTune(edges_per_block, short_row_max, persistent, hot_vertices)
         if vbase is None:
             self.ngraphs = 1
             check(lib().gchi_pr_plan_create(ctypes.byref(graph.dg), ctypes.byref(tune), ctypes.byref(self.handle),
@@ -594,12 +595,12 @@ def union_of(graphs_list):
 def autotune_pr(graph, sweeps=4, report=False):
     """Best gchi_pr_tune for the graph -> (dict, ms per sweep, [(dict, ms)...])."""
     best, ms, n = PrTune(), ctypes.c_double(), c_i32()
-    tried = (PrTune * 64)()
-    tms = (ctypes.c_double * 64)()
+    tried = (PrTune * 128)()
+    tms = (ctypes.c_double * 128)()
     check(lib().gchi_autotune_pr(ctypes.byref(graph.dg), sweeps, ctypes.byref(best), ctypes.byref(ms),
                                  ctypes.cast(tried, c_vp) if report else None,
-                                 ctypes.cast(tms, c_vp) if report else None, 64, ctypes.byref(n), stream()),
+                                 ctypes.cast(tms, c_vp) if report else None, 128, ctypes.byref(n), stream()),
           "autotune_pr")
     asd = lambda t: {"edges_per_block": t.edges_per_block, "short_row_max": t.short_row_max,  # noqa: E731
-                     "persistent": t.persistent}
+                     "persistent": t.persistent, "hot_vertices": t.hot_vertices}
     return asd(best), ms.value, [(asd(tried[i]), tms[i]) for i in range(n.value)] if report else []

@@ -140,9 +140,21 @@ typedef struct gchi_pr_tune {
     int edges_per_block;   /* W: multiple of 256 in [256,4096]; 0 = 2048.  Rows with
                               in-degree >= W are hub rows, cut into 4096-edge chunks */
     int short_row_max;     /* rows up to this length use one lane per row; 0 = 32 */
-    int persistent;        /* 1: persistent workgroups pull tiles from a device work
-                              counter; 0: one workgroup per tile */
-    int reserved;
+    int persistent;        /* 0: one workgroup per tile; 1: persistent workgroups pull
+                              tiles from a device work counter; 2: one 1024-thread
+                              workgroup per CU whose wavefronts each work through a
+                              fixed list of tiles, software-pipelined (W = 256),
+                              optionally with the LDS cache of hot_vertices; 3: four
+                              persistent 256-thread workgroups per CU with a static
+                              tile list, each with its own (smaller) LDS cache */
+    int hot_vertices;      /* 0: gathers read x by vertex id.  Otherwise the plan keeps a
+                              gather layout: sources ranked by out-degree (descending,
+                              ties by id), edges store the rank, and every sweep first
+                              copies x into that order.  > 0: persistent workgroups also
+                              keep the first hot_vertices values of the layout in LDS
+                              (rounded down to a multiple of 1024, capped by LDS;
+                              implies persistent = 2 unless 3 is given); -1: the
+                              layout without the LDS cache.  N < 2^30. */
 } gchi_pr_tune;
 
 /* Sweep plan: the graph's in-edges regrouped into regular rows (LDS-staged
@@ -164,7 +176,9 @@ typedef struct gchi_pr_plan_info {
     uint32_t ngraphs;
     uint32_t persistent;
     uint32_t persistent_workgroups;
-    uint32_t reserved;
+    uint32_t hot_vertices;     /* values cached in LDS per workgroup (0: none) */
+    uint32_t gather_sources;   /* vertices in the gather layout (0: no layout) */
+    uint32_t hot_threads;      /* workgroup size of the LDS-cached sweep */
 } gchi_pr_plan_info;
 
 int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream);
@@ -186,7 +200,8 @@ int gchi_pr_init_seeds(const gchi_pr_plan *plan, const uint32_t *d_seed_bits, fl
                        float *d_r, void *stream);
 
 /* One synchronous gather->apply->scatter sweep (functional_bulksync.hpp:91-113,
- * functional_engine.hpp:50-121): two kernels (sweep, finalize).
+ * functional_engine.hpp:50-121): two kernels (sweep, finalize), three with a
+ * gather layout (its refresh from d_x_in comes first).
  * d_stats[0] = sum |r_out - r_in| (FP64), d_stats[1] = sum r_out (FP64), both
  * reduced in a fixed order (the log_change deltas of
  * src/api/graphchi_context.hpp:78-102). */
@@ -210,7 +225,9 @@ int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
 int gchi_pr_exec_destroy(gchi_pr_exec *exec);
 
 /* Autotuner: times every candidate launch shape (edges_per_block in
- * {512..4096}, short_row_max in {8..64}, persistent or not) on the caller's
+ * {512..4096}, short_row_max in {8..64}, persistent 0/1, the
+ * wavefront-tiled sweep, W = 256, with hot_vertices in {0, -1, 16384, max},
+ * and the cached persistent workgroups, W in {1024, 2048}) on the caller's
  * graph with HIP events (sweeps_per_candidate sweeps after a warm-up) and
  * returns the fastest; cached per (N, E) for the process unless h_tried is
  * given (then every candidate is re-timed and reported). */

@@ -115,3 +115,102 @@ def test_pagerank_scale22_vs_fp64(dev):
           f"L1={err.sum():.3e} L1_rel={err.sum() / r64.sum():.3e} (oracle {time.time() - t:.1f}s)")
     assert (err <= 5e-5 * r64).all()
     assert err.sum() <= 5e-6 * r64.sum()
+
+
+HOT_TUNES = [dict(hot_vertices=-1), dict(hot_vertices=-1, persistent=1),
+             dict(hot_vertices=1024, edges_per_block=256), dict(hot_vertices=1 << 20, edges_per_block=256),
+             dict(hot_vertices=4096, edges_per_block=256, short_row_max=4),
+             dict(hot_vertices=2048, edges_per_block=256, short_row_max=64),
+             dict(persistent=2, edges_per_block=256), dict(persistent=2, hot_vertices=-1, edges_per_block=256),
+             dict(persistent=3, hot_vertices=1 << 20), dict(persistent=3, hot_vertices=2048, edges_per_block=256,
+                                                              short_row_max=4)]
+
+
+@pytest.mark.parametrize("hot", HOT_TUNES, ids=["layout", "layout_pers", "hot1k", "hot_max", "hot_s4",
+                                                "hot_s64", "wave", "wave_layout", "hotwg", "hotwg_W256"])
+def test_gather_layout_bitwise_equal(dev, golden, hot):
+    """The gather layout, its LDS cache and the wavefront-tiled launch change
+    where values are read from and which wavefront sums a row, never which
+    values are summed or in what order: ranks are bit-identical to one
+    workgroup per tile in vertex-id layout with the same W and short-row
+    limit (FP64 statistics equal to 1e-12: their grouping follows the launch)."""
+    case = next(c for c in graphs.PR_CASES if c["name"] == "pr_rmat16")
+    src, dst, n = graphs.make(case)
+    base = {k: v for k, v in hot.items() if k in ("edges_per_block", "short_row_max")}
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)
+    outs = []
+    for tune in (base, hot):
+        plan = dev.PageRankPlan(g, **tune)
+        ex = dev.PageRankExec(plan, dev.pr_op(0.15 / n, DAMPING), 1.0, 9)
+        ex.launch()
+        outs.append((ex.ranks().cpu().numpy().copy(), ex.stats.cpu().numpy().copy(), plan.info()))
+    info = outs[1][2]
+    nsrc = int(np.count_nonzero(np.bincount(src, minlength=n)))
+    hv = hot.get("hot_vertices", 0)
+    assert info["gather_sources"] == (nsrc if hv else 0)
+    if hv > 0:
+        h = info["hot_vertices"]
+        assert 0 < h <= min(hv, nsrc) and h % 1024 == 0
+        assert (info["persistent"], info["hot_threads"]) in ((2, 1024), (3, 256))
+    else:
+        assert info["hot_vertices"] == 0
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-12)
+    # and the reference's own output still holds
+    assert_close(outs[1][0], golden[f"{case['name']}__pr_norm"])
+
+
+def test_gather_layout_rejects_inconsistent_outdeg(dev):
+    """A CSC whose out-degrees do not cover its in-edge sources cannot be put
+    in the gather layout; plan creation fails instead of gathering out of bounds."""
+    import torch
+    src = np.array([0, 1, 2, 3], np.uint32)
+    dst = np.array([1, 2, 3, 0], np.uint32)
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), 4)
+    bad = dev.graph_from_csc(g.in_off, g.in_src, torch.zeros_like(g.outdeg), g.indeg, 4, 4)
+    with pytest.raises(dev.GchiError):
+        dev.PageRankPlan(bad, hot_vertices=-1)
+    dev.PageRankPlan(g, hot_vertices=-1)  # the consistent graph plans fine
+
+
+def test_gather_layout_scale22_matches_vertex_layout(dev):
+    """At the bench size (about 68 regular blocks per wavefront, so several
+    descriptor batches each) the LDS-cached sweep gives the same bits as the
+    vertex-id layout with the same block shape."""
+    s, d, cnt, mx = dev.rmat_device(22, 16 << 22, 2)
+    n = mx + 1
+    g = dev.DeviceGraph(s, d, n)
+    outs = []
+    for tune in (dict(edges_per_block=256, short_row_max=32), dict(edges_per_block=256, short_row_max=32,
+                                                                    hot_vertices=1 << 20)):
+        plan = dev.PageRankPlan(g, **tune)
+        ex = dev.PageRankExec(plan, dev.pr_op(0.15, DAMPING), 1.0, 20)
+        ex.launch()
+        outs.append(ex.ranks().cpu().numpy().copy())
+        print("\n", tune, plan.info())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_wave_sweep_long_blocks(dev):
+    """Blocks of more than 64 rows (runs of rows without in-edges) and skewed
+    sources, in the wavefront-tiled sweep with and without the LDS cache:
+    bit-identical to one workgroup per tile.  (More blocks per wavefront than
+    one descriptor batch of 64 is covered at scale 22 below.)"""
+    rng = np.random.default_rng(11)
+    n = 1 << 21
+    e = 6 << 20
+    src = (rng.zipf(1.6, e) % n).astype(np.uint32)
+    dst = rng.integers(0, n // 2, e).astype(np.uint32)  # upper half: no in-edges
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)
+    outs = []
+    for tune in (dict(edges_per_block=256, short_row_max=16), dict(edges_per_block=256, short_row_max=16,
+                                                                    persistent=2),
+                 dict(edges_per_block=256, short_row_max=16, hot_vertices=1 << 20)):
+        plan = dev.PageRankPlan(g, **tune)
+        ex = dev.PageRankExec(plan, dev.pr_op(0.15 / n, DAMPING), 1.0, 6)
+        ex.launch()
+        outs.append(ex.ranks().cpu().numpy().copy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])

@@ -68,3 +68,17 @@ def test_batched_ppr_matches_plain(dev):
     for q, seeds in enumerate(queries):
         rp, _ = plain.run(seeds, TOL)
         assert np.abs(b.ranks(q).cpu().numpy().astype(np.float64) - rp.cpu().numpy()).sum() <= 2 * TOL
+
+
+def test_ppr_with_gather_layout_plan(dev):
+    """Plain PPR over a gather-layout plan gives the vertex-layout bits; the
+    batched form needs vertex ids in the plan's edges and refuses such a plan."""
+    case = next(c for c in graphs.PR_CASES if c.get("ppr_seeds"))
+    src, dst, n, g, plan = graph_and_plan(dev, case, edges_per_block=256)
+    hot = dev.PageRankPlan(g, hot_vertices=1024, edges_per_block=256)
+    a, sa = dev.PlainPpr(plan, D).run(case["ppr_seeds"], TOL)
+    b, sb = dev.PlainPpr(hot, D).run(case["ppr_seeds"], TOL)
+    assert sa == sb
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    with pytest.raises(dev.GchiError):
+        dev.PprBatch(hot, 64)
