@@ -64,7 +64,7 @@ def oracle_mod():
 def bench_pagerank(gchi, torch, args, src, dst, E, N):
     g = gchi.DeviceGraph(src, dst, N)
     tune = {"edges_per_block": args.edges_per_block, "short_row_max": args.short_row_max,
-            "persistent": args.persistent}
+            "persistent": args.persistent, "hot_vertices": args.hot_vertices}
     tuned = None
     if args.autotune and not args.edges_per_block:
         best, best_ms, tried = gchi.autotune_pr(g, sweeps=4, report=True)
@@ -96,6 +96,15 @@ def bench_pagerank(gchi, torch, args, src, dst, E, N):
     log(f"timed {args.steps} sweeps: {ms:.4f} ms/sweep (wall {wall * 1e3:.2f} ms); last residual "
         f"{stats[-1, 0]:.3e}, rank sum {stats[-1, 1]:.6e}")
     return g, plan, info, ms, tuned, stats
+
+
+def sweep_kernels(info):
+    """Kernel names of one sweep of the plan (what rocprofv3 lists)."""
+    main = {0: "pr_sweep_kernel", 1: "pr_sweep_persistent_kernel", 2: "pr_sweep_wave_kernel",
+            3: "pr_sweep_hotwg_kernel"}[info["persistent"]]
+    ks = ([] if not info["gather_sources"] else ["pr_gather_layout_kernel"]) + [main]
+    ks += ["pr_finalize_kernel"] if info["ngraphs"] == 1 else ["pr_hub_apply_kernel", "pr_graph_reduce_kernel"]
+    return " + ".join(ks)
 
 
 # ------------------------------------------------------------------ PPR
@@ -219,54 +228,34 @@ def bench_tc(gchi, torch, scale):
 
 # ------------------------------------------------------------------ many graphs
 def bench_multigraph(gchi, torch, ngraphs, sweeps, seed=9):
-    """Many graphs of uneven size: each graph's CSC is built on its own, the CSCs
-    are concatenated into disjoint unions (a union holds < 2^32 edges), and
-    every union is swept by one persistent launch per sweep."""
+    """Many graphs of uneven size in one plan: each graph's CSC is built on its
+    own (local ids), the batch plan copies them side by side (64-bit offsets
+    across graphs, so the total may exceed 2^32 edges), and every sweep of all
+    graphs is one launch of the persistent sweep kernel plus the finalize pair."""
     rng = np.random.default_rng(seed)
     scales = rng.integers(10, 21, ngraphs)
     efs = rng.integers(8, 33, ngraphs)
-    unions, cur = [], {"off": [], "src": [], "od": [], "ind": [], "ns": [], "E": 0, "N": 0}
-
-    def close_union():
-        if not cur["ns"]:
-            return
-        offs = torch.cat(cur["off"] + [torch.tensor([cur["E"]], dtype=torch.int64, device="cuda")]).to(torch.int32)
-        g = gchi.graph_from_csc(offs, torch.cat(cur["src"]), torch.cat(cur["od"]), torch.cat(cur["ind"]),
-                                cur["N"], cur["E"])
-        vbase = np.concatenate([[0], np.cumsum(cur["ns"])]).astype(np.uint32)
-        unions.append((g, vbase, list(cur["ns"])))
-        for k in ("off", "src", "od", "ind", "ns"):
-            cur[k] = []
-        cur["E"] = cur["N"] = 0
-
+    graphs, ns = [], []
     for i in range(ngraphs):
         s, d, cnt, mx = gchi.rmat_device(int(scales[i]), int(efs[i]) << int(scales[i]), 10000 + i)
-        gi = gchi.DeviceGraph(s, d, mx + 1)
-        if cur["E"] + cnt >= (1 << 32) - 1:
-            close_union()
-        cur["off"].append(gi.in_off[:mx + 1].to(torch.int64) + cur["E"])
-        cur["src"].append(gi.in_src[:cnt] + cur["N"])
-        cur["od"].append(gi.outdeg[:mx + 1])
-        cur["ind"].append(gi.indeg[:mx + 1])
-        cur["ns"].append(mx + 1)
-        cur["E"] += cnt
-        cur["N"] += mx + 1
-        del s, d, gi
-    close_union()
-    execs, E_total = [], 0
-    for g, vbase, ns in unions:
-        tele = torch.tensor([0.15 / n for n in ns], dtype=torch.float64, device="cuda")
-        op = gchi.pr_op(0.15, 0.85, graph_teleport=tele)
-        plan = gchi.PageRankPlan(g, persistent=1, vbase=vbase)
-        ex = gchi.PageRankExec(plan, op, 1.0, sweeps)
-        execs.append((ex, plan, op, tele, g))
-        E_total += g.E
-    for ex in execs:
-        ex[0].launch()
-    _, t = timed(lambda: [ex[0].launch() for ex in execs], torch)
+        graphs.append(gchi.DeviceGraph(s, d, mx + 1))
+        ns.append(mx + 1)
+        del s, d
+    E_total = sum(g.E for g in graphs)
+    plan, t_plan = timed(lambda: gchi.PageRankPlan.batch(graphs, persistent=1), torch)
+    del graphs
+    torch.cuda.empty_cache()
+    tele = torch.tensor([0.15 / n for n in ns], dtype=torch.float64, device="cuda")
+    op = gchi.pr_op(0.15, 0.85, graph_teleport=tele)
+    ex = gchi.PageRankExec(plan, op, 1.0, sweeps)
+    ex.launch()
+    _, t = timed(ex.launch, torch)
+    info = plan.info()
     return {"workload": f"{ngraphs} directed R-MAT graphs, scales uniform 10-20, edge factor uniform 8-32, "
-                        f"{E_total} edges total in {len(unions)} union(s); kernel 0.15/N_g, {sweeps} sweeps",
-            "launches_per_sweep": 2 * len(unions),
+                        f"{E_total} edges, {info['nvertices']} vertices in one batch plan; kernel 0.15/N_g, "
+                        f"{sweeps} sweeps (init + sweeps replayed as one HIP graph)",
+            "launches_per_sweep": len(sweep_kernels(info).split(" + ")),
+            "sweep_kernels": sweep_kernels(info), "plan_build_s": round(t_plan, 3),
             "ms_per_sweep_all_graphs": round(t * 1e3 / sweeps, 3), "edges_per_s": round(E_total * sweeps / t, 1),
             "graphs_per_s": round(ngraphs * sweeps / t, 1)}
 
@@ -288,11 +277,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--cpu-sweeps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--edges-per-block", type=int, default=0)
     ap.add_argument("--short-row-max", type=int, default=0)
     ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--hot-vertices", type=int, default=0)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--only", default="", help="comma list of sections: pagerank,ppr,sort,cc,tc,multigraph")
     ap.add_argument("--ppr-queries", type=int, default=1024)
@@ -300,7 +290,7 @@ def main():
     ap.add_argument("--ppr-check", type=int, default=1)
     ap.add_argument("--cc-scale", type=int, default=24)
     ap.add_argument("--tc-scale", type=int, default=21)
-    ap.add_argument("--graphs", type=int, default=256)
+    ap.add_argument("--graphs", type=int, default=2048)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration for HBM bytes)")
     args = ap.parse_args()
@@ -333,7 +323,7 @@ def main():
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("pr_sweep_rmat22_traffic"),
-                "kernel": "pr_sweep_kernel (+ pr_finalize_kernel), one sweep", "algorithmic_bytes": algo_bytes}
+                "kernel": sweep_kernels(info) + ", one sweep (all launches)", "algorithmic_bytes": algo_bytes}
     out = {
         "metric": METRIC, "value": E / (ms * 1e-3), "unit": "edges/s/iter", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",

@@ -12,9 +12,13 @@
 // reads exactly what the reference's gather reads.
 //
 // Layout (built once by gchi_pr_plan_create from the CSC):
-//   regular rows (in-degree < W): reg_off[N+1], reg_src[E_reg]; a row's "work"
-//     is deg+1, and block b owns rows with reg_off[v]+v in [b*W, (b+1)*W), so a
-//     block sees < 2W edges and < 2W rows.  Its in-edges are streamed coalesced
+//   regular rows (in-degree < W): reg_off[N+G] (graph g's rows at slots
+//     v+g, relative to the graph's first regular edge ebase[g], plus one end
+//     slot per graph), reg_src[E_reg]; a row's "work" is deg+1, and block b
+//     owns rows with reg_off[v+g]+(v-vbase[g]) in [b*W, (b+1)*W) of its graph,
+//     so a block sees < 2W edges and < 2W rows.  Offsets are 32-bit per graph
+//     and 64-bit across graphs, so a batch of graphs may hold more than 2^32
+//     edges in total.  Its in-edges are streamed coalesced
 //     (non-temporal), x[src] gathered into LDS, then rows are reduced: one lane
 //     per row up to short_row_max edges (sequential adds in src order, as the
 //     reference's gather), one wavefront per longer row.
@@ -58,11 +62,14 @@ struct gchi_pr_plan {
     uint32_t *hub_bits = nullptr;
     uint32_t *blk_row = nullptr;
     uint32_t *blk_graph = nullptr;
-    uint32_t *tile_e0 = nullptr;   // [nblk+1] first regular edge of each block
+    uint32_t *tile_e0 = nullptr;   // [nblk+1] first regular edge of each block (relative to its graph)
+    uint32_t *tile_e1 = nullptr;   // [nblk+1] end regular edge of each block (relative to its graph)
+    uint64_t *ebase = nullptr;     // [G] first regular edge of each graph in reg_src
+    uint32_t *own_outdeg = nullptr;  // out-degrees copied from per-graph CSCs (batch plans)
     uint32_t *hub_v = nullptr;
     uint32_t *hub_graph = nullptr;
     uint32_t *hub_chunk_off = nullptr;
-    uint32_t *chunk_beg = nullptr;
+    uint64_t *chunk_beg = nullptr;
     uint32_t *chunk_len = nullptr;
     uint32_t *hub_src = nullptr;
     uint32_t *gblk = nullptr;  // [G+1] block range per graph
@@ -106,8 +113,9 @@ struct PrConsts {
 };
 
 struct PrArgs {
-    const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *tile_e0, *hub_bits, *chunk_beg, *chunk_len, *hub_src,
-        *outdeg;
+    const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *tile_e0, *tile_e1, *hub_bits;
+    const uint64_t *ebase, *chunk_beg;
+    const uint32_t *chunk_len, *hub_src, *outdeg;
     const float *xg;  // gather source: x by vertex id, or the gather layout
     const float *r_in;
     float *x_out, *r_out, *chunk_part;
@@ -160,7 +168,8 @@ GCHI_DEV void wave_lds_sync() {
 template <bool HOT>
 GCHI_DEV float hub_chunk_sum(const PrArgs &a, uint32_t c, const float *hot, uint32_t H, int lane) {
     constexpr int PER = kHubChunk / kWave;
-    const uint32_t cb = a.chunk_beg[c], cl = a.chunk_len[c];
+    const uint64_t cb = a.chunk_beg[c];
+    const uint32_t cl = a.chunk_len[c];
     // Every load is issued unconditionally at an in-bounds address (lanes past
     // the chunk read its first edge and discard it), so the compiler emits no
     // branch around loads and its wait counts stay exact.
@@ -208,15 +217,17 @@ GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *
     const uint32_t b = tile - a.nhubt;
     const uint32_t rb = a.blk_row[b], re = a.blk_row[b + 1];
     const uint32_t g = a.blk_graph[b];
-    const uint32_t e0 = a.reg_off[rb], e1 = a.reg_off[re];
+    const uint32_t *ro = a.reg_off + g;  // the graph's offset slots: row v at ro[v]
+    const uint32_t e0 = a.tile_e0[b], e1 = a.tile_e1[b];
     const uint32_t span = e1 - e0;
+    const uint32_t *rs = a.reg_src + a.ebase[g] + e0;
     if (tid == 0) *s_nlong = 0;
     for (uint32_t base = 0; base < span; base += 8 * T) {
         uint32_t idx[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             uint32_t k = base + (uint32_t)j * T + tid;
-            idx[j] = k < span ? __builtin_nontemporal_load(&a.reg_src[e0 + k]) : 0u;
+            idx[j] = k < span ? __builtin_nontemporal_load(&rs[k]) : 0u;
         }
         float val[8];
 #pragma unroll
@@ -236,7 +247,7 @@ GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *
     double res = 0.0, sum = 0.0;
     for (uint32_t v = rb + tid; v < re; v += T) {
         if ((a.hub_bits[v >> 5] >> (v & 31)) & 1u) continue;
-        uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
+        uint32_t beg = ro[v] - e0, end = ro[v + 1] - e0;
         if ((int)(end - beg) <= a.short_max) {
             float s = 0.0f;
             for (uint32_t k = beg; k < end; k++) s += s_val[k];
@@ -250,7 +261,7 @@ GCHI_DEV void pr_tile(const PrArgs &a, const PrConsts &c, uint32_t tile, float *
     const uint32_t nlong = *s_nlong;
     for (uint32_t li = wave; li < nlong; li += NW) {
         uint32_t v = s_long[li];
-        uint32_t beg = a.reg_off[v] - e0, end = a.reg_off[v + 1] - e0;
+        uint32_t beg = ro[v] - e0, end = ro[v + 1] - e0;
         float s = 0.0f;
         for (uint32_t k = beg + lane; k < end; k += kWave) s += s_val[k];
         s = wave_sum(s);
@@ -336,8 +347,9 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_hotwg_kernel(PrArgs a, Pr
 // bit-identical to the workgroup-tiled kernels.
 
 struct TileDesc {
-    uint32_t rb, re, e0, e1, g;
-    double tele;  // the graph's constant term
+    uint32_t rb, re, e0, e1, g;  // e0/e1 relative to the graph's first regular edge
+    uint64_t eb;                 // that edge's position in reg_src
+    double tele;                 // the graph's constant term
 };
 
 // Row metadata of row v = rb + lane of a block (zero when v >= re).
@@ -350,7 +362,7 @@ struct RowMeta {
 struct DescBatch {
     uint32_t base;
     uint32_t rb, re, e0, e1, g;
-    uint32_t tlo, thi;
+    uint32_t tlo, thi, eblo, ebhi;
 };
 
 GCHI_DEV void load_desc_batch(const PrArgs &a, const PrConsts &c, uint32_t base, uint32_t bend, int lane,
@@ -360,12 +372,15 @@ GCHI_DEV void load_desc_batch(const PrArgs &a, const PrConsts &c, uint32_t base,
     d.rb = a.blk_row[b];
     d.re = a.blk_row[b + 1];
     d.e0 = a.tile_e0[b];
-    d.e1 = a.tile_e0[b + 1];
+    d.e1 = a.tile_e1[b];
     d.g = a.blk_graph[b];
     double t = c.teleport;
     if (c.g_tele) t = c.g_tele[d.g];
     d.tlo = (uint32_t)__double_as_longlong(t);
     d.thi = (uint32_t)(__double_as_longlong(t) >> 32);
+    const uint64_t eb = a.ebase[d.g];
+    d.eblo = (uint32_t)eb;
+    d.ebhi = (uint32_t)(eb >> 32);
 }
 
 GCHI_DEV TileDesc get_desc(const DescBatch &d, uint32_t b) {
@@ -380,6 +395,8 @@ GCHI_DEV TileDesc get_desc(const DescBatch &d, uint32_t b) {
     const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane(d.tlo, j);
     const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane(d.thi, j);
     t.tele = __longlong_as_double((long long)(lo | (hi << 32)));
+    t.eb = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.eblo, j) |
+           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.ebhi, j) << 32);
     return t;
 }
 
@@ -393,10 +410,11 @@ GCHI_DEV TileDesc get_desc(const DescBatch &d, uint32_t b) {
 template <int PER>
 GCHI_DEV void load_ids(const PrArgs &a, const TileDesc &t, int lane, uint32_t (&idx)[PER]) {
     const uint32_t span = t.e1 - t.e0;
+    const uint32_t *rs = a.reg_src + t.eb + t.e0;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
         const uint32_t k = (uint32_t)j * kWave + lane;
-        idx[j] = __builtin_nontemporal_load(&a.reg_src[k < span ? t.e0 + k : 0u]);
+        idx[j] = __builtin_nontemporal_load(k < span ? &rs[k] : a.reg_src);
     }
 }
 
@@ -431,8 +449,8 @@ GCHI_DEV void gather_ids(const PrArgs &a, const TileDesc &t, const float *hot, u
 GCHI_DEV void load_meta(const PrArgs &a, const PrConsts &c, const TileDesc &t, int lane, RowMeta &m) {
     const uint32_t v = t.rb + lane;
     const uint32_t w = v < t.re ? v : t.rb;  // rb < re: every block has a row
-    m.beg = a.reg_off[w];
-    m.end = a.reg_off[w + 1];
+    m.beg = a.reg_off[w + t.g];
+    m.end = a.reg_off[w + t.g + 1];
     m.hubw = a.hub_bits[w >> 5];
     m.seedw = (c.seed_bits ? c.seed_bits : a.hub_bits)[w >> 5];
     m.od = a.outdeg[w];
@@ -458,7 +476,8 @@ GCHI_DEV void finish_row_pf(uint32_t v, double tele, uint32_t seedw, const PrCon
 constexpr int kChunkPer = kHubChunk / kWave;
 
 GCHI_DEV void chunk_ids(const PrArgs &a, uint32_t ch, int lane, uint32_t (&idx)[kChunkPer]) {
-    const uint32_t cb = a.chunk_beg[ch], cl = a.chunk_len[ch];
+    const uint64_t cb = a.chunk_beg[ch];
+    const uint32_t cl = a.chunk_len[ch];
 #pragma unroll
     for (int j = 0; j < kChunkPer; j++) {
         const uint32_t k = (uint32_t)j * kWave + lane;
@@ -569,7 +588,7 @@ GCHI_DEV void wave_rows(const PrArgs &a, const PrConsts &c, uint32_t b, const Ti
         const uint32_t v = v0 + lane;
         bool is_long = false;
         if (v < t.re && !((a.hub_bits[v >> 5] >> (v & 31)) & 1u)) {
-            const uint32_t beg = a.reg_off[v] - t.e0, end = a.reg_off[v + 1] - t.e0;
+            const uint32_t beg = a.reg_off[v + t.g] - t.e0, end = a.reg_off[v + t.g + 1] - t.e0;
             if ((int)(end - beg) <= a.short_max) {
                 float s = 0.0f;
                 for (uint32_t k = beg; k < end; k++) s += st[k];
@@ -582,7 +601,7 @@ GCHI_DEV void wave_rows(const PrArgs &a, const PrConsts &c, uint32_t b, const Ti
         while (mask) {
             const uint32_t u = v0 + (uint32_t)__builtin_ctzll(mask);
             mask &= mask - 1;
-            const uint32_t beg = a.reg_off[u] - t.e0, end = a.reg_off[u + 1] - t.e0;
+            const uint32_t beg = a.reg_off[u + t.g] - t.e0, end = a.reg_off[u + t.g + 1] - t.e0;
             float s = 0.0f;
             for (uint32_t k = beg + lane; k < end; k += kWave) s += st[k];
             s = wave_sum(s);
@@ -843,11 +862,43 @@ GCHI_DEV uint32_t graph_of(const uint32_t *vbase, uint32_t G, uint32_t v) {
     return lo;
 }
 
-__global__ void plan_rows(const uint32_t *in_off, uint32_t N, int W, uint32_t *reg_deg, uint32_t *hub_flag,
-                          uint32_t *hub_bits) {
+// The plan build reads every graph's in-edges through one view: graph g owns
+// vertices [vbase[g], vbase[g+1]) and offset slots [vbase[g]+g, vbase[g+1]+g]
+// of off_ext, which hold its in-edge offsets relative to its first in-edge at
+// address g_src[g]; a stored source id plus g_idoff[g] is the plan vertex id.
+struct CscView {
+    const uint32_t *off_ext;  // [N+G]
+    const uint64_t *g_src;    // [G] device addresses of each graph's in_src
+    const uint32_t *g_idoff;  // [G] added to stored ids (vbase for per-graph ids, 0 for a union)
+    const uint32_t *vbase;    // [G+1]
+    uint32_t G, N;
+    int local_ids;            // ids are per graph (checked against its vertex count)
+};
+
+GCHI_DEV uint32_t view_deg(const CscView &cv, uint32_t v, uint32_t g) {
+    return cv.off_ext[v + g + 1] - cv.off_ext[v + g];
+}
+
+// off_ext of a union CSC (global offsets, global ids): slot v+g = in_off[v] -
+// in_off[vbase[g]] for v in [vbase[g], vbase[g+1]]; g_src[g] = in_src + in_off[vbase[g]].
+__global__ void plan_ext_union(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *vbase, uint32_t G,
+                               uint32_t N, uint32_t *off_ext, uint64_t *g_src, uint32_t *g_idoff) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) {
-        uint32_t d = in_off[v + 1] - in_off[v];
+        const uint32_t g = graph_of(vbase, G, v);
+        off_ext[v + g] = in_off[v] - in_off[vbase[g]];
+    }
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
+        off_ext[vbase[g + 1] + g] = in_off[vbase[g + 1]] - in_off[vbase[g]];
+        g_src[g] = (uint64_t)(uintptr_t)(in_src + in_off[vbase[g]]);
+        g_idoff[g] = 0u;
+    }
+}
+
+__global__ void plan_rows(CscView cv, int W, uint32_t *reg_deg, uint32_t *hub_flag, uint32_t *hub_bits) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < cv.N; v += stride) {
+        const uint32_t d = view_deg(cv, v, graph_of(cv.vbase, cv.G, v));
         bool hub = d >= (uint32_t)W;
         reg_deg[v] = hub ? 0u : d;
         hub_flag[v] = hub ? 1u : 0u;
@@ -855,13 +906,13 @@ __global__ void plan_rows(const uint32_t *in_off, uint32_t N, int W, uint32_t *r
     }
 }
 
-__global__ void plan_hubs(const uint32_t *in_off, uint32_t N, const uint32_t *hub_flag, const uint32_t *hub_idx,
-                          uint32_t *hub_v, uint32_t *hub_deg, uint32_t *hub_nchunk) {
+__global__ void plan_hubs(CscView cv, const uint32_t *hub_flag, const uint32_t *hub_idx, uint32_t *hub_v,
+                          uint32_t *hub_deg, uint32_t *hub_nchunk) {
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < cv.N; v += stride) {
         if (hub_flag[v]) {
             uint32_t h = hub_idx[v];
-            uint32_t d = in_off[v + 1] - in_off[v];
+            uint32_t d = view_deg(cv, v, graph_of(cv.vbase, cv.G, v));
             hub_v[h] = v;
             hub_deg[h] = d;
             hub_nchunk[h] = (d + kHubChunk - 1) / kHubChunk;
@@ -869,22 +920,43 @@ __global__ void plan_hubs(const uint32_t *in_off, uint32_t N, const uint32_t *hu
     }
 }
 
+// regular offsets relative to each graph (slot v+g, plus the end slot of each
+// graph) and each graph's first regular edge, from the global u64 scan reg64[N+1]
+__global__ void plan_reg_rel(const uint64_t *reg64, const uint32_t *vbase, uint32_t G, uint32_t N, uint32_t *reg_rel,
+                             uint64_t *ebase) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) {
+        const uint32_t g = graph_of(vbase, G, v);
+        reg_rel[v + g] = (uint32_t)(reg64[v] - reg64[vbase[g]]);
+    }
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
+        reg_rel[vbase[g + 1] + g] = (uint32_t)(reg64[vbase[g + 1]] - reg64[vbase[g]]);
+        ebase[g] = reg64[vbase[g]];
+    }
+}
+
 // one wavefront per row: copy the row's in-edges into the regular or hub array,
-// mapped to gather-layout positions when grank is given.  A source id >= N, or
-// one without a layout position (its out-degree says 0), sets *bad and is
-// stored as 0, so no sweep ever gathers out of bounds; the plan is refused.
-__global__ void plan_copy(const uint32_t *in_off, const uint32_t *in_src, uint32_t N, const uint32_t *hub_flag,
-                          const uint32_t *hub_idx, const uint32_t *reg_off, const uint32_t *hub_off,
-                          const uint32_t *grank, uint32_t *reg_src, uint32_t *hub_src, uint32_t *bad) {
+// mapped to gather-layout positions when grank is given.  A source id outside
+// its graph, or one without a layout position (its out-degree says 0), sets
+// *bad and is stored as 0, so no sweep ever gathers out of bounds; the plan is
+// refused.
+__global__ void plan_copy(CscView cv, const uint32_t *hub_flag, const uint32_t *hub_idx, const uint32_t *reg_rel,
+                          const uint64_t *ebase, const uint64_t *hub_off, const uint32_t *grank, uint32_t *reg_src,
+                          uint32_t *hub_src, uint32_t *bad) {
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
-    for (uint64_t v = wid; v < N; v += nw) {
-        uint32_t b = in_off[v], e = in_off[v + 1];
-        uint32_t *dst = hub_flag[v] ? hub_src + hub_off[hub_idx[v]] : reg_src + reg_off[v];
+    for (uint64_t v = wid; v < cv.N; v += nw) {
+        const uint32_t g = graph_of(cv.vbase, cv.G, (uint32_t)v);
+        const uint32_t b = cv.off_ext[v + g], e = cv.off_ext[v + g + 1];
+        const uint32_t *src = reinterpret_cast<const uint32_t *>((uintptr_t)cv.g_src[g]);
+        const uint32_t idoff = cv.g_idoff[g];
+        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.N;
+        uint32_t *dst = hub_flag[v] ? hub_src + hub_off[hub_idx[v]] : reg_src + ebase[g] + reg_rel[v + g];
         for (uint32_t k = b + lane; k < e; k += kWave) {
-            uint32_t u = in_src[k];
-            bool ok = u < N;
+            uint32_t u = src[k];
+            bool ok = u < lim;
+            u += idoff;
             if (ok && grank) {
                 u = grank[u];
                 ok = u != 0xffffffffu;
@@ -918,52 +990,62 @@ __global__ void plan_gather_rank(const uint64_t *keys, uint32_t N, uint32_t *ord
     }
 }
 
-__global__ void plan_chunks(const uint32_t *hub_off, const uint32_t *hub_chunk_off, uint32_t nhub, uint32_t *chunk_beg,
+__global__ void plan_chunks(const uint64_t *hub_off, const uint32_t *hub_chunk_off, uint32_t nhub, uint64_t *chunk_beg,
                             uint32_t *chunk_len) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhub; h += stride) {
-        uint32_t b = hub_off[h], d = hub_off[h + 1] - b;
+        const uint64_t b = hub_off[h];
+        const uint32_t d = (uint32_t)(hub_off[h + 1] - b);
         uint32_t c0 = hub_chunk_off[h], c1 = hub_chunk_off[h + 1];
         for (uint32_t c = c0; c < c1; c++) {
             uint32_t j = c - c0;
-            chunk_beg[c] = b + j * kHubChunk;
+            chunk_beg[c] = b + (uint64_t)j * kHubChunk;
             chunk_len[c] = min((uint32_t)kHubChunk, d - j * kHubChunk);
         }
     }
 }
 
 // blocks per graph: the last row's work position within its graph / W + 1
-__global__ void plan_graph_blocks(const uint32_t *reg_off, const uint32_t *vbase, uint32_t G, int W,
+__global__ void plan_graph_blocks(const uint32_t *reg_rel, const uint32_t *vbase, uint32_t G, int W,
                                   uint32_t *nblk_g) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride) {
         uint32_t v0 = vbase[g], v1 = vbase[g + 1];
-        uint64_t p_last = (uint64_t)(reg_off[v1 - 1] - reg_off[v0]) + (v1 - 1 - v0);
+        uint64_t p_last = (uint64_t)reg_rel[v1 - 1 + g] + (v1 - 1 - v0);
         nblk_g[g] = (uint32_t)(p_last / (uint64_t)W + 1);
     }
 }
 
-__global__ void plan_blocks(const uint32_t *reg_off, const uint32_t *vbase, uint32_t G, const uint32_t *gblk,
+__global__ void plan_blocks(const uint32_t *reg_rel, const uint32_t *vbase, uint32_t G, const uint32_t *gblk,
                             uint32_t N, int W, uint32_t nblk, uint32_t *blk_row, uint32_t *blk_graph) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) {
         const uint32_t g = graph_of(vbase, G, v);
         const uint32_t v0 = vbase[g];
         const uint64_t base = gblk[g];
-        const uint64_t bv = base + ((uint64_t)(reg_off[v] - reg_off[v0]) + (v - v0)) / (uint64_t)W;
+        const uint64_t bv = base + ((uint64_t)reg_rel[v + g] + (v - v0)) / (uint64_t)W;
         int64_t bp = (int64_t)base - 1;
-        if (v > v0) bp = (int64_t)(base + ((uint64_t)(reg_off[v - 1] - reg_off[v0]) + (v - 1 - v0)) / (uint64_t)W);
+        if (v > v0) bp = (int64_t)(base + ((uint64_t)reg_rel[v - 1 + g] + (v - 1 - v0)) / (uint64_t)W);
         for (int64_t bb = bp + 1; bb <= (int64_t)bv; bb++) {
             blk_row[bb] = v;
             blk_graph[bb] = g;
         }
-        if (v == N - 1) blk_row[nblk] = N;
+        if (v == N - 1) {
+            blk_row[nblk] = N;
+            blk_graph[nblk] = G - 1;
+        }
     }
 }
 
-__global__ void plan_tile_e0(const uint32_t *reg_off, const uint32_t *blk_row, uint32_t nblk, uint32_t *tile_e0) {
+// first and end regular edge of every block, relative to its graph
+__global__ void plan_tile_edges(const uint32_t *reg_rel, const uint32_t *blk_row, const uint32_t *blk_graph,
+                                uint32_t nblk, uint32_t *tile_e0, uint32_t *tile_e1) {
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nblk; b += stride) tile_e0[b] = reg_off[blk_row[b]];
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
+        const uint32_t g = blk_graph[b];
+        tile_e0[b] = reg_rel[blk_row[b] + g];
+        tile_e1[b] = reg_rel[blk_row[b + 1] + g];
+    }
 }
 
 __global__ void plan_hub_graphs(const uint32_t *hub_v, uint32_t nhub, const uint32_t *vbase, uint32_t G,
@@ -1009,61 +1091,113 @@ static int read_u32(const uint32_t *d, uint32_t *h, hipStream_t s) {
     return GCHI_OK;
 }
 
-static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi_pr_plan *p, hipStream_t s) {
-    const uint32_t N = g->nvertices;
-    const uint64_t E = g->nedges;
-    const uint32_t G = p->G;
+static int read_u64(const uint64_t *d, uint64_t *h, hipStream_t s) {
+    GCHI_HIP(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    return GCHI_OK;
+}
+
+// The graphs a plan is built from: one CSC whose vertex ranges are the graphs
+// (global ids and offsets), or one CSC per graph (local ids and offsets).
+struct PlanInput {
+    uint32_t N = 0, G = 1;
+    uint64_t E = 0;
+    const uint32_t *h_vbase = nullptr;  // [G+1]
+    const gchi_dgraph *unionc = nullptr;
+    const gchi_dgraph *graphs = nullptr;  // [G]
+};
+
+static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s) {
+    const uint32_t N = in.N;
+    const uint64_t E = in.E;
+    const uint32_t G = in.G;
     TmpPool tp;
     const size_t nwords = (N + 31) / 32;
+    uint32_t *off_ext = tp.alloc<uint32_t>((size_t)N + G);
+    uint64_t *g_src = tp.alloc<uint64_t>(G);
+    uint32_t *g_idoff = tp.alloc<uint32_t>(G);
     uint32_t *reg_deg = tp.alloc<uint32_t>(N);
     uint32_t *hub_flag = tp.alloc<uint32_t>(N);
     uint32_t *hub_idx = tp.alloc<uint32_t>(N + 1);
-    uint32_t *reg_off = tp.alloc<uint32_t>(N + 1);
+    uint64_t *reg64 = tp.alloc<uint64_t>(N + 1);
+    uint32_t *reg_rel = tp.alloc<uint32_t>((size_t)N + G);
+    uint64_t *ebase = tp.alloc<uint64_t>(G);
     uint32_t *bits = tp.alloc<uint32_t>(nwords);
     uint32_t *vbase = tp.alloc<uint32_t>(G + 1);
     uint32_t *nblk_g = tp.alloc<uint32_t>(G + 1);
     uint32_t *gblk = tp.alloc<uint32_t>(G + 1);
     const size_t swb = scan_workspace((size_t)std::max(N, G) + 1);
     void *sw = tp.alloc<char>(swb);
-    GCHI_CHECK(reg_deg && hub_flag && hub_idx && reg_off && bits && vbase && nblk_g && gblk && sw, GCHI_ERR_HIP,
-               "pr_plan: hipMalloc failed");
-    GCHI_HIP(hipMemcpyAsync(vbase, h_vbase, 4 * (size_t)(G + 1), hipMemcpyHostToDevice, s));
+    GCHI_CHECK(off_ext && g_src && g_idoff && reg_deg && hub_flag && hub_idx && reg64 && reg_rel && ebase && bits &&
+                   vbase && nblk_g && gblk && sw,
+               GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
+    GCHI_HIP(hipMemcpyAsync(vbase, in.h_vbase, 4 * (size_t)(G + 1), hipMemcpyHostToDevice, s));
+    if (in.unionc) {
+        p->outdeg = in.unionc->outdeg;
+        hipLaunchKernelGGL(plan_ext_union, dim3(grid_for(std::max(N, G), 256, 65536)), dim3(256), 0, s,
+                           in.unionc->in_off, in.unionc->in_src, vbase, G, N, off_ext, g_src, g_idoff);
+        GCHI_LAUNCH_CHECK();
+    } else {
+        // per-graph CSCs: offsets and out-degrees copied side by side, ids offset by vbase
+        GCHI_HIP(hipMalloc(&p->own_outdeg, (size_t)std::max<uint32_t>(N, 1) * 4));
+        p->outdeg = p->own_outdeg;
+        std::vector<uint64_t> h_src(G);
+        for (uint32_t k = 0; k < G; k++) {
+            const gchi_dgraph &gk = in.graphs[k];
+            const uint32_t nk = gk.nvertices;
+            GCHI_HIP(hipMemcpyAsync(off_ext + in.h_vbase[k] + k, gk.in_off, (size_t)(nk + 1) * 4,
+                                    hipMemcpyDeviceToDevice, s));
+            GCHI_HIP(hipMemcpyAsync(p->own_outdeg + in.h_vbase[k], gk.outdeg, (size_t)nk * 4,
+                                    hipMemcpyDeviceToDevice, s));
+            h_src[k] = (uint64_t)(uintptr_t)gk.in_src;
+        }
+        GCHI_HIP(hipMemcpyAsync(g_src, h_src.data(), 8 * (size_t)G, hipMemcpyHostToDevice, s));
+        GCHI_HIP(hipMemcpyAsync(g_idoff, in.h_vbase, 4 * (size_t)G, hipMemcpyHostToDevice, s));
+        GCHI_HIP(hipStreamSynchronize(s));  // h_src is a host temporary
+    }
+    const CscView cv{off_ext, g_src, g_idoff, vbase, G, N, in.unionc ? 0 : 1};
     launch_fill_u32(bits, 0u, nwords, s);
-    hipLaunchKernelGGL(plan_rows, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, g->in_off, N, W, reg_deg, hub_flag,
-                       bits);
+    hipLaunchKernelGGL(plan_rows, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, cv, W, reg_deg, hub_flag, bits);
     GCHI_LAUNCH_CHECK();
-    GCHI_TRY(exclusive_scan_u32(reg_deg, reg_off, N, sw, swb, s));
+    GCHI_TRY(exclusive_scan_u32_to_u64(reg_deg, reg64, N, sw, swb, s));
     GCHI_TRY(exclusive_scan_u32(hub_flag, hub_idx, N, sw, swb, s));
-    hipLaunchKernelGGL(plan_graph_blocks, dim3(grid_for(G, 256, 4096)), dim3(256), 0, s, reg_off, vbase, G, W, nblk_g);
+    hipLaunchKernelGGL(plan_reg_rel, dim3(grid_for(std::max(N, G), 256, 65536)), dim3(256), 0, s, reg64, vbase, G, N,
+                       reg_rel, ebase);
+    GCHI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(plan_graph_blocks, dim3(grid_for(G, 256, 4096)), dim3(256), 0, s, reg_rel, vbase, G, W, nblk_g);
     GCHI_LAUNCH_CHECK();
     GCHI_TRY(exclusive_scan_u32(nblk_g, gblk, G, sw, swb, s));
-    uint32_t e_reg = 0, nhub = 0, nblk = 0;
-    GCHI_TRY(read_u32(reg_off + N, &e_reg, s));
+    uint64_t e_reg = 0;
+    uint32_t nhub = 0, nblk = 0;
+    GCHI_TRY(read_u64(reg64 + N, &e_reg, s));
     GCHI_TRY(read_u32(hub_idx + N, &nhub, s));
     GCHI_TRY(read_u32(gblk + G, &nblk, s));
     p->e_reg = e_reg;
-    p->e_hub = E - e_reg;
     p->nhub = nhub;
     p->nblk = nblk;
 
     uint32_t *hub_v = tp.alloc<uint32_t>(nhub + 1);
     uint32_t *hub_deg = tp.alloc<uint32_t>(nhub + 1);
     uint32_t *hub_nch = tp.alloc<uint32_t>(nhub + 1);
-    uint32_t *hub_off = tp.alloc<uint32_t>(nhub + 1);
+    uint64_t *hub_off = tp.alloc<uint64_t>(nhub + 1);
     uint32_t *hub_choff = tp.alloc<uint32_t>(nhub + 1);
     const size_t swb2 = scan_workspace((size_t)nhub + 1);
     void *sw2 = tp.alloc<char>(swb2);
     GCHI_CHECK(hub_v && hub_deg && hub_nch && hub_off && hub_choff && sw2, GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
     if (nhub) {
-        hipLaunchKernelGGL(plan_hubs, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, g->in_off, N, hub_flag, hub_idx,
-                           hub_v, hub_deg, hub_nch);
+        hipLaunchKernelGGL(plan_hubs, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, cv, hub_flag, hub_idx, hub_v,
+                           hub_deg, hub_nch);
         GCHI_LAUNCH_CHECK();
     }
-    GCHI_TRY(exclusive_scan_u32(hub_deg, hub_off, nhub, sw2, swb2, s));
+    GCHI_TRY(exclusive_scan_u32_to_u64(hub_deg, hub_off, nhub, sw2, swb2, s));
     GCHI_TRY(exclusive_scan_u32(hub_nch, hub_choff, nhub, sw2, swb2, s));
     uint32_t nchunk = 0;
+    uint64_t e_hub = 0;
     GCHI_TRY(read_u32(hub_choff + nhub, &nchunk, s));
+    GCHI_TRY(read_u64(hub_off + nhub, &e_hub, s));
+    p->e_hub = e_hub;
     p->nchunk = nchunk;
+    GCHI_CHECK(e_reg + e_hub == E, GCHI_ERR_INVALID, "pr_plan_create: in-edge offsets do not add up to nedges");
     // one wavefront per hub row; at least 64 workgroups for the statistics
     p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(64, (nhub + kFinThreads / kWave - 1) / (kFinThreads / kWave)));
 
@@ -1083,7 +1217,7 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
         GCHI_CHECK(keys && grank && gorder_tmp && dA && sws, GCHI_ERR_HIP, "pr_plan: hipMalloc failed");
         launch_fill_u32(grank, 0xffffffffu, N, s);
         launch_fill_u32(dA, 0u, 1, s);
-        hipLaunchKernelGGL(plan_order_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, g->outdeg, N, keys);
+        hipLaunchKernelGGL(plan_order_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->outdeg, N, keys);
         GCHI_LAUNCH_CHECK();
         GCHI_TRY(gchi_sort_u64(keys, nullptr, N, 64, sws, sb, s));
         hipLaunchKernelGGL(plan_gather_rank, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, keys, N, gorder_tmp,
@@ -1094,7 +1228,7 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     p->A = A;
 
     Sizer zz;
-    zz.take<uint32_t>(N + 1);
+    zz.take<uint32_t>((size_t)N + G);
     zz.take<uint32_t>(std::max<uint64_t>(p->e_reg, 1));
     zz.take<uint32_t>(nwords);
     zz.take<uint32_t>(p->nblk + 1);
@@ -1102,7 +1236,7 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     zz.take<uint32_t>(nhub + 1);
     zz.take<uint32_t>(nhub + 1);
     zz.take<uint32_t>(nhub + 1);
-    zz.take<uint32_t>(nchunk + 1);
+    zz.take<uint64_t>(nchunk + 1);
     zz.take<uint32_t>(nchunk + 1);
     zz.take<uint32_t>(std::max<uint64_t>(p->e_hub, 1));
     zz.take<uint32_t>(G + 1);
@@ -1116,9 +1250,11 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     zz.take<uint32_t>(std::max<uint32_t>(A, 1));
     zz.take<float>(std::max<uint32_t>(A, 1));
     zz.take<uint32_t>(p->nblk + 1);
+    zz.take<uint32_t>(p->nblk + 1);
+    zz.take<uint64_t>(G);
     GCHI_HIP(hipMalloc(&p->mem, zz.off));
     Carver c(p->mem, zz.off);
-    p->reg_off = c.take<uint32_t>(N + 1);
+    p->reg_off = c.take<uint32_t>((size_t)N + G);
     p->reg_src = c.take<uint32_t>(std::max<uint64_t>(p->e_reg, 1));
     p->hub_bits = c.take<uint32_t>(nwords);
     p->blk_row = c.take<uint32_t>(p->nblk + 1);
@@ -1126,7 +1262,7 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     p->hub_v = c.take<uint32_t>(nhub + 1);
     p->hub_graph = c.take<uint32_t>(nhub + 1);
     p->hub_chunk_off = c.take<uint32_t>(nhub + 1);
-    p->chunk_beg = c.take<uint32_t>(nchunk + 1);
+    p->chunk_beg = c.take<uint64_t>(nchunk + 1);
     p->chunk_len = c.take<uint32_t>(nchunk + 1);
     p->hub_src = c.take<uint32_t>(std::max<uint64_t>(p->e_hub, 1));
     p->gblk = c.take<uint32_t>(G + 1);
@@ -1140,8 +1276,11 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     p->gorder = c.take<uint32_t>(std::max<uint32_t>(A, 1));
     p->xg = c.take<float>(std::max<uint32_t>(A, 1));
     p->tile_e0 = c.take<uint32_t>(p->nblk + 1);
+    p->tile_e1 = c.take<uint32_t>(p->nblk + 1);
+    p->ebase = c.take<uint64_t>(G);
     if (A) GCHI_HIP(hipMemcpyAsync(p->gorder, gorder_tmp, (size_t)A * 4, hipMemcpyDeviceToDevice, s));
-    GCHI_HIP(hipMemcpyAsync(p->reg_off, reg_off, (size_t)(N + 1) * 4, hipMemcpyDeviceToDevice, s));
+    GCHI_HIP(hipMemcpyAsync(p->reg_off, reg_rel, ((size_t)N + G) * 4, hipMemcpyDeviceToDevice, s));
+    GCHI_HIP(hipMemcpyAsync(p->ebase, ebase, (size_t)G * 8, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_bits, bits, nwords * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_v, hub_v, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->hub_chunk_off, hub_choff, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
@@ -1149,8 +1288,8 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     GCHI_HIP(hipMemcpyAsync(p->vbase, vbase, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, s));
     launch_fill_u32(p->counters, 0u, 2, s);
     if (E)
-        hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, g->in_off,
-                           g->in_src, N, hub_flag, hub_idx, reg_off, hub_off, grank, p->reg_src, p->hub_src, bad);
+        hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, cv,
+                           hub_flag, hub_idx, p->reg_off, p->ebase, hub_off, grank, p->reg_src, p->hub_src, bad);
     if (nhub) {
         hipLaunchKernelGGL(plan_chunks, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, hub_off, hub_choff, nhub,
                            p->chunk_beg, p->chunk_len);
@@ -1162,24 +1301,23 @@ static int plan_build(const gchi_dgraph *g, const uint32_t *h_vbase, int W, gchi
     hipLaunchKernelGGL(plan_blocks, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->reg_off, p->vbase, G, p->gblk, N,
                        W, p->nblk, p->blk_row, p->blk_graph);
     GCHI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(plan_tile_e0, dim3(grid_for(p->nblk + 1, 256, 65536)), dim3(256), 0, s, p->reg_off, p->blk_row,
-                       p->nblk, p->tile_e0);
+    hipLaunchKernelGGL(plan_tile_edges, dim3(grid_for(p->nblk, 256, 65536)), dim3(256), 0, s, p->reg_off, p->blk_row,
+                       p->blk_graph, p->nblk, p->tile_e0, p->tile_e1);
     GCHI_LAUNCH_CHECK();
     uint32_t h_bad = 0;
     GCHI_TRY(read_u32(bad, &h_bad, s));
     GCHI_CHECK(!h_bad, GCHI_ERR_INVALID,
-               "pr_plan_create: an in-edge source is >= N or has out-degree 0 (in_src and outdeg disagree)");
+               "pr_plan_create: an in-edge source is outside its graph or has out-degree 0 (in_src and outdeg "
+               "disagree)");
     return GCHI_OK;
 }
 
-static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G, const gchi_pr_tune *tune,
-                       gchi_pr_plan **out, hipStream_t s) {
-    GCHI_CHECK(g && out && h_vbase, GCHI_ERR_INVALID, "pr_plan_create: null argument");
-    GCHI_CHECK(g->nvertices > 0 && G >= 1, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
-    GCHI_CHECK(g->in_off && g->outdeg && (g->nedges == 0 || g->in_src), GCHI_ERR_INVALID,
-               "pr_plan_create: graph arrays missing");
-    GCHI_CHECK(g->nedges < (1ull << 32), GCHI_ERR_UNSUPPORTED, "pr_plan_create: E must be < 2^32");
-    GCHI_CHECK(h_vbase[0] == 0 && h_vbase[G] == g->nvertices, GCHI_ERR_INVALID,
+static int plan_create(const PlanInput &in, const gchi_pr_tune *tune, gchi_pr_plan **out, hipStream_t s) {
+    const uint32_t G = in.G;
+    const uint32_t *h_vbase = in.h_vbase;
+    GCHI_CHECK(out && h_vbase, GCHI_ERR_INVALID, "pr_plan_create: null argument");
+    GCHI_CHECK(in.N > 0 && G >= 1, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
+    GCHI_CHECK(h_vbase[0] == 0 && h_vbase[G] == in.N, GCHI_ERR_INVALID,
                "pr_plan_create: vertex bases must start at 0 and end at N");
     for (uint32_t i = 0; i < G; i++)
         GCHI_CHECK(h_vbase[i + 1] > h_vbase[i], GCHI_ERR_INVALID, "pr_plan_create: every graph needs a vertex");
@@ -1191,8 +1329,8 @@ static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G
     GCHI_CHECK(2 * W / (short_max + 1) <= kMaxLong, GCHI_ERR_INVALID,
                "pr_plan_create: short_row_max too small for edges_per_block (more than 512 long rows per block)");
     gchi_pr_plan *p = new gchi_pr_plan();
-    p->N = g->nvertices;
-    p->E = g->nedges;
+    p->N = in.N;
+    p->E = in.E;
     p->G = G;
     p->W = W;
     p->short_max = short_max;
@@ -1209,13 +1347,12 @@ static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G
             delete p;
             return fail(GCHI_ERR_INVALID, "pr_plan_create: hot_vertices must be -1, 0 or positive");
         }
-        if (g->nvertices >= (1u << 30)) {
+        if (in.N >= (1u << 30)) {
             delete p;
             return fail(GCHI_ERR_UNSUPPORTED, "pr_plan_create: the gather layout needs N < 2^30");
         }
     }
-    p->outdeg = g->outdeg;
-    int st = plan_build(g, h_vbase, W, p, s);
+    int st = plan_build(in, W, p, s);
     if (st == GCHI_OK) {
         int dev = 0, ncu = 256;
         size_t lds_cu = 160 * 1024, lds_wg = 160 * 1024;
@@ -1267,6 +1404,7 @@ static int plan_create(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t G
     }
     if (st != GCHI_OK) {
         if (p->mem) (void)hipFree(p->mem);
+        if (p->own_outdeg) (void)hipFree(p->own_outdeg);
         delete p;
         return st;
     }
@@ -1293,8 +1431,8 @@ int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, con
         xg = p->xg;
     }
     const uint32_t nhubt = (p->nchunk + kPrWaves - 1) / kPrWaves;
-    PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->tile_e0, p->hub_bits, p->chunk_beg, p->chunk_len,
-             p->hub_src,
+    PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->tile_e0, p->tile_e1, p->hub_bits, p->ebase,
+             p->chunk_beg, p->chunk_len, p->hub_src,
              p->outdeg, xg, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + 1, p->nchunk, nhubt,
              p->nblk, nhubt + p->nblk, p->short_max};
     const size_t shmem = (size_t)2 * p->W * sizeof(float);
@@ -1329,13 +1467,16 @@ int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, con
 
 struct PlanView {
     uint32_t N, nblk, nhub, nchunk, npersist;
-    const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off, *chunk_beg, *chunk_len, *hub_src,
-        *outdeg;
+    const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off;
+    const uint64_t *chunk_beg;
+    const uint32_t *chunk_len, *hub_src, *outdeg;
 };
 
-// Row structure of a plan whose edges hold vertex ids (no gather layout).
+// Row structure of a single-graph plan whose edges hold vertex ids (no gather
+// layout): reg_off[v] is row v's first regular edge in reg_src.
 int pr_plan_view(const gchi_pr_plan *p, PlanView *v) {
     GCHI_CHECK(p && v, GCHI_ERR_INVALID, "plan view: null argument");
+    GCHI_CHECK(p->G == 1, GCHI_ERR_UNSUPPORTED, "plan view: needs a single-graph plan");
     GCHI_CHECK(!p->hot, GCHI_ERR_UNSUPPORTED, "plan view: the plan's edges hold gather-layout positions "
                                               "(hot_vertices != 0); build it with hot_vertices = 0");
     *v = PlanView{p->N, p->nblk, p->nhub, p->nchunk, p->npersist, p->reg_off, p->reg_src, p->blk_row, p->hub_bits,
@@ -1347,20 +1488,64 @@ int pr_plan_view(const gchi_pr_plan *p, PlanView *v) {
 
 using namespace gchi;
 
+static int check_csc(const gchi_dgraph *g) {
+    GCHI_CHECK(g, GCHI_ERR_INVALID, "pr_plan_create: null graph");
+    GCHI_CHECK(g->nvertices > 0, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
+    GCHI_CHECK(g->in_off && g->outdeg && (g->nedges == 0 || g->in_src), GCHI_ERR_INVALID,
+               "pr_plan_create: graph arrays missing");
+    GCHI_CHECK(g->nedges < (1ull << 32), GCHI_ERR_UNSUPPORTED, "pr_plan_create: a graph's E must be < 2^32");
+    return GCHI_OK;
+}
+
 extern "C" int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream) {
-    GCHI_CHECK(g, GCHI_ERR_INVALID, "pr_plan_create: null argument");
+    GCHI_TRY(check_csc(g));
     uint32_t vb[2] = {0, g->nvertices};
-    return plan_create(g, vb, 1, tune, out, as_stream(stream));
+    PlanInput in;
+    in.N = g->nvertices;
+    in.E = g->nedges;
+    in.h_vbase = vb;
+    in.unionc = g;
+    return plan_create(in, tune, out, as_stream(stream));
 }
 
 extern "C" int gchi_pr_plan_create_multi(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t ngraphs,
                                          const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream) {
-    return plan_create(g, h_vbase, ngraphs, tune, out, as_stream(stream));
+    GCHI_TRY(check_csc(g));
+    GCHI_CHECK(ngraphs >= 1, GCHI_ERR_INVALID, "pr_plan_create_multi: no graphs");
+    PlanInput in;
+    in.N = g->nvertices;
+    in.E = g->nedges;
+    in.G = ngraphs;
+    in.h_vbase = h_vbase;
+    in.unionc = g;
+    return plan_create(in, tune, out, as_stream(stream));
+}
+
+extern "C" int gchi_pr_plan_create_batch(const gchi_dgraph *graphs, uint32_t ngraphs, const gchi_pr_tune *tune,
+                                         gchi_pr_plan **out, void *stream) {
+    GCHI_CHECK(graphs && ngraphs >= 1, GCHI_ERR_INVALID, "pr_plan_create_batch: no graphs");
+    std::vector<uint32_t> vb(ngraphs + 1, 0);
+    uint64_t n = 0, e = 0;
+    for (uint32_t k = 0; k < ngraphs; k++) {
+        GCHI_TRY(check_csc(&graphs[k]));
+        n += graphs[k].nvertices;
+        e += graphs[k].nedges;
+        GCHI_CHECK(n < (1ull << 32), GCHI_ERR_UNSUPPORTED, "pr_plan_create_batch: total vertices must be < 2^32");
+        vb[k + 1] = (uint32_t)n;
+    }
+    PlanInput in;
+    in.N = (uint32_t)n;
+    in.E = e;
+    in.G = ngraphs;
+    in.h_vbase = vb.data();
+    in.graphs = graphs;
+    return plan_create(in, tune, out, as_stream(stream));
 }
 
 extern "C" int gchi_pr_plan_destroy(gchi_pr_plan *p) {
     if (!p) return GCHI_OK;
     if (p->mem) (void)hipFree(p->mem);
+    if (p->own_outdeg) (void)hipFree(p->own_outdeg);
     delete p;
     return GCHI_OK;
 }

@@ -13,7 +13,7 @@
 // destination row (lane = query) turns every in-edge into one coalesced row
 // gather shared by the 64 queries; the in-edge index stream is read once per
 // query block instead of once per query.  Rows come from the PageRank plan
-// (regular blocks of < 2W edges, hub rows cut into 4096-edge chunks whose
+// (regular blocks of < 2W edges, hub rows cut into 512-edge chunks whose
 // 64-wide partials are combined in chunk order).  A persistent kernel pulls
 // (query block, tile) items, query-block-major, from a device work counter.
 //
@@ -31,8 +31,9 @@ namespace gchi {
 // plan internals shared with pagerank.hip
 struct PlanView {
     uint32_t N, nblk, nhub, nchunk, npersist;
-    const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off, *chunk_beg, *chunk_len, *hub_src,
-        *outdeg;
+    const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off;
+    const uint64_t *chunk_beg;
+    const uint32_t *chunk_len, *hub_src, *outdeg;
 };
 int pr_plan_view(const gchi_pr_plan *p, PlanView *v);
 }  // namespace gchi
@@ -195,8 +196,9 @@ __global__ __launch_bounds__(kPpThreads) void ppr_sweep_kernel(PpArgs a, const T
         const T *Rin = Rin_all + qb * stride_q;
         T *Rout = Rout_all + qb * stride_q;
         if (lt < a.pv.nchunk) {
-            // hub chunk: 4 waves split <= 4096 edges, partials combined in wave order
-            const uint32_t cb = a.pv.chunk_beg[lt], cl = a.pv.chunk_len[lt];
+            // hub chunk: 4 waves split <= 512 edges, partials combined in wave order
+            const uint64_t cb = a.pv.chunk_beg[lt];
+            const uint32_t cl = a.pv.chunk_len[lt];
             const uint32_t per = (cl + kPpWaves - 1) / kPpWaves;
             const uint32_t b0 = min(cl, per * wave), b1 = min(cl, per * (wave + 1));
             float acc = gather_rows<T>(a.pv.hub_src + cb, b0, b1, a.inv, Rin, lane);

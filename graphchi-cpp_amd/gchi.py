@@ -319,6 +319,23 @@ class PageRankPlan:
                                                   ctypes.byref(tune), ctypes.byref(self.handle), stream()),
                   "pr_plan_create_multi")
 
+    @classmethod
+    def batch(cls, graphs, edges_per_block=0, short_row_max=0, persistent=0, hot_vertices=0):
+        """Plan over separate graphs (each a DeviceGraph with local ids): graph k
+        owns plan vertices [vbase[k], vbase[k+1]) (self.vbase).  The total edge
+        count may exceed 2^32; the graphs may be freed after this call."""
+        self = cls.__new__(cls)
+        self.graph = None
+        self.handle = c_vp()
+        tune = PrTune(edges_per_block, short_row_max, persistent, hot_vertices)
+        arr = (DGraph * len(graphs))(*[g.dg for g in graphs])
+        check(lib().gchi_pr_plan_create_batch(arr, len(graphs), ctypes.byref(tune), ctypes.byref(self.handle),
+                                              stream()), "pr_plan_create_batch")
+        self.ngraphs = len(graphs)
+        self.vbase = np.concatenate([[0], np.cumsum([g.N for g in graphs])]).astype(np.uint64)
+        self.N = int(self.vbase[-1])
+        return self
+
     def info(self):
         inf = PrPlanInfo()
         check(lib().gchi_pr_plan_info_get(self.handle, ctypes.byref(inf)))
@@ -352,7 +369,7 @@ class PageRankExec:
 
     def __init__(self, plan, op, init, nsweeps, do_init=True, x=None, r=None):
         torch = _torch()
-        N = plan.graph.N
+        N = plan.graph.N if plan.graph is not None else plan.N
         self.plan, self.nsweeps = plan, nsweeps
         self.x = x if x is not None else [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]
         self.r = r if r is not None else [torch.empty(N, dtype=torch.float32, device="cuda") for _ in range(2)]

@@ -189,6 +189,16 @@ int gchi_pr_plan_create(const gchi_dgraph *g, const gchi_pr_tune *tune, gchi_pr_
  * run in one launch per sweep. */
 int gchi_pr_plan_create_multi(const gchi_dgraph *g, const uint32_t *h_vbase, uint32_t ngraphs,
                               const gchi_pr_tune *tune, gchi_pr_plan **out, void *stream);
+/* Plan over ngraphs separate graphs, each its own CSC (local vertex ids 0..N_k-1,
+ * its own in_off/in_src/outdeg, any device allocations): graph k owns plan
+ * vertices [V_k, V_k + N_k), V_k = N_0 + ... + N_{k-1}, in the x/r vectors;
+ * d_graph_teleport and d_stats are per graph as for gchi_pr_plan_create_multi.
+ * Each graph needs E_k < 2^32; the total may exceed 2^32 (offsets are 32-bit
+ * per graph, 64-bit across graphs), total vertices < 2^32.  The plan copies
+ * what it needs: the graphs may be freed once it is created.  All graphs of a
+ * batch are swept by one launch of the sweep kernel. */
+int gchi_pr_plan_create_batch(const gchi_dgraph *graphs, uint32_t ngraphs, const gchi_pr_tune *tune,
+                              gchi_pr_plan **out, void *stream);
 int gchi_pr_plan_destroy(gchi_pr_plan *plan);
 int gchi_pr_plan_info_get(const gchi_pr_plan *plan, gchi_pr_plan_info *info);
 
