@@ -131,6 +131,18 @@ GCHI_DEV float gather_rows(const uint32_t *src_idx, uint32_t beg, uint32_t end, 
         const uint32_t my = lane < (int)n ? __builtin_nontemporal_load(&src_idx[e + lane]) : 0u;
         const float myi = lane < (int)n ? inv[my] : 0.0f;
         uint32_t j = 0;
+        // 8 row gathers in flight per wavefront; the additions stay in edge order
+        for (; j + 8 <= n; j += 8) {
+            float v[8], w[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t sq = __builtin_amdgcn_readlane(my, j + q);
+                w[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j + q));
+                v[q] = Store<T>::ld(R + (size_t)sq * kQ + lane);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += v[q] * w[q];
+        }
         for (; j + 4 <= n; j += 4) {
             uint32_t s0 = __builtin_amdgcn_readlane(my, j), s1 = __builtin_amdgcn_readlane(my, j + 1);
             uint32_t s2 = __builtin_amdgcn_readlane(my, j + 2), s3 = __builtin_amdgcn_readlane(my, j + 3);
