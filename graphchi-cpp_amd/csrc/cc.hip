@@ -36,13 +36,17 @@ __global__ void cc_init(uint32_t *parent, uint32_t n) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) parent[v] = v;
 }
 
+// One union per edge.  With `half` (a symmetric list: every u-v present as both
+// (u,v) and (v,u)) only the entries with u < v are used: the other half would
+// union the same sets again.
 __global__ void cc_hook(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst, uint64_t e,
-                        uint32_t n, uint32_t *parent) {
+                        uint32_t n, uint32_t *parent, int half) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
         uint32_t u = __builtin_nontemporal_load(&src[i]);
         uint32_t v = __builtin_nontemporal_load(&dst[i]);
         if (u >= n || v >= n) continue;  // ids beyond N are rejected on the host side
+        if (half && u >= v) continue;
         u = uf_find(parent, u);
         v = uf_find(parent, v);
         while (u != v) {
@@ -115,8 +119,9 @@ extern "C" int gchi_cc_workspace(uint32_t nvertices, size_t *bytes) {
     return GCHI_OK;
 }
 
-extern "C" int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
-                                uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream) {
+extern "C" int gchi_cc_minlabel_ex(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                                   int flags, uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream) {
+    GCHI_CHECK((flags & ~GCHI_CC_SYMMETRIC) == 0, GCHI_ERR_INVALID, "cc_minlabel: unknown flags");
     GCHI_CHECK(d_labels && nvertices > 0, GCHI_ERR_INVALID, "cc_minlabel: null labels or no vertices");
     GCHI_CHECK(nedges == 0 || (d_src && d_dst), GCHI_ERR_INVALID, "cc_minlabel: null edges");
     size_t need = 0;
@@ -127,11 +132,16 @@ extern "C" int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, ui
     hipLaunchKernelGGL(cc_init, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, parent, nvertices);
     if (nedges)
         hipLaunchKernelGGL(cc_hook, dim3(grid_for(nedges, 256, 1u << 16)), dim3(256), 0, s, d_src, d_dst, nedges,
-                           nvertices, parent);
+                           nvertices, parent, (flags & GCHI_CC_SYMMETRIC) ? 1 : 0);
     hipLaunchKernelGGL(cc_compress, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, parent, d_labels,
                        nvertices);
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
+}
+
+extern "C" int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                                uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream) {
+    return gchi_cc_minlabel_ex(d_src, d_dst, nedges, nvertices, 0, d_labels, d_work, work_bytes, stream);
 }
 
 extern "C" int gchi_cc_labelprop_init(uint32_t *d_labels, uint32_t nvertices, void *stream) {

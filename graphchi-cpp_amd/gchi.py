@@ -399,14 +399,20 @@ class PageRankExec:
 
 
 # ---------------------------------------------------------------- connected components
-def cc_minlabel(src, dst, n):
+CC_SYMMETRIC = 1
+
+
+def cc_minlabel(src, dst, n, symmetric=False):
+    """Minimum-id label of every weakly connected component.  symmetric=True
+    promises that every edge is listed in both directions (only u < v is used)."""
     L = lib()
     labels = empty_u32(n)
     nb = c_sz()
     check(L.gchi_cc_workspace(n, ctypes.byref(nb)))
     ws = workspace(nb.value)
     e = src.numel() if src is not None else 0
-    check(L.gchi_cc_minlabel(dptr(src), dptr(dst), e, n, dptr(labels), dptr(ws), nb.value, stream()), "cc_minlabel")
+    check(L.gchi_cc_minlabel_ex(dptr(src), dptr(dst), e, n, CC_SYMMETRIC if symmetric else 0, dptr(labels),
+                                dptr(ws), nb.value, stream()), "cc_minlabel")
     return labels
 
 

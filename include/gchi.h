@@ -294,6 +294,13 @@ int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedg
  * synchronous min-propagation sweep over every edge in both directions;
  * *d_changed = number of labels that changed (0 = converged,
  * connectedcomponents.cpp:125-138). */
+/* As gchi_cc_minlabel.  flags: GCHI_CC_SYMMETRIC promises a symmetric list (every
+ * u-v present as both (u,v) and (v,u), the form the reference's undirected
+ * inputs take in its edge files), so only the entries with u < v are unioned;
+ * the labels are the same. */
+#define GCHI_CC_SYMMETRIC 1
+int gchi_cc_minlabel_ex(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices, int flags,
+                        uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream);
 int gchi_cc_labelprop_init(uint32_t *d_labels, uint32_t nvertices, void *stream);
 int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                             const uint32_t *d_labels_in, uint32_t *d_labels_out, uint64_t *d_changed, void *stream);

@@ -37,7 +37,21 @@ def test_cc_scale24_vs_unionfind(dev):
     src, dst = graphs.rmat_pairs(24, 16, 3, undirected=True)
     src, dst = graphs.preprocess(src, dst, undirected=True)
     n = int(max(src.max(), dst.max())) + 1
-    lab = dev.to_host_u32(dev.cc_minlabel(dev.to_dev_u32(src), dev.to_dev_u32(dst), n), n)
+    ds, dd = dev.to_dev_u32(src), dev.to_dev_u32(dst)
+    lab = dev.to_host_u32(dev.cc_minlabel(ds, dd, n), n)
     want = oracle.cc_unionfind(n, src, dst)
     np.testing.assert_array_equal(lab, want)
+    # the list holds every edge both ways: the half-list union gives the same labels
+    np.testing.assert_array_equal(dev.to_host_u32(dev.cc_minlabel(ds, dd, n, symmetric=True), n), want)
     print(f"\nscale24 CC: N={n} E={len(src)} components={len(np.unique(want))}")
+
+
+@pytest.mark.parametrize("case", graphs.CC_CASES, ids=[c["name"] for c in graphs.CC_CASES])
+def test_cc_symmetric_flag_on_symmetrised_golden(dev, golden, case):
+    """The reference's CC treats edges as undirected: symmetrising a golden
+    input and unioning only u < v reproduces the reference's labels."""
+    src, dst, n = graphs.make(case)
+    s2 = np.concatenate([src, dst]).astype(np.uint32)
+    d2 = np.concatenate([dst, src]).astype(np.uint32)
+    lab = dev.to_host_u32(dev.cc_minlabel(dev.to_dev_u32(s2), dev.to_dev_u32(d2), n, symmetric=True), n)
+    np.testing.assert_array_equal(lab, golden[f"{case['name']}__labels"])
