@@ -146,3 +146,34 @@ def test_batch_plan_beyond_2p32_edges(dev):
     r = ex.ranks()
     for k in (0, copies // 2, copies - 1):
         np.testing.assert_array_equal(r[k * n:(k + 1) * n].cpu().numpy(), r1)
+
+
+def test_batch_and_union_with_edgeless_graphs(dev):
+    """Graphs without edges inside a batch or union plan: their ranks are the
+    per-graph constant, and their neighbours are swept as if alone."""
+    import torch
+    gl = make_graphs(6, 21)
+    n_empty = 7
+    empty = torch.empty(0, dtype=torch.int32, device="cuda")
+    devs = []
+    for i, (s, d, n) in enumerate(gl):
+        devs.append(dev.DeviceGraph(dev.to_dev_u32(s), dev.to_dev_u32(d), n))
+        if i % 2 == 0:
+            devs.append(dev.DeviceGraph(empty, empty, n_empty))
+    sizes = [g.N for g in devs]
+    plan = dev.PageRankPlan.batch(devs)
+    tele = torch.tensor([0.15 / n for n in sizes], dtype=torch.float64, device="cuda")
+    ex = dev.PageRankExec(plan, dev.pr_op(0.15, D, graph_teleport=tele), 1.0, 4)
+    ex.launch()
+    r = ex.ranks().cpu().numpy()
+    vb = plan.vbase.astype(np.int64)
+    k = 0
+    for i, g in enumerate(devs):
+        part = r[vb[i]:vb[i + 1]]
+        if g.E == 0:
+            np.testing.assert_array_equal(part, np.full(g.N, np.float32(0.15 / g.N)))
+        else:
+            s, d, n = gl[k]
+            ri, _ = run_single(dev, s, d, n, 4, {})
+            np.testing.assert_array_equal(part, ri)
+            k += 1

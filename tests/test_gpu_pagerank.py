@@ -214,3 +214,32 @@ def test_wave_sweep_long_blocks(dev):
         outs.append(ex.ranks().cpu().numpy().copy())
     np.testing.assert_array_equal(outs[0], outs[1])
     np.testing.assert_array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("tune", [dict(), dict(persistent=1), dict(edges_per_block=256, persistent=2),
+                                  dict(edges_per_block=256, hot_vertices=1024), dict(persistent=3, hot_vertices=1024),
+                                  dict(hot_vertices=-1)],
+                         ids=["wg", "persistent", "wave", "wave_hot", "hotwg", "layout"])
+def test_pagerank_graph_without_edges(dev, tune):
+    """No edges at all: every rank is the constant term after each sweep, in
+    every launch form (empty plans, no gather layout sources)."""
+    import torch
+    n = 5
+    empty = torch.empty(0, dtype=torch.int32, device="cuda")
+    g = dev.DeviceGraph(empty, empty, n)
+    plan = dev.PageRankPlan(g, **tune)
+    ex = dev.PageRankExec(plan, dev.pr_op(0.15 / n, DAMPING), 1.0, 3)
+    ex.launch()
+    r = ex.ranks().cpu().numpy()
+    np.testing.assert_array_equal(r, np.full(n, np.float32(0.15 / n)))
+
+
+def test_pagerank_single_vertex_and_self_loop_only(dev):
+    """A lone vertex, and a graph whose only edges were self-loops (dropped by
+    the generator's preprocessing; here absent from the edge list)."""
+    import torch
+    empty = torch.empty(0, dtype=torch.int32, device="cuda")
+    g = dev.DeviceGraph(empty, empty, 1)
+    ex = dev.PageRankExec(dev.PageRankPlan(g), dev.pr_op(0.15, DAMPING), 1.0, 2)
+    ex.launch()
+    assert ex.ranks().cpu().numpy()[0] == np.float32(0.15)
