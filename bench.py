@@ -36,14 +36,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    kept in profiles/<name>.json by the profiling runs; None when absent."""
+def shape_key(info):
+    """Launch shape of a PageRank plan as used to key profiles."""
+    return (f"W{info['edges_per_block']}_s{info['short_row_max']}_p{info['persistent']}"
+            f"_h{info['hot_vertices']}_l{1 if info['gather_sources'] else 0}")
+
+
+def load_traffic(name, key):
+    """Bytes per sweep between L2 and the fabric, measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE) for this launch shape and kept in
+    profiles/<name>.json; None when that shape was not profiled."""
     p = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        shapes = json.load(f).get("shapes", {})
+    entry = shapes.get(key)
+    return entry.get("bytes_per_sweep") if entry else None
 
 
 def timed(fn, torch):
@@ -96,6 +105,23 @@ def bench_pagerank(gchi, torch, args, src, dst, E, N):
     log(f"timed {args.steps} sweeps: {ms:.4f} ms/sweep (wall {wall * 1e3:.2f} ms); last residual "
         f"{stats[-1, 0]:.3e}, rank sum {stats[-1, 1]:.6e}")
     return g, plan, info, ms, tuned, stats
+
+
+def bench_load(gchi, torch, src, dst, E, N, tune):
+    """What a caller holding the edges in host memory pays once per graph: the
+    upload of the (src, dst) arrays from pinned host memory, the CSC build and
+    the sweep plan; reported beside the HBM-resident sweep rate, never as it."""
+    hs = torch.empty(E, dtype=torch.int32, pin_memory=True)
+    hd = torch.empty(E, dtype=torch.int32, pin_memory=True)
+    hs.copy_(src[:E])
+    hd.copy_(dst[:E])
+    torch.cuda.synchronize()
+    (ds, dd), t_up = timed(lambda: (hs.to("cuda", non_blocking=True), hd.to("cuda", non_blocking=True)), torch)
+    g, t_csc = timed(lambda: gchi.DeviceGraph(ds, dd, N), torch)
+    plan, t_plan = timed(lambda: gchi.PageRankPlan(g, **tune), torch)
+    del plan, g, ds, dd
+    return {"upload_ms": round(t_up * 1e3, 2), "upload_GBps": round(8 * E / t_up / 1e9, 1),
+            "csc_build_ms": round(t_csc * 1e3, 2), "plan_build_ms": round(t_plan * 1e3, 2)}
 
 
 def sweep_kernels(info):
@@ -287,7 +313,7 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--hot-vertices", type=int, default=0)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
-    ap.add_argument("--only", default="", help="comma list of sections: pagerank,ppr,sort,cc,tc,multigraph")
+    ap.add_argument("--only", default="", help="comma list of sections: pagerank,load,ppr,sort,cc,tc,multigraph")
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--ppr-plain-queries", type=int, default=64)
     ap.add_argument("--ppr-check", type=int, default=1)
@@ -325,7 +351,8 @@ def main():
     algo_bytes = 8 * E + 20 * N + 4
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("pr_sweep_rmat22_traffic"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic("pr_sweep_rmat22_traffic", shape_key(info)), "shape": shape_key(info),
                 "kernel": sweep_kernels(info) + ", one sweep (all launches)", "algorithmic_bytes": algo_bytes}
     out = {
         "metric": METRIC, "value": E / (ms * 1e-3), "unit": "edges/s/iter", "n_gpus": 1, "steps": args.steps,
@@ -340,7 +367,10 @@ def main():
         "roofline": roofline,
     }
     extras = {}
-    sections = [("ppr", lambda: bench_ppr(gchi, torch, args, g, plan, N)),
+    tune_used = {k: info[k] for k in ("edges_per_block", "short_row_max", "persistent")}
+    tune_used["hot_vertices"] = (tuned["best"]["hot_vertices"] if tuned else args.hot_vertices)
+    sections = [("load", lambda: bench_load(gchi, torch, src, dst, E, N, tune_used)),
+                ("ppr", lambda: bench_ppr(gchi, torch, args, g, plan, N)),
                 ("sort", lambda: bench_sort_and_shards(gchi, torch, src, dst, E, maxid)),
                 ("cc", lambda: bench_cc(gchi, torch, args.cc_scale)),
                 ("tc", lambda: bench_tc(gchi, torch, args.tc_scale)),

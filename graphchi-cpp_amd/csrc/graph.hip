@@ -10,21 +10,19 @@
 
 namespace gchi {
 
-// Degree histograms.  Counts of the ids below kPriv are kept per workgroup in
-// LDS and flushed once: on skewed graphs a few vertices own a large share of
-// the edges, and one global counter per such vertex would serialise every
-// atomic on it.  Higher ids count with global atomics directly.
-constexpr uint32_t kPriv = 4096;
+// Degree histograms.  On skewed graphs a few vertices own a large share of the
+// edges, and one global counter per such vertex would serialise every atomic
+// on it: each workgroup first combines counts in LDS hash tables (LdsCombiner)
+// and adds them to the global counts once.
 constexpr int kHistThreads = 256;
+constexpr int kHistSlots = 2048;
 
 __global__ __launch_bounds__(kHistThreads) void degree_hist(const uint32_t *src, const uint32_t *dst, uint64_t n,
                                                             uint32_t nv, uint32_t *outdeg, uint32_t *indeg,
                                                             uint32_t *err) {
-    __shared__ uint32_t s_out[kPriv], s_in[kPriv];
-    for (uint32_t i = threadIdx.x; i < kPriv; i += kHistThreads) {
-        s_out[i] = 0u;
-        s_in[i] = 0u;
-    }
+    __shared__ LdsCombiner<kHistSlots> c_out, c_in;
+    comb_clear<kHistSlots, kHistThreads>(c_out);
+    comb_clear<kHistSlots, kHistThreads>(c_in);
     __syncthreads();
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -33,21 +31,12 @@ __global__ __launch_bounds__(kHistThreads) void degree_hist(const uint32_t *src,
             atomicOr(err, 1u);
             continue;
         }
-        if (outdeg) {
-            if (s < kPriv) atomicAdd(&s_out[s], 1u);
-            else atomicAdd(&outdeg[s], 1u);
-        }
-        if (indeg) {
-            if (d < kPriv) atomicAdd(&s_in[d], 1u);
-            else atomicAdd(&indeg[d], 1u);
-        }
+        if (outdeg) comb_add(c_out, s, outdeg);
+        if (indeg) comb_add(c_in, d, indeg);
     }
     __syncthreads();
-    const uint32_t lim = nv < kPriv ? nv : kPriv;
-    for (uint32_t v = threadIdx.x; v < lim; v += kHistThreads) {
-        if (outdeg && s_out[v]) atomicAdd(&outdeg[v], s_out[v]);
-        if (indeg && s_in[v]) atomicAdd(&indeg[v], s_in[v]);
-    }
+    if (outdeg) comb_flush<kHistSlots, kHistThreads>(c_out, outdeg);
+    if (indeg) comb_flush<kHistSlots, kHistThreads>(c_in, indeg);
 }
 
 __global__ void pack_keys(const uint32_t *a, const uint32_t *b, uint64_t n, int b_bits, uint64_t *keys) {

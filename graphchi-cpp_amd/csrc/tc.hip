@@ -27,14 +27,14 @@ namespace gchi {
 constexpr int kTcWaves = 4;
 constexpr int kTcLds = 1024;
 
-// in+out degree; ids below kTcPriv count in LDS per workgroup first (skewed
+// in+out degree, combined per workgroup in an LDS hash table first (skewed
 // graphs would otherwise serialise on a few global counters)
-constexpr uint32_t kTcPriv = 8192;
+constexpr int kTcDegSlots = 4096;
 
 __global__ __launch_bounds__(256) void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n,
                                                   uint32_t *deg, uint32_t *err) {
-    __shared__ uint32_t s_deg[kTcPriv];
-    for (uint32_t i = threadIdx.x; i < kTcPriv; i += 256) s_deg[i] = 0u;
+    __shared__ LdsCombiner<kTcDegSlots> comb;
+    comb_clear<kTcDegSlots, 256>(comb);
     __syncthreads();
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
@@ -44,15 +44,11 @@ __global__ __launch_bounds__(256) void tc_degrees(const uint32_t *src, const uin
             continue;
         }
         if (s == d) continue;
-        if (s < kTcPriv) atomicAdd(&s_deg[s], 1u);
-        else atomicAdd(&deg[s], 1u);
-        if (d < kTcPriv) atomicAdd(&s_deg[d], 1u);
-        else atomicAdd(&deg[d], 1u);
+        comb_add(comb, s, deg);
+        comb_add(comb, d, deg);
     }
     __syncthreads();
-    const uint32_t lim = n < kTcPriv ? n : kTcPriv;
-    for (uint32_t v = threadIdx.x; v < lim; v += 256)
-        if (s_deg[v]) atomicAdd(&deg[v], s_deg[v]);
+    comb_flush<kTcDegSlots, 256>(comb, deg);
 }
 
 __global__ void tc_degree_keys(const uint32_t *deg, uint32_t n, int idb, uint64_t *keys) {

@@ -138,7 +138,7 @@ typedef struct gchi_pr_op {
 /* Launch shape of the pull sweep (filled by gchi_autotune_pr or defaults). */
 typedef struct gchi_pr_tune {
     int edges_per_block;   /* W: multiple of 256 in [256,4096]; 0 = 2048.  Rows with
-                              in-degree >= W are hub rows, cut into 4096-edge chunks */
+                              in-degree >= W are hub rows, cut into 512-edge chunks */
     int short_row_max;     /* rows up to this length use one lane per row; 0 = 32 */
     int persistent;        /* 0: one workgroup per tile; 1: persistent workgroups pull
                               tiles from a device work counter; 2: one 1024-thread
