@@ -121,54 +121,6 @@ GCHI_DEV void st_relaxed_agent(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Per-workgroup counting of vertex ids in an LDS hash table (open addressing,
-// a few probes), for histograms whose counts concentrate on a few vertices:
-// every count of an id that holds a slot is an LDS atomic, and the table is
-// added to the global counts once; an id that finds no slot counts globally.
-constexpr uint32_t kCombEmpty = 0xffffffffu;
-GCHI_DEV uint32_t comb_hash(uint32_t x) { return (x * 2654435761u) >> 16; }
-
-template <int SLOTS>
-struct LdsCombiner {
-    uint32_t key[SLOTS];
-    uint32_t cnt[SLOTS];
-};
-
-template <int SLOTS, int THREADS>
-GCHI_DEV void comb_clear(LdsCombiner<SLOTS> &t) {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)SLOTS; i += THREADS) {
-        t.key[i] = kCombEmpty;
-        t.cnt[i] = 0u;
-    }
-}
-
-template <int SLOTS>
-GCHI_DEV void comb_add(LdsCombiner<SLOTS> &t, uint32_t x, uint32_t *global) {
-    uint32_t slot = comb_hash(x) & (SLOTS - 1);
-#pragma unroll
-    for (int probe = 0; probe < 4; probe++) {
-        uint32_t k = t.key[slot];
-        if (k == kCombEmpty) {
-            uint32_t old = atomicCAS(&t.key[slot], kCombEmpty, x);
-            k = old == kCombEmpty ? x : old;
-        }
-        if (k == x) {
-            atomicAdd(&t.cnt[slot], 1u);
-            return;
-        }
-        slot = (slot + 1) & (SLOTS - 1);
-    }
-    atomicAdd(&global[x], 1u);
-}
-
-template <int SLOTS, int THREADS>
-GCHI_DEV void comb_flush(const LdsCombiner<SLOTS> &t, uint32_t *global) {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)SLOTS; i += THREADS) {
-        const uint32_t k = t.key[i], c = t.cnt[i];
-        if (k != kCombEmpty && c) atomicAdd(&global[k], c);
-    }
-}
-
 // Fill kernel (used instead of hipMemsetAsync so that capture into a HIP graph
 // never records a memset node).
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -10,49 +10,6 @@
 
 namespace gchi {
 
-// Degree histograms.  On skewed graphs a few vertices own a large share of the
-// edges, and one global counter per such vertex would serialise every atomic
-// on it: each workgroup first combines counts in LDS hash tables (LdsCombiner)
-// and adds them to the global counts once.
-constexpr int kHistThreads = 256;
-constexpr int kHistSlots = 2048;
-
-__global__ __launch_bounds__(kHistThreads) void degree_hist(const uint32_t *src, const uint32_t *dst, uint64_t n,
-                                                            uint32_t nv, uint32_t *outdeg, uint32_t *indeg,
-                                                            uint32_t *err) {
-    __shared__ LdsCombiner<kHistSlots> c_out, c_in;
-    comb_clear<kHistSlots, kHistThreads>(c_out);
-    comb_clear<kHistSlots, kHistThreads>(c_in);
-    __syncthreads();
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t s = src[i], d = dst[i];
-        if (s >= nv || d >= nv) {
-            atomicOr(err, 1u);
-            continue;
-        }
-        if (outdeg) comb_add(c_out, s, outdeg);
-        if (indeg) comb_add(c_in, d, indeg);
-    }
-    __syncthreads();
-    if (outdeg) comb_flush<kHistSlots, kHistThreads>(c_out, outdeg);
-    if (indeg) comb_flush<kHistSlots, kHistThreads>(c_in, indeg);
-}
-
-__global__ void pack_keys(const uint32_t *a, const uint32_t *b, uint64_t n, int b_bits, uint64_t *keys) {
-    // key = (a << b_bits) | b
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        keys[i] = ((uint64_t)a[i] << b_bits) | (uint64_t)b[i];
-}
-
-__global__ void low_bits_u32(const uint64_t *keys, uint64_t n, int bits, uint32_t *out) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t m = bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = (uint32_t)(keys[i] & m);
-}
-
 unsigned grid_for(uint64_t n, unsigned threads, unsigned cap) {
     uint64_t b = div_up(n, threads);
     if (b == 0) b = 1;
@@ -60,26 +17,84 @@ unsigned grid_for(uint64_t n, unsigned threads, unsigned cap) {
     return (unsigned)b;
 }
 
-int degrees(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, uint32_t *outdeg, uint32_t *indeg,
-            uint32_t *err, hipStream_t s) {
-    if (outdeg) launch_fill_u32(outdeg, 0u, nv, s);
-    if (indeg) launch_fill_u32(indeg, 0u, nv, s);
-    if (n)
-        hipLaunchKernelGGL(degree_hist, dim3(grid_for(n, kHistThreads * 64, 2048)), dim3(kHistThreads), 0, s, src,
-                           dst, n, nv, outdeg, indeg, err);
+// Counting by sorted runs.  Device-wide atomics on random counters run at a few
+// billion per second on this part whatever the spread of the ids, so the CSC
+// build counts degrees from runs of equal ids in sorted keys instead: the key
+// is (dst << sb) | src with sb = vb rounded up to a whole 8-bit digit, the
+// first sb/8 radix passes leave the keys ordered by src alone (out-degrees are
+// the lengths of its runs), and the remaining passes finish the (dst, src)
+// order (in-degrees are the lengths of the dst runs).
+
+// run_start[f] = index of the first key of each run of equal fields f.
+__global__ void field_run_starts(const uint64_t *__restrict__ k, uint64_t n, int shift, uint32_t mask, uint32_t nv,
+                                 uint32_t *__restrict__ run_start) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t f = (uint32_t)(k[i] >> shift) & mask;
+        const bool first = i == 0 || ((uint32_t)(k[i - 1] >> shift) & mask) != f;
+        if (first && f < nv) run_start[f] = (uint32_t)i;
+    }
+}
+
+// cnt[f] = (or +=, with accumulate) the length of the run of field f, written
+// at its last key by one thread; fields with no run keep what cnt holds.
+// low_out, when given, receives key & low_mask.
+__global__ void field_run_lengths(const uint64_t *__restrict__ k, uint64_t n, int shift, uint32_t mask, uint32_t nv,
+                                  const uint32_t *__restrict__ run_start, uint32_t *__restrict__ cnt,
+                                  uint32_t *__restrict__ low_out, uint32_t low_mask, int accumulate) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t key = k[i];
+        const uint32_t f = (uint32_t)(key >> shift) & mask;
+        if (low_out) low_out[i] = (uint32_t)key & low_mask;
+        const bool last = i + 1 == n || ((uint32_t)(k[i + 1] >> shift) & mask) != f;
+        if (last && f < nv) {
+            const uint32_t len = (uint32_t)(i + 1) - run_start[f];
+            cnt[f] = accumulate ? cnt[f] + len : len;
+        }
+    }
+}
+
+int count_field_runs(const uint64_t *keys, uint64_t n, int shift, uint32_t mask, uint32_t nv, uint32_t *run_start,
+                     uint32_t *cnt, uint32_t *low_out, uint32_t low_mask, hipStream_t s, bool accumulate) {
+    if (n == 0) return GCHI_OK;
+    const unsigned g = grid_for(n, 256, 65536);
+    hipLaunchKernelGGL(field_run_starts, dim3(g), dim3(256), 0, s, keys, n, shift, mask, nv, run_start);
+    hipLaunchKernelGGL(field_run_lengths, dim3(g), dim3(256), 0, s, keys, n, shift, mask, nv, run_start, cnt, low_out,
+                       low_mask, accumulate ? 1 : 0);
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
 }
 
-template <typename C>
-static void csc_layout(C &c, uint64_t E, uint32_t N, uint64_t **keys, uint32_t **indeg, uint32_t **err, void **sw,
-                       size_t *sw_bytes) {
+__global__ void pack_csc_keys(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, int sb,
+                              uint64_t *keys, uint32_t *err) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t s = src[i], d = dst[i];
+        if (s >= nv || d >= nv) atomicOr(err, 1u);
+        keys[i] = ((uint64_t)d << sb) | (uint64_t)s;
+    }
+}
+
+struct CscBits {
+    int vb;  // bits of a vertex id
+    int sb;  // bits the src field takes in a key (vb rounded up to 8)
+};
+static CscBits csc_bits(uint32_t N) {
     int vb = bits_for(N ? N - 1 : 0);
     if (vb == 0) vb = 1;
+    return CscBits{vb, (vb + 7) / 8 * 8};
+}
+
+template <typename C>
+static void csc_layout(C &c, uint64_t E, uint32_t N, uint64_t **keys, uint32_t **indeg, uint32_t **run_start,
+                       uint32_t **err, void **sw, size_t *sw_bytes) {
+    const CscBits cb = csc_bits(N);
     *keys = c.template take<uint64_t>(E ? E : 1);
     *indeg = c.template take<uint32_t>(N ? N : 1);
+    *run_start = c.template take<uint32_t>(N ? N : 1);
     *err = c.template take<uint32_t>(1);
-    size_t ss = std::max(sort_workspace_bytes(E, 2 * vb, false), scan_workspace((size_t)N + 1));
+    size_t ss = std::max(sort_workspace_bytes(E, std::max(cb.sb, cb.vb), false), scan_workspace((size_t)N + 1));
     *sw = c.template take<char>(ss);
     *sw_bytes = ss;
 }
@@ -90,22 +105,30 @@ int csc_build(const uint32_t *src, const uint32_t *dst, uint64_t E, uint32_t N, 
     GCHI_CHECK(in_off && (E == 0 || (src && dst && in_src)), GCHI_ERR_INVALID, "csc_build: null pointer");
     Carver c(work, work_bytes);
     uint64_t *keys;
-    uint32_t *indeg, *err;
+    uint32_t *indeg, *run_start, *err;
     void *sw;
     size_t swb;
-    csc_layout(c, E, N, &keys, &indeg, &err, &sw, &swb);
+    csc_layout(c, E, N, &keys, &indeg, &run_start, &err, &sw, &swb);
     GCHI_CHECK(c.ok(), GCHI_ERR_WORKSPACE, "csc_build: workspace too small");
     uint32_t *ind = indeg_out ? indeg_out : indeg;
+    const CscBits cb = csc_bits(N);
+    const uint32_t vmask = cb.vb >= 32 ? 0xffffffffu : ((1u << cb.vb) - 1u);
     launch_fill_u32(err, 0u, 1, s);
-    GCHI_TRY(degrees(src, dst, E, N, outdeg, ind, err, s));
-    int vb = bits_for(N ? N - 1 : 0);
-    if (vb == 0) vb = 1;
+    if (outdeg) launch_fill_u32(outdeg, 0u, N, s);
+    launch_fill_u32(ind, 0u, N, s);
     if (E) {
-        hipLaunchKernelGGL(pack_keys, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, dst, src, E, vb, keys);
+        const unsigned g = grid_for(E, 256, 65536);
+        hipLaunchKernelGGL(pack_csc_keys, dim3(g), dim3(256), 0, s, src, dst, E, N, cb.sb, keys, err);
         GCHI_LAUNCH_CHECK();
-        GCHI_TRY(radix_sort_u64(keys, nullptr, E, 2 * vb, sw, swb, s));
-        hipLaunchKernelGGL(low_bits_u32, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, keys, E, vb, in_src);
-        GCHI_LAUNCH_CHECK();
+        // src passes: keys ordered by src; out-degrees from its runs
+        uint64_t *by_src = nullptr, *by_dst = nullptr;
+        GCHI_TRY(radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, cb.sb, sw, swb, s, &by_src));
+        if (outdeg) GCHI_TRY(count_field_runs(by_src, E, 0, vmask, N, run_start, outdeg, nullptr, 0u, s));
+        // dst passes (stable, so src order within a dst run is kept); the
+        // buffer not holding by_src takes the other half of the ping-pong
+        uint64_t *other = by_src == keys ? nullptr : keys;
+        GCHI_TRY(radix_sort_u64_bits(by_src, other, nullptr, E, cb.sb, cb.sb + cb.vb, sw, swb, s, &by_dst));
+        GCHI_TRY(count_field_runs(by_dst, E, cb.sb, vmask, N, run_start, ind, in_src, vmask, s));
     }
     GCHI_TRY(exclusive_scan_u32(ind, in_off, N, sw, swb, s));
     return GCHI_OK;
@@ -114,10 +137,10 @@ int csc_build(const uint32_t *src, const uint32_t *dst, uint64_t E, uint32_t N, 
 size_t csc_workspace(uint64_t E, uint32_t N) {
     SizerAdapter a;
     uint64_t *k;
-    uint32_t *i, *e;
+    uint32_t *i, *r, *e;
     void *sw;
     size_t swb;
-    csc_layout(a, E, N, &k, &i, &e, &sw, &swb);
+    csc_layout(a, E, N, &k, &i, &r, &e, &sw, &swb);
     return a.z.off;
 }
 

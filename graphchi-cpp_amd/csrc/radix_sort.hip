@@ -255,25 +255,32 @@ size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
     return a.z.off;
 }
 
-// Sorts keys (and vals) in place on their low key_bits bits.
-int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, void *work, size_t work_bytes,
-                   hipStream_t s) {
-    if (n <= 1 || key_bits <= 0) return GCHI_OK;
+// Sorts keys (and vals) on key bits [lo, hi), stably, in 8-bit digits from bit lo
+// up.  The passes alternate between keys and a second buffer: alt when given,
+// else one carved from the workspace.  With out == nullptr the result is copied
+// back into keys; otherwise *out receives the buffer that holds it (keys or the
+// second buffer) and nothing is copied.  Sorting on [0, b) and then on [b, hi)
+// gives the same order as sorting on [0, hi) at once (every pass is stable).
+int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
+                        size_t work_bytes, hipStream_t s, uint64_t **out) {
+    if (out) *out = keys;
+    if (n <= 1 || hi <= lo) return GCHI_OK;
     GCHI_CHECK(keys, GCHI_ERR_INVALID, "sort: null keys");
-    GCHI_CHECK(key_bits <= 64, GCHI_ERR_INVALID, "sort: key_bits > 64");
+    GCHI_CHECK(lo >= 0 && hi <= 64, GCHI_ERR_INVALID, "sort: key bits outside [0, 64)");
     GCHI_CHECK(n < (1ull << 30), GCHI_ERR_INVALID, "sort: n must be < 2^30");
+    GCHI_CHECK(!(out && vals), GCHI_ERR_INVALID, "sort: values are always copied back");
     const bool hv = vals != nullptr;
-    GCHI_CHECK(work_bytes >= sort_workspace_bytes(n, key_bits, hv), GCHI_ERR_WORKSPACE,
-               "sort: workspace too small");
+    const int bits = hi - lo;
+    GCHI_CHECK(work_bytes >= sort_workspace_bytes(n, bits, hv), GCHI_ERR_WORKSPACE, "sort: workspace too small");
     Carver c(work, work_bytes);
     SortWork w;
-    sort_layout(c, n, key_bits, hv, &w);
-    const int np = npasses_for(key_bits);
+    sort_layout(c, n, bits, hv, &w);
+    const int np = npasses_for(bits);
     const uint64_t ntiles = div_up(n, kRsTile);
     PassDesc pd;
     for (int p = 0; p < kRsMaxPasses; p++) {
-        pd.shift[p] = 8 * p;
-        pd.bits[p] = p < np ? std::min(8, key_bits - 8 * p) : 0;
+        pd.shift[p] = lo + 8 * p;
+        pd.bits[p] = p < np ? std::min(8, bits - 8 * p) : 0;
     }
     launch_fill_u32(w.ghist, 0u, (size_t)np * kRsRadix, s);
     launch_fill_u32(w.status, 0u, (size_t)np * ntiles * kRsRadix, s);
@@ -282,7 +289,7 @@ int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, voi
     hipLaunchKernelGGL(rs_histogram, dim3(hblocks), dim3(kRsThreads), 0, s, keys, n, np, pd, w.ghist);
     hipLaunchKernelGGL(rs_hist_scan, dim3(np), dim3(kRsRadix), 0, s, w.ghist);
     GCHI_LAUNCH_CHECK();
-    uint64_t *ka = keys, *kb = w.ktmp;
+    uint64_t *ka = keys, *kb = alt ? alt : w.ktmp;
     uint32_t *va = vals, *vb = w.vtmp;
     for (int p = 0; p < np; p++) {
         uint32_t *st = w.status + (size_t)p * ntiles * kRsRadix;
@@ -298,13 +305,23 @@ int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, voi
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (ka != keys) {
+    if (out) {
+        *out = ka;
+    } else if (ka != keys) {
         unsigned b = (unsigned)std::min<uint64_t>(div_up(n, 256), 65536);
         hipLaunchKernelGGL(copy_u64, dim3(b), dim3(256), 0, s, ka, keys, n);
         if (hv) hipLaunchKernelGGL(copy_u32, dim3(b), dim3(256), 0, s, va, vals, n);
         GCHI_LAUNCH_CHECK();
     }
     return GCHI_OK;
+}
+
+// Sorts keys (and vals) in place on their low key_bits bits.
+int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, void *work, size_t work_bytes,
+                   hipStream_t s) {
+    if (n <= 1 || key_bits <= 0) return GCHI_OK;
+    GCHI_CHECK(key_bits <= 64, GCHI_ERR_INVALID, "sort: key_bits > 64");
+    return radix_sort_u64_bits(keys, nullptr, vals, n, 0, key_bits, work, work_bytes, s, nullptr);
 }
 
 // ---- (src,dst) pair sort by (dst,src) ----

@@ -100,14 +100,33 @@ __global__ void sh_interval_cut(const uint64_t *P, uint32_t n, uint32_t max_id, 
     *nshards_out = shardnum;
 }
 
-__global__ void sh_keys(const uint32_t *src, const uint32_t *dst, uint64_t e, const uint32_t *ends, int ns, int idb,
-                        uint64_t *keys) {
+// (src << db) | dst, db = the id bits rounded up to a whole radix digit, so that
+// the first db/8 sort passes order the edges by dst alone.
+__global__ void sh_pack(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t nv, int db, uint64_t *keys,
+                        uint32_t *err) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
-        uint32_t d = dst[i];
-        int p = 0;
-        while (p < ns - 1 && d > ends[p]) p++;
-        keys[i] = ((uint64_t)p << (2 * idb)) | ((uint64_t)src[i] << idb) | d;
+        const uint32_t sv = src[i], dv = dst[i];
+        if (sv >= nv || dv >= nv) atomicOr(err, 1u);
+        keys[i] = ((uint64_t)sv << db) | dv;
+    }
+}
+
+// (src << db | dst) -> (shard << 2*idb | src << idb | dst), shard = the first
+// interval whose end is >= dst (the last one for larger ids).
+__global__ void sh_rekey(uint64_t *keys, uint64_t e, int db, int idb, const uint32_t *ends, int ns) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t m = (1ull << idb) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        const uint64_t k = keys[i];
+        const uint32_t d = (uint32_t)(k & m), sv = (uint32_t)((k >> db) & m);
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (d <= ends[mid]) hi = mid;
+            else lo = mid + 1;
+        }
+        keys[i] = ((uint64_t)lo << (2 * idb)) | ((uint64_t)sv << idb) | d;
     }
 }
 
@@ -290,13 +309,15 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
     Sizer z;
     z.take<uint32_t>(N);            // indeg
     z.take<uint32_t>(N);            // outdeg
+    z.take<uint32_t>(N);            // run starts
     z.take<uint64_t>((size_t)N + 1);  // P
     z.take<uint32_t>(nshards);      // ends
     z.take<uint64_t>(nshards);      // counts
     z.take<int>(4);                 // nshards_out, status
     z.take<uint32_t>(1);            // err
     z.take<uint64_t>(E + 1);        // keys
-    size_t swb = std::max(sort_workspace_bytes(E, pb + 2 * idb, false), scan_workspace((size_t)N + 1));
+    const int db = (idb + 7) / 8 * 8;
+    size_t swb = std::max(sort_workspace_bytes(E, std::max(db, pb), false), scan_workspace((size_t)N + 1));
     z.take<char>(swb);
     void *scratch = nullptr;
     if (hipMalloc(&scratch, z.off) != hipSuccess) {
@@ -307,6 +328,7 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
     Carver sc(scratch, z.off);
     uint32_t *indeg = sc.take<uint32_t>(N);
     uint32_t *outdeg = sc.take<uint32_t>(N);
+    uint32_t *run_start = sc.take<uint32_t>(N);
     uint64_t *P = sc.take<uint64_t>((size_t)N + 1);
     uint32_t *d_ends = sc.take<uint32_t>(nshards);
     uint64_t *d_counts = sc.take<uint64_t>(nshards);
@@ -324,8 +346,25 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
         }
         return st;
     };
+    // Degrees from runs of equal ids in sorted keys (device-wide atomics on
+    // random counters would be far slower): the dst passes give in-degrees,
+    // the src passes the (src, dst) order and out-degrees.
     launch_fill_u32(err, 0u, 1, s);
-    int st = degrees(d_src, d_dst, E, N, outdeg, indeg, err, s);
+    launch_fill_u32(indeg, 0u, N, s);
+    launch_fill_u32(outdeg, 0u, N, s);
+    const uint32_t idmask = idb >= 32 ? 0xffffffffu : ((1u << idb) - 1u);
+    uint64_t *by_src_dst = keys;
+    int st = GCHI_OK;
+    if (E) {
+        hipLaunchKernelGGL(sh_pack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N, db, keys, err);
+        uint64_t *by_dst = nullptr;
+        st = radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, db, sw, swb, s, &by_dst);
+        if (st == GCHI_OK) st = count_field_runs(by_dst, E, 0, idmask, N, run_start, indeg, nullptr, 0u, s);
+        if (st == GCHI_OK)
+            st = radix_sort_u64_bits(by_dst, by_dst == keys ? nullptr : keys, nullptr, E, db, db + idb, sw, swb, s,
+                                     &by_src_dst);
+        if (st == GCHI_OK) st = count_field_runs(by_src_dst, E, db, idmask, N, run_start, outdeg, nullptr, 0u, s);
+    }
     if (st == GCHI_OK) {
         hipLaunchKernelGGL(sh_degrees_interleave, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, indeg, outdeg, N,
                            sh->degs);
@@ -350,11 +389,14 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
         }
     }
     if (st == GCHI_OK && E) {
-        hipLaunchKernelGGL(sh_keys, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, d_ends,
-                           sh->nshards, idb, keys);
-        st = radix_sort_u64(keys, nullptr, E, pb + 2 * idb, sw, swb, s);
+        // stable pass on the shard id: (shard, src, dst) order
+        hipLaunchKernelGGL(sh_rekey, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, by_src_dst, E, db, idb, d_ends,
+                           sh->nshards);
+        uint64_t *sorted = nullptr;
+        st = radix_sort_u64_bits(by_src_dst, by_src_dst == keys ? nullptr : keys, nullptr, E, 2 * idb,
+                                 2 * idb + pb, sw, swb, s, &sorted);
         if (st == GCHI_OK)
-            hipLaunchKernelGGL(sh_unpack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, keys, E, idb, sh->src,
+            hipLaunchKernelGGL(sh_unpack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, sorted, E, idb, sh->src,
                                sh->dst);
         hipError_t e = hipStreamSynchronize(s);
         if (st == GCHI_OK && e != hipSuccess) st = hip_fail(e, "shard_build", __FILE__, __LINE__);

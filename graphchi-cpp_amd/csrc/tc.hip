@@ -27,28 +27,19 @@ namespace gchi {
 constexpr int kTcWaves = 4;
 constexpr int kTcLds = 1024;
 
-// in+out degree, combined per workgroup in an LDS hash table first (skewed
-// graphs would otherwise serialise on a few global counters)
-constexpr int kTcDegSlots = 4096;
-
-__global__ __launch_bounds__(256) void tc_degrees(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n,
-                                                  uint32_t *deg, uint32_t *err) {
-    __shared__ LdsCombiner<kTcDegSlots> comb;
-    comb_clear<kTcDegSlots, 256>(comb);
-    __syncthreads();
+// Degree keys: (dst << sb) | src with sb = the id field width rounded up to a
+// whole radix digit; the id field is wide enough to hold n itself, which marks
+// self-loops (and out-of-range ids, reported through err) so that run counting
+// skips them.
+__global__ void tc_degree_pack(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, int sb,
+                               uint64_t *keys, uint32_t *err) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
         uint32_t s = src[i], d = dst[i];
-        if (s >= n || d >= n) {
-            atomicOr(err, 1u);
-            continue;
-        }
-        if (s == d) continue;
-        comb_add(comb, s, deg);
-        comb_add(comb, d, deg);
+        if (s >= n || d >= n) atomicOr(err, 1u);
+        if (s >= n || d >= n || s == d) s = d = n;
+        keys[i] = ((uint64_t)d << sb) | s;
     }
-    __syncthreads();
-    comb_flush<kTcDegSlots, 256>(comb, deg);
 }
 
 __global__ void tc_degree_keys(const uint32_t *deg, uint32_t n, int idb, uint64_t *keys) {
@@ -68,7 +59,7 @@ __global__ void tc_orient_keys(const uint32_t *src, const uint32_t *dst, uint64_
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
         uint32_t s = src[i], d = dst[i];
-        if (s == d || s >= n || d >= n) {  // out-of-range ids are reported by tc_degrees
+        if (s == d || s >= n || d >= n) {  // out-of-range ids are reported by tc_degree_pack
             keys[i] = sentinel;
             continue;
         }
@@ -86,15 +77,34 @@ __global__ void tc_unique_flags(const uint64_t *keys, uint64_t e, uint64_t senti
     }
 }
 
+// Runs of equal lo = key >> idb in the sorted oriented keys (sentinel keys,
+// which sort last, excluded).  lo_start[lo] = pos at the run's first key.
+__global__ void tc_lo_run_starts(const uint64_t *keys, const uint32_t *pos, uint64_t e, int idb, uint64_t sentinel,
+                                 uint32_t *lo_start) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
+        const uint64_t k = keys[i];
+        if (k == sentinel) continue;
+        const uint64_t lo = k >> idb;
+        if (i == 0 || (keys[i - 1] >> idb) != lo) lo_start[lo] = pos[i];
+    }
+}
+
+// adj[pos] = hi of every unique key; at a run's last key, lo_count[lo] = the
+// unique keys of the run (no atomics: one writer per run).
 __global__ void tc_scatter_unique(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos, uint64_t e,
-                                  int idb, uint32_t *adj, uint32_t *lo_count) {
+                                  int idb, uint64_t sentinel, const uint32_t *lo_start, uint32_t *adj,
+                                  uint32_t *lo_count) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t m = (1ull << idb) - 1ull;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
-        if (!flags[i]) continue;
-        uint64_t k = keys[i];
-        adj[pos[i]] = (uint32_t)(k & m);
-        atomicAdd(&lo_count[k >> idb], 1u);
+        const uint64_t k = keys[i];
+        if (k == sentinel) continue;
+        const uint64_t lo = k >> idb;
+        const uint32_t f = flags[i], p = pos[i];
+        if (f) adj[p] = (uint32_t)(k & m);
+        const bool last = i + 1 == e || keys[i + 1] == sentinel || (keys[i + 1] >> idb) != lo;
+        if (last) lo_count[lo] = p + f - lo_start[lo];
     }
 }
 
@@ -231,6 +241,27 @@ static void tc_prep_layout(C &c, uint64_t E, uint32_t N, uint32_t **deg, uint64_
     *swb = s;
 }
 
+// deg[v] += in + out degree of v without self-loops (deg zeroed by the caller),
+// from runs of equal ids: the src passes order the keys by src, the dst passes
+// by dst.  vkeys (N+1 entries) serves as run-start scratch, ekeys and sw as the
+// sort buffers.
+static int tc_degree_counts(const uint32_t *src, const uint32_t *dst, uint64_t E, uint32_t N, uint32_t *deg,
+                            uint32_t *err, uint64_t *vkeys, uint64_t *ekeys, void *sw, size_t swb, hipStream_t s) {
+    if (!E) return GCHI_OK;
+    const int fb = bits_for(N);  // wide enough for n, the skip mark
+    const int sb = (fb + 7) / 8 * 8;
+    const uint32_t fmask = fb >= 32 ? 0xffffffffu : ((1u << fb) - 1u);
+    uint32_t *run_start = reinterpret_cast<uint32_t *>(vkeys);
+    hipLaunchKernelGGL(tc_degree_pack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, src, dst, E, N, sb, ekeys, err);
+    GCHI_LAUNCH_CHECK();
+    uint64_t *by_src = nullptr, *by_dst = nullptr;
+    GCHI_TRY(radix_sort_u64_bits(ekeys, nullptr, nullptr, E, 0, sb, sw, swb, s, &by_src));
+    GCHI_TRY(count_field_runs(by_src, E, 0, fmask, N, run_start, deg, nullptr, 0u, s, true));
+    GCHI_TRY(radix_sort_u64_bits(by_src, by_src == ekeys ? nullptr : ekeys, nullptr, E, sb, sb + fb, sw, swb, s,
+                                 &by_dst));
+    return count_field_runs(by_dst, E, sb, fmask, N, run_start, deg, nullptr, 0u, s, true);
+}
+
 }  // namespace gchi
 
 using namespace gchi;
@@ -270,8 +301,7 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
     GCHI_CHECK(idb + degb <= 64, GCHI_ERR_UNSUPPORTED, "tc_prepare: degree/id keys exceed 64 bits");
     launch_fill_u32(deg, 0u, N, s);
     launch_fill_u32(err, 0u, 1, s);
-    if (E)
-        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(E, 256 * 64, 2048)), dim3(256), 0, s, d_src, d_dst, E, N, deg, err);
+    GCHI_TRY(tc_degree_counts(d_src, d_dst, E, N, deg, err, vkeys, ekeys, sw, swb, s));
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();
     GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
@@ -288,8 +318,11 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
                            flags);
         GCHI_LAUNCH_CHECK();
         GCHI_TRY(exclusive_scan_u32(flags, pos, E, sw, swb, s));
+        uint32_t *lo_start = reinterpret_cast<uint32_t *>(vkeys);  // vertexmap is built: vkeys is free
+        hipLaunchKernelGGL(tc_lo_run_starts, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, pos, E, idb,
+                           sentinel, lo_start);
         hipLaunchKernelGGL(tc_scatter_unique, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, flags, pos, E,
-                           idb, d_adj, lo_count);
+                           idb, sentinel, lo_start, d_adj, lo_count);
         GCHI_LAUNCH_CHECK();
     }
     GCHI_TRY(exclusive_scan_u32(lo_count, d_off, N, sw, swb, s));
@@ -383,9 +416,7 @@ extern "C" int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, u
     const int degb = bits_for(2 * nedges);
     launch_fill_u32(deg, 0u, N, s);
     launch_fill_u32(err, 0u, 1, s);
-    if (nedges)
-        hipLaunchKernelGGL(tc_degrees, dim3(grid_for(nedges, 256 * 64, 2048)), dim3(256), 0, s, d_src, d_dst, nedges, N,
-                           deg, err);
+    GCHI_TRY(tc_degree_counts(d_src, d_dst, nedges, N, deg, err, vkeys, ekeys, sw, swb, s));
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();
     GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
