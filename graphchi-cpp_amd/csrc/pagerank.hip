@@ -935,38 +935,66 @@ __global__ void plan_reg_rel(const uint64_t *reg64, const uint32_t *vbase, uint3
     }
 }
 
-// one wavefront per row: copy the row's in-edges into the regular or hub array,
-// mapped to gather-layout positions when grank is given.  A source id outside
-// its graph, or one without a layout position (its out-degree says 0), sets
-// *bad and is stored as 0, so no sweep ever gathers out of bounds; the plan is
-// refused.
-__global__ void plan_copy(CscView cv, const uint32_t *hub_flag, const uint32_t *hub_idx, const uint32_t *reg_rel,
-                          const uint64_t *ebase, const uint64_t *hub_off, const uint32_t *grank, uint32_t *reg_src,
-                          uint32_t *hub_src, uint32_t *bad) {
+// Copies the rows' in-edges into the regular and hub arrays, mapped to
+// gather-layout positions when grank is given.  A source id outside its graph,
+// or one without a layout position (its out-degree says 0), sets *bad and is
+// stored as 0, so no sweep ever gathers out of bounds; the plan is refused.
+// Regular rows (fewer than W <= 4096 in-edges) take one wavefront each; hub
+// rows are copied per 512-edge chunk, one wavefront per chunk, so that a row
+// of 10^5 or more in-edges does not hold up the whole copy.
+GCHI_DEV uint32_t plan_map_src(uint32_t u, uint32_t idoff, uint32_t lim, const uint32_t *grank, uint32_t *bad) {
+    bool ok = u < lim;
+    u += idoff;
+    if (ok && grank) {
+        u = grank[u];
+        ok = u != 0xffffffffu;
+    }
+    if (!ok) {
+        atomicOr(bad, 1u);
+        u = 0;
+    }
+    return u;
+}
+
+__global__ void plan_copy(CscView cv, const uint32_t *hub_flag, const uint32_t *reg_rel, const uint64_t *ebase,
+                          const uint32_t *grank, uint32_t *reg_src, uint32_t *bad) {
     const int lane = threadIdx.x & (kWave - 1);
-    uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
     for (uint64_t v = wid; v < cv.N; v += nw) {
+        if (hub_flag[v]) continue;
         const uint32_t g = graph_of(cv.vbase, cv.G, (uint32_t)v);
         const uint32_t b = cv.off_ext[v + g], e = cv.off_ext[v + g + 1];
         const uint32_t *src = reinterpret_cast<const uint32_t *>((uintptr_t)cv.g_src[g]);
         const uint32_t idoff = cv.g_idoff[g];
         const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.N;
-        uint32_t *dst = hub_flag[v] ? hub_src + hub_off[hub_idx[v]] : reg_src + ebase[g] + reg_rel[v + g];
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            uint32_t u = src[k];
-            bool ok = u < lim;
-            u += idoff;
-            if (ok && grank) {
-                u = grank[u];
-                ok = u != 0xffffffffu;
-            }
-            if (!ok) {
-                atomicOr(bad, 1u);
-                u = 0;
-            }
-            dst[k - b] = u;
+        uint32_t *dst = reg_src + ebase[g] + reg_rel[v + g];
+        for (uint32_t k = b + lane; k < e; k += kWave) dst[k - b] = plan_map_src(src[k], idoff, lim, grank, bad);
+    }
+}
+
+__global__ void plan_copy_hubs(CscView cv, const uint32_t *hub_v, const uint32_t *hub_chunk_off, uint32_t nhub,
+                               uint32_t nchunk, const uint64_t *chunk_beg, const uint32_t *chunk_len,
+                               const uint32_t *grank, uint32_t *hub_src, uint32_t *bad) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
+    for (uint32_t c = wid; c < nchunk; c += nw) {
+        uint32_t lo = 0, hi = nhub - 1;  // the hub of chunk c: largest h with hub_chunk_off[h] <= c
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo + 1) / 2;
+            if (hub_chunk_off[mid] <= c) lo = mid;
+            else hi = mid - 1;
         }
+        const uint32_t v = hub_v[lo], j = c - hub_chunk_off[lo];
+        const uint32_t g = graph_of(cv.vbase, cv.G, v);
+        const uint32_t b = cv.off_ext[v + g] + j * kHubChunk;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>((uintptr_t)cv.g_src[g]) + b;
+        const uint32_t idoff = cv.g_idoff[g];
+        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.N;
+        uint32_t *dst = hub_src + chunk_beg[c];
+        const uint32_t len = chunk_len[c];
+        for (uint32_t k = lane; k < len; k += kWave) dst[k] = plan_map_src(src[k], idoff, lim, grank, bad);
     }
 }
 
@@ -1289,10 +1317,12 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
     launch_fill_u32(p->counters, 0u, 2, s);
     if (E)
         hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, cv,
-                           hub_flag, hub_idx, p->reg_off, p->ebase, hub_off, grank, p->reg_src, p->hub_src, bad);
+                           hub_flag, p->reg_off, p->ebase, grank, p->reg_src, bad);
     if (nhub) {
         hipLaunchKernelGGL(plan_chunks, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, hub_off, hub_choff, nhub,
                            p->chunk_beg, p->chunk_len);
+        hipLaunchKernelGGL(plan_copy_hubs, dim3(grid_for((uint64_t)nchunk * kWave, 256, 1u << 16)), dim3(256), 0, s,
+                           cv, hub_v, hub_choff, nhub, nchunk, p->chunk_beg, p->chunk_len, grank, p->hub_src, bad);
         hipLaunchKernelGGL(plan_hub_graphs, dim3(grid_for(nhub, 256, 4096)), dim3(256), 0, s, p->hub_v, nhub,
                            p->vbase, G, p->hub_graph);
     }

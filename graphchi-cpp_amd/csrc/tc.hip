@@ -124,7 +124,7 @@ GCHI_DEV int bsearch_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t x
 
 __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_t *__restrict__ off,
                                                                    const uint32_t *__restrict__ adj, uint32_t n,
-                                                                   unsigned long long *cnt,
+                                                                   uint32_t min_da, unsigned long long *cnt,
                                                                    unsigned long long *total, uint32_t *work) {
     __shared__ uint32_t sA[kTcWaves][kTcLds];
     __shared__ uint32_t sW[kTcWaves][kTcLds];  // triangles found per position of N+(a) (the w credit)
@@ -133,12 +133,13 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
     __shared__ uint32_t sCnt[kTcWaves][kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     for (;;) {
-        uint32_t a = 0;
-        if (lane == 0) a = atomicAdd(work, 1u);
-        a = __shfl(a, 0, kWave);
-        if (a >= n) break;
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        t = __shfl(t, 0, kWave);
+        if (t >= n) break;
+        const uint32_t a = n - 1 - t;  // highest rank (most work) first
         const uint32_t ab = off[a], da = off[a + 1] - ab;
-        if (da < 2) continue;
+        if (da < 2 || da <= min_da) continue;
         const uint32_t *A = adj + ab;
         const bool lds = da <= (uint32_t)kTcLds;
         if (lds)
@@ -213,6 +214,124 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Hash form, for |N+(a)| <= kTcHashMax (every vertex of a degree-ordered
+// graph with up to ~500K edges, and nearly all of larger ones).  Vertices are
+// taken from the highest rank down: the work per vertex grows with its rank,
+// so the longest ones start first.  One wavefront per vertex a puts N+(a) into
+// an LDS hash table (open addressing, linear
+// probing, at most half full, a quarter when |N+(a)| <= kTcHashSlots / 4), then
+// walks b in N+(a) one at a time and looks up every w in N+(b) lane-wide, four
+// w per lane in flight.  A hit is the triangle a < b < w.  Counts: w's and b's
+// credits go to their slots (16-bit halves of LDS words: at most 2|N+(a)| per
+// slot) and are added to the global counts once per vertex a.
+constexpr int kTcHashWaves = 4;
+constexpr uint32_t kTcHashSlots = 2048;
+constexpr uint32_t kTcHashMax = kTcHashSlots / 2;
+constexpr uint32_t kTcEmpty = 0xffffffffu;
+
+GCHI_DEV uint32_t tc_slot(uint32_t w, int sh) { return (w * 2654435761u) >> sh; }
+
+GCHI_DEV int tc_find(const uint32_t *keys, uint32_t mask, int sh, uint32_t w) {
+    uint32_t h = tc_slot(w, sh);
+    for (;;) {
+        const uint32_t k = keys[h];
+        if (k == w) return (int)h;
+        if (k == kTcEmpty) return -1;
+        h = (h + 1) & mask;
+    }
+}
+
+GCHI_DEV void tc_credit(uint32_t *cw, uint32_t slot, uint32_t c) { atomicAdd(&cw[slot >> 1], c << ((slot & 1) * 16)); }
+
+__global__ __launch_bounds__(kTcHashWaves *kWave) void tc_count_hash_kernel(const uint32_t *__restrict__ off,
+                                                                            const uint32_t *__restrict__ adj,
+                                                                            uint32_t n, unsigned long long *cnt,
+                                                                            unsigned long long *total,
+                                                                            uint32_t *work) {
+    __shared__ uint32_t sKey[kTcHashWaves][kTcHashSlots];
+    __shared__ uint32_t sCw[kTcHashWaves][kTcHashSlots / 2];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    uint32_t *keys = sKey[wave];
+    uint32_t *cw = sCw[wave];
+    unsigned long long acc_total = 0;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        t = __shfl(t, 0, kWave);
+        if (t >= n) break;
+        const uint32_t a = n - 1 - t;  // highest rank (most work) first
+        const uint32_t ab = off[a], da = off[a + 1] - ab;
+        if (da < 2 || da > kTcHashMax) continue;
+        const uint32_t want = 4 * da;  // slots: pow2 >= 4 da, at most kTcHashSlots
+        const int lg = min(32 - __clz((int)(want - 1)), 11);
+        const uint32_t S = 1u << lg, mask = S - 1;
+        const int sh = 32 - lg;
+        for (uint32_t k = lane; k < S; k += kWave) keys[k] = kTcEmpty;
+        for (uint32_t k = lane; k < S / 2; k += kWave) cw[k] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t *A = adj + ab;
+        for (uint32_t k = lane; k < da; k += kWave) {
+            const uint32_t x = A[k];
+            uint32_t h = tc_slot(x, sh);
+            while (atomicCAS(&keys[h], kTcEmpty, x) != kTcEmpty) h = (h + 1) & mask;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t acc_a = 0;
+        // the last element of N+(a) closes no triangle here (no w above it)
+        for (uint32_t bc = 0; bc + 1 < da; bc += kWave) {
+            const uint32_t k = bc + lane;
+            const bool valid = k + 1 < da;
+            const uint32_t b = valid ? A[k] : 0u;
+            const uint32_t bb = valid ? off[b] : 0u;
+            const uint32_t lb = valid ? off[b + 1] - bb : 0u;
+            const uint32_t nb = min((uint32_t)kWave, da - 1 - bc);
+            for (uint32_t j = 0; j < nb; j++) {
+                const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)j);
+                const uint32_t bbj = (uint32_t)__builtin_amdgcn_readlane((int)bb, (int)j);
+                const uint32_t lbj = (uint32_t)__builtin_amdgcn_readlane((int)lb, (int)j);
+                const uint32_t *B = adj + bbj;
+                uint32_t hits = 0;
+                for (uint32_t t = lane; t < lbj; t += 4 * kWave) {
+                    uint32_t w[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t tq = t + q * kWave;
+                        w[q] = tq < lbj ? B[tq] : kTcEmpty;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (w[q] == kTcEmpty) continue;
+                        const int h = tc_find(keys, mask, sh, w[q]);
+                        if (h >= 0) {
+                            tc_credit(cw, (uint32_t)h, 1u);
+                            hits++;
+                        }
+                    }
+                }
+                hits = wave_sum(hits);
+                if (hits) {
+                    if (lane == 0) tc_credit(cw, (uint32_t)tc_find(keys, mask, sh, bj), hits);
+                    acc_a += hits;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = lane; k < S; k += kWave) {
+            const uint32_t key = keys[k];
+            const uint32_t c = (cw[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+            if (key != kTcEmpty && c) atomicAdd(&cnt[key], (unsigned long long)c);
+        }
+        if (lane == 0 && acc_a) atomicAdd(&cnt[a], (unsigned long long)acc_a);
+        acc_total += acc_a;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0 && acc_total) atomicAdd(total, acc_total);
 }
 
 __global__ void tc_to_u32(const unsigned long long *cnt, uint32_t n, uint32_t *out, uint32_t *overflow) {
@@ -368,8 +487,10 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
     }
-    hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices, cnt,
-                       tot, wk);
+    hipLaunchKernelGGL(tc_count_hash_kernel, dim3(ncu * 6), dim3(kTcHashWaves * kWave), 0, s, d_off, d_adj,
+                       nvertices, cnt, tot, wk);
+    hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices,
+                       kTcHashMax, cnt, tot, wk + 1);
     hipLaunchKernelGGL(tc_to_u32, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, cnt, nvertices,
                        d_per_vertex, d_overflow);
     GCHI_HIP(hipMemcpyAsync(d_total, tot, 8, hipMemcpyDeviceToDevice, s));
