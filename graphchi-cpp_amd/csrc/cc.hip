@@ -11,6 +11,17 @@
 //   gchi_cc_labelprop_sweep : one synchronous min-propagation sweep
 //       (label_out[v] = min(label_in[v], label_in[u] for every edge u-v)) with a
 //       count of changed labels -- the reference's per-iteration semantics.
+//   gchi_cc_scheduled : the reference's update under its selective scheduler
+//       (scheduler on: connectedcomponents.cpp:81,85,112 with the two bitsets
+//       of bitset_scheduler.hpp:38-104).  Round 0 schedules every vertex; a
+//       vertex whose label drops schedules all its neighbours for the next
+//       round; the rounds end when no label drops.  Device form: byte flags for
+//       the current and next schedule (plain stores of 1 instead of atomic bit
+//       sets), the scheduled vertices compacted per round into a list of short
+//       rows (one lane each) and 4096-edge chunks of long rows (one wavefront
+//       each, atomicMin on the row's label), labels updated in place -- so a
+//       drop is seen by rows evaluated later in the same round, as add_task's
+//       also_this_iteration does.  Same fixed point as the other forms.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -107,6 +118,164 @@ __global__ void cc_iota(uint32_t *l, uint32_t n) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) l[v] = v;
 }
 
+// ---- selective scheduling ----
+constexpr uint32_t kCcsShort = 64;    // rows up to this length: one lane each
+constexpr uint32_t kCcsChunk = 4096;  // longer rows: one wavefront per chunk
+
+// ctr: [0] short rows, [1] long-row chunks.  stat: [0] labels dropped,
+// [1] rows evaluated, [2] neighbour ids >= n seen.
+__global__ void ccs_compact(const uint32_t *off, uint32_t n, const uint8_t *cur, uint32_t *short_list,
+                            uint64_t *items, uint32_t *ctr, unsigned long long *stat) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of the wavefront
+    uint32_t nact = 0;
+    // whole wavefronts step together (ballots and shuffles below)
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; base < n; base += stride) {
+        const uint64_t v64 = base + lane;
+        const uint32_t v = (uint32_t)v64;
+        const bool act = v64 < n && cur[v];
+        const uint32_t deg = act ? off[v + 1] - off[v] : 0u;
+        const bool sh = act && deg <= kCcsShort;
+        const uint32_t nit = (act && deg > kCcsShort) ? (deg + kCcsChunk - 1) / kCcsChunk : 0u;
+        nact += act;
+        const uint64_t bs = __ballot(sh);
+        uint32_t sbase = 0;
+        if (lane == 0 && bs) sbase = atomicAdd(&ctr[0], (uint32_t)__popcll(bs));
+        sbase = __shfl(sbase, 0, kWave);
+        if (sh) short_list[sbase + __popcll(bs & lanemask_lt())] = v;
+        uint32_t incl = nit;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        const uint32_t tot = __shfl(incl, kWave - 1, kWave);
+        uint32_t ibase = 0;
+        if (lane == 0 && tot) ibase = atomicAdd(&ctr[1], tot);
+        ibase = __shfl(ibase, 0, kWave);
+        for (uint32_t c = 0; c < nit; c++) items[ibase + incl - nit + c] = ((uint64_t)v << 32) | c;
+    }
+    nact = wave_sum(nact);
+    if (lane == 0 && nact) atomicAdd(&stat[1], (unsigned long long)nact);
+}
+
+// stat[2] += rows whose offsets decrease or pass off[n].
+__global__ void ccs_check_off(const uint32_t *off, uint32_t n, unsigned long long *stat) {
+    const uint32_t E = off[n];
+    uint32_t bad = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        bad += off[v] > off[v + 1] || off[v + 1] > E;
+    bad = wave_sum(bad);
+    if ((threadIdx.x & (kWave - 1)) == 0 && bad) atomicAdd(&stat[2], (unsigned long long)bad);
+}
+
+// One lane per short row: label = min over the row, neighbours scheduled if it dropped.
+__global__ void ccs_short(const uint32_t *off, const uint32_t *adj, uint32_t n, uint32_t *label,
+                          const uint32_t *short_list, const uint32_t *ctr, uint8_t *next, unsigned long long *stat) {
+    const uint32_t cnt = ctr[0];
+    uint32_t dropped = 0, bad = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+        const uint32_t v = short_list[i], b = off[v], e = off[v + 1];
+        const uint32_t lv = label[v];
+        uint32_t m = lv;
+        for (uint32_t k = b; k < e; k++) {
+            const uint32_t u = adj[k];
+            if (u < n) m = min(m, label[u]);
+            else bad = 1;
+        }
+        if (m < lv) {
+            label[v] = m;  // the only writer of a short row's label
+            dropped++;
+            for (uint32_t k = b; k < e; k++) {
+                const uint32_t u = adj[k];
+                if (u < n) next[u] = 1;
+            }
+        }
+    }
+    dropped = wave_sum(dropped);
+    bad = wave_sum(bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (dropped) atomicAdd(&stat[0], (unsigned long long)dropped);
+        if (bad) atomicAdd(&stat[2], (unsigned long long)bad);
+    }
+}
+
+// One wavefront per chunk of a long row: min over the chunk, atomicMin into the row.
+__global__ void ccs_long_min(const uint32_t *off, const uint32_t *adj, uint32_t n, uint32_t *label,
+                             const uint64_t *items, const uint32_t *ctr, uint8_t *dropped_flag,
+                             unsigned long long *stat) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t cnt = ctr[1];
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nw = (gridDim.x * blockDim.x) / kWave;
+    uint32_t bad = 0;
+    for (uint32_t j = wid; j < cnt; j += nw) {
+        const uint64_t it = items[j];
+        const uint32_t v = (uint32_t)(it >> 32), c = (uint32_t)it;
+        const uint32_t b = off[v] + c * kCcsChunk, e = min(off[v + 1], b + kCcsChunk);
+        const uint32_t lv = label[v];
+        uint32_t m = 0xffffffffu;
+        for (uint32_t k = b + lane; k < e; k += kWave) {
+            const uint32_t u = adj[k];
+            if (u < n) m = min(m, label[u]);
+            else bad = 1;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, kWave));
+        if (lane == 0 && m < lv) {
+            const uint32_t old = atomicMin(&label[v], m);
+            if (m < old) dropped_flag[v] = 1;
+        }
+    }
+    bad = wave_sum(bad);
+    if (lane == 0 && bad) atomicAdd(&stat[2], (unsigned long long)bad);
+}
+
+// Schedules the neighbours of every long row whose label dropped (per chunk).
+__global__ void ccs_long_mark(const uint32_t *off, const uint32_t *adj, uint32_t n, const uint64_t *items,
+                              const uint32_t *ctr, const uint8_t *dropped_flag, uint8_t *next,
+                              unsigned long long *stat) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t cnt = ctr[1];
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nw = (gridDim.x * blockDim.x) / kWave;
+    uint32_t dropped = 0;
+    for (uint32_t j = wid; j < cnt; j += nw) {
+        const uint64_t it = items[j];
+        const uint32_t v = (uint32_t)(it >> 32), c = (uint32_t)it;
+        if (!dropped_flag[v]) continue;
+        if (c == 0) dropped++;
+        const uint32_t b = off[v] + c * kCcsChunk, e = min(off[v + 1], b + kCcsChunk);
+        for (uint32_t k = b + lane; k < e; k += kWave) {
+            const uint32_t u = adj[k];
+            if (u < n) next[u] = 1;
+        }
+    }
+    if (lane == 0 && dropped) atomicAdd(&stat[0], (unsigned long long)dropped);
+}
+
+struct CcsWork {
+    uint8_t *flags[2];
+    uint8_t *dropped;
+    uint32_t *short_list;
+    uint64_t *items;
+    uint32_t *ctr;
+    unsigned long long *stat;
+};
+
+template <typename C>
+static void ccs_layout(C &c, uint32_t n, uint64_t e, CcsWork *w) {
+    const size_t nb = align_up((size_t)n + 1, 4);
+    auto *f0 = c.template take<uint8_t>(nb);
+    auto *f1 = c.template take<uint8_t>(nb);
+    auto *dr = c.template take<uint8_t>(nb);
+    auto *sl = c.template take<uint32_t>((size_t)n + 1);
+    auto *it = c.template take<uint64_t>((size_t)n + e / kCcsChunk + 1);
+    auto *ct = c.template take<uint32_t>(2);
+    auto *st = c.template take<unsigned long long>(3);
+    if (w) *w = CcsWork{{f0, f1}, dr, sl, it, ct, st};
+}
+
 }  // namespace gchi
 
 using namespace gchi;
@@ -168,5 +337,80 @@ extern "C" int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_
     hipLaunchKernelGGL(cc_count_changed, dim3(grid_for(nvertices, 256, 4096)), dim3(256), 0, s, d_labels_in,
                        d_labels_out, nvertices, d_changed);
     GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
+extern "C" int gchi_cc_scheduled_workspace(uint32_t nvertices, uint64_t nedges, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    SizerAdapter a;
+    ccs_layout(a, nvertices, nedges, nullptr);
+    *bytes = a.z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
+                                 uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, void *d_work,
+                                 size_t work_bytes, void *stream) {
+    GCHI_CHECK(d_off && d_labels && h_rounds && nvertices > 0, GCHI_ERR_INVALID, "cc_scheduled: null argument");
+    hipStream_t s = as_stream(stream);
+    uint32_t e32 = 0;
+    GCHI_HIP(hipMemcpyAsync(&e32, d_off + nvertices, 4, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    const uint64_t E = e32;
+    GCHI_CHECK(E == 0 || d_adj, GCHI_ERR_INVALID, "cc_scheduled: null adjacency");
+    size_t need = 0;
+    gchi_cc_scheduled_workspace(nvertices, E, &need);
+    GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "cc_scheduled: workspace too small");
+    Carver c(d_work, work_bytes);
+    CcsWork w;
+    ccs_layout(c, nvertices, E, &w);
+    const uint32_t n = nvertices;
+    const size_t nwords = align_up((size_t)n + 1, 4) / 4;
+    const unsigned gv = grid_for(n, 256, 65536);
+    const unsigned gw = 4096;  // wavefronts loop over the chunk list
+    launch_fill_u64(reinterpret_cast<uint64_t *>(w.stat), 0ull, 3, s);
+    hipLaunchKernelGGL(ccs_check_off, dim3(gv), dim3(256), 0, s, d_off, n, w.stat);
+    unsigned long long h_bad = 0;
+    GCHI_HIP(hipMemcpyAsync(&h_bad, w.stat + 2, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(h_bad == 0, GCHI_ERR_INVALID, "cc_scheduled: offsets are not a CSR");
+    hipLaunchKernelGGL(cc_iota, dim3(gv), dim3(256), 0, s, d_labels, n);
+    launch_fill_u32(reinterpret_cast<uint32_t *>(w.flags[0]), 0x01010101u, nwords, s);  // round 0: every vertex
+    // labels only drop and stay ids of the component: at most n rounds
+    const uint64_t limit = max_rounds ? max_rounds : (uint64_t)n + 1;
+    uint32_t rounds = 0;
+    uint64_t evaluated = 0;
+    bool fixed = false;
+    int cur = 0;
+    while (rounds < limit) {
+        uint8_t *fc = w.flags[cur], *fn = w.flags[cur ^ 1];
+        launch_fill_u32(reinterpret_cast<uint32_t *>(fn), 0u, nwords, s);
+        launch_fill_u32(reinterpret_cast<uint32_t *>(w.dropped), 0u, nwords, s);
+        launch_fill_u32(w.ctr, 0u, 2, s);
+        launch_fill_u64(reinterpret_cast<uint64_t *>(w.stat), 0ull, 3, s);
+        hipLaunchKernelGGL(ccs_compact, dim3(gv), dim3(256), 0, s, d_off, n, fc, w.short_list, w.items, w.ctr,
+                           w.stat);
+        hipLaunchKernelGGL(ccs_short, dim3(gv), dim3(256), 0, s, d_off, d_adj, n, d_labels, w.short_list, w.ctr, fn,
+                           w.stat);
+        hipLaunchKernelGGL(ccs_long_min, dim3(gw), dim3(256), 0, s, d_off, d_adj, n, d_labels, w.items, w.ctr,
+                           w.dropped, w.stat);
+        hipLaunchKernelGGL(ccs_long_mark, dim3(gw), dim3(256), 0, s, d_off, d_adj, n, w.items, w.ctr, w.dropped, fn,
+                           w.stat);
+        GCHI_LAUNCH_CHECK();
+        unsigned long long st[3] = {0, 0, 0};
+        GCHI_HIP(hipMemcpyAsync(st, w.stat, sizeof(st), hipMemcpyDeviceToHost, s));
+        GCHI_HIP(hipStreamSynchronize(s));
+        GCHI_CHECK(st[2] == 0, GCHI_ERR_INVALID, "cc_scheduled: neighbour id >= nvertices");
+        rounds++;
+        evaluated += st[1];
+        if (st[0] == 0) {
+            fixed = true;
+            break;
+        }
+        cur ^= 1;
+    }
+    *h_rounds = rounds;
+    if (h_evaluated) *h_evaluated = evaluated;
+    GCHI_CHECK(fixed, GCHI_ERR_INVALID, "cc_scheduled: no fixed point within max_rounds");
     return GCHI_OK;
 }

@@ -103,6 +103,9 @@ _SIGS = {
     "gchi_cc_minlabel": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_sz, c_vp]),
     "gchi_cc_labelprop_init": (c_i32, [c_vp, c_u32, c_vp]),
     "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
+    "gchi_cc_scheduled_workspace": (c_i32, [c_u32, c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_cc_scheduled": (c_i32, [c_vp, c_vp, c_u32, c_u32, c_vp, ctypes.POINTER(c_u32), ctypes.POINTER(c_u64), c_vp,
+                                  c_sz, c_vp]),
     "gchi_tc_prepare_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
     "gchi_tc_prepare": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, ctypes.POINTER(c_u64), c_vp, c_sz,
                                 c_vp]),
@@ -431,6 +434,21 @@ def cc_labelprop(src, dst, n, max_sweeps=100000):
         if int(changed.item()) == 0:
             return lab[(k + 1) & 1], k + 1
     raise GchiError("cc_labelprop: no fixed point within max_sweeps")
+
+
+def cc_scheduled(off, adj, n, max_rounds=0):
+    """Selective-scheduling min-label components over a symmetric CSR
+    (off[n+1], adj[E]); returns (labels, rounds, vertex evaluations)."""
+    L = lib()
+    labels = empty_u32(n)
+    e = int(adj.numel()) if adj is not None else 0
+    nb = c_sz()
+    check(L.gchi_cc_scheduled_workspace(n, e, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    rounds, evals = c_u32(), c_u64()
+    check(L.gchi_cc_scheduled(dptr(off), dptr(adj), n, max_rounds, dptr(labels), ctypes.byref(rounds),
+                              ctypes.byref(evals), dptr(ws), nb.value, stream()), "cc_scheduled")
+    return labels, rounds.value, evals.value
 
 
 # ---------------------------------------------------------------- triangle counting

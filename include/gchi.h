@@ -290,10 +290,6 @@ int gchi_ppr_batch_extract(const gchi_ppr_batch *batch, uint32_t query, float *d
 int gchi_cc_workspace(uint32_t nvertices, size_t *bytes);
 int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                      uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream);
-/* The reference's per-iteration form: labels = ids (iteration 0), then one
- * synchronous min-propagation sweep over every edge in both directions;
- * *d_changed = number of labels that changed (0 = converged,
- * connectedcomponents.cpp:125-138). */
 /* As gchi_cc_minlabel.  flags: GCHI_CC_SYMMETRIC promises a symmetric list (every
  * u-v present as both (u,v) and (v,u), the form the reference's undirected
  * inputs take in its edge files), so only the entries with u < v are unioned;
@@ -301,9 +297,30 @@ int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedg
 #define GCHI_CC_SYMMETRIC 1
 int gchi_cc_minlabel_ex(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices, int flags,
                         uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream);
+/* The reference's per-iteration form: labels = ids (iteration 0), then one
+ * synchronous min-propagation sweep over every edge in both directions;
+ * *d_changed = number of labels that changed (0 = converged,
+ * connectedcomponents.cpp:125-138). */
 int gchi_cc_labelprop_init(uint32_t *d_labels, uint32_t nvertices, void *stream);
 int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                             const uint32_t *d_labels_in, uint32_t *d_labels_out, uint64_t *d_changed, void *stream);
+
+/* The reference's CC under its selective scheduler (ConnectedComponentsProgram
+ * with scheduler on, connectedcomponents.cpp:79-121 and :81,85,112; the two
+ * bitsets of src/engine/bitset_scheduler.hpp:38-104): round 0 evaluates every
+ * vertex (label = min of its id and its neighbours' labels); a vertex whose label
+ * drops schedules all its neighbours for the next round; the rounds end when no
+ * label drops.  Input: a symmetric adjacency in CSR form (d_off[N+1] with
+ * d_off[N] = E, d_adj[E]; every edge present in both rows, as gchi_csc_build
+ * gives for a list holding each edge both ways).  d_labels[N] ends equal to
+ * gchi_cc_minlabel's.  max_rounds 0 = no limit; a limit reached before the fixed
+ * point is an error.  *h_rounds = rounds run, *h_evaluated (optional) = vertex
+ * evaluations summed over the rounds.  Host-synchronous (one 24-byte read per
+ * round). */
+int gchi_cc_scheduled_workspace(uint32_t nvertices, uint64_t nedges, size_t *bytes);
+int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
+                      uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, void *d_work, size_t work_bytes,
+                      void *stream);
 
 /* ------------------------------------------------------------ triangle counting */
 /* Degree ordering (order_by_degree, src/preprocessing/util/orderbydegree.hpp:58-100:

@@ -55,3 +55,64 @@ def test_cc_symmetric_flag_on_symmetrised_golden(dev, golden, case):
     d2 = np.concatenate([dst, src]).astype(np.uint32)
     lab = dev.to_host_u32(dev.cc_minlabel(dev.to_dev_u32(s2), dev.to_dev_u32(d2), n, symmetric=True), n)
     np.testing.assert_array_equal(lab, golden[f"{case['name']}__labels"])
+
+
+def _sym_csr(dev, src, dst, n):
+    """CSR of the symmetrised list (each edge in both rows) on the device."""
+    s2 = np.concatenate([src, dst]).astype(np.uint32)
+    d2 = np.concatenate([dst, src]).astype(np.uint32)
+    g = dev.DeviceGraph(dev.to_dev_u32(s2), dev.to_dev_u32(d2), n)
+    return g
+
+
+@pytest.mark.parametrize("case", graphs.CC_CASES, ids=[c["name"] for c in graphs.CC_CASES])
+def test_cc_scheduled_vs_reference_golden(dev, golden, case):
+    """Selective scheduling reaches the reference's labels (its own output)."""
+    src, dst, n = graphs.make(case)
+    g = _sym_csr(dev, src, dst, n)
+    lab, rounds, evals = dev.cc_scheduled(g.in_off, g.in_src, n)
+    np.testing.assert_array_equal(dev.to_host_u32(lab, n), golden[f"{case['name']}__labels"])
+    assert rounds >= 1 and evals >= n  # round 0 evaluates every vertex
+
+
+def test_cc_scheduled_long_rows_and_many_rounds(dev):
+    """Rows longer than one 4096-edge chunk (a 20000-leaf star whose centre has
+    the largest id), a 3000-vertex path with ids falling along it (the minimum
+    starts at the far end), and isolated vertices."""
+    star_c, leaves = 30000, np.arange(1, 20001, dtype=np.uint32)
+    path = np.arange(23000, 20000, -1, dtype=np.uint32)  # 23000 - 22999 - ... - 20001
+    src = np.concatenate([np.full(len(leaves), star_c, np.uint32), path[:-1]])
+    dst = np.concatenate([leaves, path[1:]])
+    n = 30005  # 0, 23001..29999 and 30001..30004 have no edges
+    g = _sym_csr(dev, src, dst, n)
+    lab, rounds, _ = dev.cc_scheduled(g.in_off, g.in_src, n)
+    lab = dev.to_host_u32(lab, n)
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(n, src, dst))
+    assert lab[star_c] == 1 and lab[23000] == 20001 and lab[0] == 0 and lab[30004] == 30004
+    assert rounds >= 2
+
+
+def test_cc_scheduled_scale24_vs_unionfind(dev):
+    """The CC workload through the scheduled form: labels bit-exact vs union-find."""
+    src, dst = graphs.rmat_pairs(24, 16, 3, undirected=True)
+    src, dst = graphs.preprocess(src, dst, undirected=True)
+    n = int(max(src.max(), dst.max())) + 1
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)  # already both ways
+    lab, rounds, evals = dev.cc_scheduled(g.in_off, g.in_src, n)
+    np.testing.assert_array_equal(dev.to_host_u32(lab, n), oracle.cc_unionfind(n, src, dst))
+    print(f"\nscale24 scheduled CC: rounds={rounds} evaluations={evals} (N={n})")
+
+
+def test_cc_scheduled_rejects_bad_input(dev):
+    off = dev.to_dev_u32(np.array([0, 1, 2], np.uint32))
+    adj = dev.to_dev_u32(np.array([1, 5], np.uint32))  # 5 >= n
+    with pytest.raises(dev.GchiError):
+        dev.cc_scheduled(off, adj, 2)
+    bad_off = dev.to_dev_u32(np.array([0, 2, 1], np.uint32))  # decreasing
+    with pytest.raises(dev.GchiError):
+        dev.cc_scheduled(bad_off, dev.to_dev_u32(np.array([1, 0], np.uint32)), 2)
+    # a path needs more than one round: a limit of 1 is an error, not a wrong answer
+    path = np.arange(100, 0, -1, dtype=np.uint32)
+    g = _sym_csr(dev, path[:-1], path[1:], 101)
+    with pytest.raises(dev.GchiError):
+        dev.cc_scheduled(g.in_off, g.in_src, 101, max_rounds=1)
