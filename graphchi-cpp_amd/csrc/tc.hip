@@ -11,13 +11,13 @@
 //       and oriented from lower to higher rank, sorted and deduplicated into an
 //       oriented CSR (N+(v): the higher-ranked neighbours of v, ascending).
 //   gchi_tc_count   : every triangle a<b<w is found once, at vertex a, as
-//       w in N+(a) n N+(b) for b in N+(a).  One wavefront per vertex a (taken
-//       from an atomic work counter): N+(a) is staged in LDS (binary search in
-//       global memory when longer than 1024), the (b, w in N+(b)) pairs of 64
-//       consecutive b are flattened over the 64 lanes, and each lane
-//       binary-searches its w in N+(a).  Counts: a, b and w each get +1 per
-//       triangle, gathered per wavefront (a: one sum per vertex, b: one per
-//       (a,b) pair, w: an LDS counter per position of N+(a)) and flushed with
+//       w in N+(a) n N+(b) for b in N+(a).  One wavefront per vertex a.
+//       |N+(a)| <= 1024 (the hash form, tc_count_hash_kernel): N+(a) goes
+//       into an LDS hash table and every w of N+(b) is looked up lane-wide.
+//       Longer N+(a) (tc_count_kernel): the (b, w in N+(b)) pairs of 64
+//       consecutive b are flattened over the 64 lanes and each lane
+//       binary-searches its w in N+(a) in global memory.  Counts: a, b and w
+//       each get +1 per triangle, gathered per wavefront and flushed with
 //       uint64 atomics, converted to uint32 with an overflow flag.
 #include "common.hpp"
 #include "internal.hpp"
@@ -125,21 +125,24 @@ GCHI_DEV int bsearch_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t x
 __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_t *__restrict__ off,
                                                                    const uint32_t *__restrict__ adj, uint32_t n,
                                                                    uint32_t min_da, unsigned long long *cnt,
-                                                                   unsigned long long *total, uint32_t *work) {
+                                                                   unsigned long long *total) {
     __shared__ uint32_t sA[kTcWaves][kTcLds];
     __shared__ uint32_t sW[kTcWaves][kTcLds];  // triangles found per position of N+(a) (the w credit)
     __shared__ uint32_t sPre[kTcWaves][kWave];
     __shared__ uint32_t sBeg[kTcWaves][kWave];
     __shared__ uint32_t sCnt[kTcWaves][kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(work, 1u);
-        t = __shfl(t, 0, kWave);
-        if (t >= n) break;
-        const uint32_t a = n - 1 - t;  // highest rank (most work) first
+    // A static scan, 64 vertices per step, picks the vertices above min_da (no
+    // work counter: the few long rows would not pay for a ticket per vertex).
+    const uint64_t step = (uint64_t)gridDim.x * kTcWaves * kWave;
+    for (uint64_t base = ((uint64_t)blockIdx.x * kTcWaves + wave) * kWave; base < n; base += step) {
+        const uint64_t v = base + lane;
+        const uint32_t dv = v < n ? off[v + 1] - off[v] : 0u;
+        uint64_t todo = __ballot(dv >= 2 && dv > min_da);
+        while (todo) {
+        const uint32_t a = (uint32_t)base + (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1;
         const uint32_t ab = off[a], da = off[a + 1] - ab;
-        if (da < 2 || da <= min_da) continue;
         const uint32_t *A = adj + ab;
         const bool lds = da <= (uint32_t)kTcLds;
         if (lds)
@@ -213,6 +216,7 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        }
     }
 }
 
@@ -230,6 +234,12 @@ constexpr int kTcHashWaves = 4;
 constexpr uint32_t kTcHashSlots = 2048;
 constexpr uint32_t kTcHashMax = kTcHashSlots / 2;
 constexpr uint32_t kTcEmpty = 0xffffffffu;
+// Vertex tickets come from kTcStripes counters, each in its own 128-byte line:
+// stripe k hands out the ranks n-1-k, n-1-k-kTcStripes, ...  Atomics on one
+// address complete one after another (about 10 ns each here), so a single
+// counter would cap the kernel at ~10^8 vertices per second.
+constexpr uint32_t kTcStripes = 32;
+constexpr uint32_t kTcStripeGap = 32;
 
 GCHI_DEV uint32_t tc_slot(uint32_t w, int sh) { return (w * 2654435761u) >> sh; }
 
@@ -256,12 +266,18 @@ __global__ __launch_bounds__(kTcHashWaves *kWave) void tc_count_hash_kernel(cons
     uint32_t *keys = sKey[wave];
     uint32_t *cw = sCw[wave];
     unsigned long long acc_total = 0;
+    uint32_t stripe = (blockIdx.x * kTcHashWaves + wave) % kTcStripes, spent = 0;
     for (;;) {
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(work, 1u);
+        if (lane == 0) t = atomicAdd(&work[stripe * kTcStripeGap], 1u);
         t = __shfl(t, 0, kWave);
-        if (t >= n) break;
-        const uint32_t a = n - 1 - t;  // highest rank (most work) first
+        const uint64_t idx = stripe + (uint64_t)kTcStripes * t;
+        if (idx >= n) {  // this stripe is drained: move on, stop after all of them
+            if (++spent == kTcStripes) break;
+            stripe = (stripe + 1) % kTcStripes;
+            continue;
+        }
+        const uint32_t a = n - 1 - (uint32_t)idx;  // highest rank (most work) first
         const uint32_t ab = off[a], da = off[a + 1] - ab;
         if (da < 2 || da > kTcHashMax) continue;
         const uint32_t want = 4 * da;  // slots: pow2 >= 4 da, at most kTcHashSlots
@@ -459,7 +475,7 @@ extern "C" int gchi_tc_count_workspace(uint32_t nvertices, size_t *bytes) {
     Sizer z;
     z.take<unsigned long long>(nvertices + 1);
     z.take<unsigned long long>(1);
-    z.take<uint32_t>(2);
+    z.take<uint32_t>(kTcStripes * kTcStripeGap);
     *bytes = z.off;
     return GCHI_OK;
 }
@@ -476,10 +492,10 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
     Carver c(d_work, work_bytes);
     auto *cnt = c.take<unsigned long long>(nvertices + 1);
     auto *tot = c.take<unsigned long long>(1);
-    auto *wk = c.take<uint32_t>(2);
+    auto *wk = c.take<uint32_t>(kTcStripes * kTcStripeGap);
     launch_fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, nvertices, s);
     launch_fill_u64(reinterpret_cast<uint64_t *>(tot), 0ull, 1, s);
-    launch_fill_u32(wk, 0u, 2, s);
+    launch_fill_u32(wk, 0u, kTcStripes * kTcStripeGap, s);
     launch_fill_u32(d_overflow, 0u, 1, s);
     int ncu = 256;
     int dev = 0;
@@ -490,7 +506,7 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
     hipLaunchKernelGGL(tc_count_hash_kernel, dim3(ncu * 6), dim3(kTcHashWaves * kWave), 0, s, d_off, d_adj,
                        nvertices, cnt, tot, wk);
     hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices,
-                       kTcHashMax, cnt, tot, wk + 1);
+                       kTcHashMax, cnt, tot);
     hipLaunchKernelGGL(tc_to_u32, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, cnt, nvertices,
                        d_per_vertex, d_overflow);
     GCHI_HIP(hipMemcpyAsync(d_total, tot, 8, hipMemcpyDeviceToDevice, s));
