@@ -123,40 +123,55 @@ constexpr uint32_t kCcsShort = 64;    // rows up to this length: one lane each
 constexpr uint32_t kCcsChunk = 4096;  // longer rows: one wavefront per chunk
 
 // ctr: [0] short rows, [1] long-row chunks.  stat: [0] labels dropped,
-// [1] rows evaluated, [2] neighbour ids >= n seen.
+// [1] rows evaluated, [2] neighbour ids >= n seen.  Each wavefront compacts a
+// contiguous vertex range in two passes (count, reserve once, write), so the
+// shared counters see one atomic per wavefront and round, not one per 64
+// vertices: atomics on one address complete one after another.
 __global__ void ccs_compact(const uint32_t *off, uint32_t n, const uint8_t *cur, uint32_t *short_list,
                             uint64_t *items, uint32_t *ctr, unsigned long long *stat) {
     const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of the wavefront
-    uint32_t nact = 0;
-    // whole wavefronts step together (ballots and shuffles below)
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; base < n; base += stride) {
-        const uint64_t v64 = base + lane;
-        const uint32_t v = (uint32_t)v64;
-        const bool act = v64 < n && cur[v];
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    const uint64_t per = ((n + nw - 1) / nw + kWave - 1) / kWave * kWave;
+    const uint64_t lo = wid * per, hi = lo + per < n ? lo + per : n;
+    uint32_t ns = 0, ni = 0, na = 0;
+    for (uint64_t v = lo + lane; v < hi; v += kWave) {
+        if (!cur[v]) continue;
+        const uint32_t deg = off[v + 1] - off[v];
+        na++;
+        if (deg <= kCcsShort) ns++;
+        else ni += (deg + kCcsChunk - 1) / kCcsChunk;
+    }
+    ns = wave_sum(ns);
+    ni = wave_sum(ni);
+    na = wave_sum(na);
+    uint32_t sbase = 0, ibase = 0;
+    if (lane == 0) {
+        if (ns) sbase = atomicAdd(&ctr[0], ns);
+        if (ni) ibase = atomicAdd(&ctr[1], ni);
+        if (na) atomicAdd(&stat[1], (unsigned long long)na);
+    }
+    sbase = __shfl(sbase, 0, kWave);
+    ibase = __shfl(ibase, 0, kWave);
+    if (ns == 0 && ni == 0) return;
+    for (uint64_t base = lo; base < hi; base += kWave) {  // whole wavefronts: ballots and shuffles
+        const uint64_t v = base + lane;
+        const bool act = v < hi && cur[v];
         const uint32_t deg = act ? off[v + 1] - off[v] : 0u;
         const bool sh = act && deg <= kCcsShort;
         const uint32_t nit = (act && deg > kCcsShort) ? (deg + kCcsChunk - 1) / kCcsChunk : 0u;
-        nact += act;
         const uint64_t bs = __ballot(sh);
-        uint32_t sbase = 0;
-        if (lane == 0 && bs) sbase = atomicAdd(&ctr[0], (uint32_t)__popcll(bs));
-        sbase = __shfl(sbase, 0, kWave);
-        if (sh) short_list[sbase + __popcll(bs & lanemask_lt())] = v;
+        if (sh) short_list[sbase + __popcll(bs & lanemask_lt())] = (uint32_t)v;
+        sbase += (uint32_t)__popcll(bs);
         uint32_t incl = nit;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const uint32_t t = __shfl_up(incl, o, kWave);
             if (lane >= o) incl += t;
         }
-        const uint32_t tot = __shfl(incl, kWave - 1, kWave);
-        uint32_t ibase = 0;
-        if (lane == 0 && tot) ibase = atomicAdd(&ctr[1], tot);
-        ibase = __shfl(ibase, 0, kWave);
-        for (uint32_t c = 0; c < nit; c++) items[ibase + incl - nit + c] = ((uint64_t)v << 32) | c;
+        for (uint32_t c = 0; c < nit; c++) items[ibase + incl - nit + c] = (v << 32) | c;
+        ibase += __shfl(incl, kWave - 1, kWave);
     }
-    nact = wave_sum(nact);
-    if (lane == 0 && nact) atomicAdd(&stat[1], (unsigned long long)nact);
 }
 
 // stat[2] += rows whose offsets decrease or pass off[n].
@@ -180,6 +195,7 @@ __global__ void ccs_short(const uint32_t *off, const uint32_t *adj, uint32_t n, 
         const uint32_t v = short_list[i], b = off[v], e = off[v + 1];
         const uint32_t lv = label[v];
         uint32_t m = lv;
+#pragma unroll 4
         for (uint32_t k = b; k < e; k++) {
             const uint32_t u = adj[k];
             if (u < n) m = min(m, label[u]);
@@ -367,6 +383,7 @@ extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, u
     const uint32_t n = nvertices;
     const size_t nwords = align_up((size_t)n + 1, 4) / 4;
     const unsigned gv = grid_for(n, 256, 65536);
+    const unsigned gl = grid_for(n, 256, 2048);  // list kernels loop over their lists
     const unsigned gw = 4096;  // wavefronts loop over the chunk list
     launch_fill_u64(reinterpret_cast<uint64_t *>(w.stat), 0ull, 3, s);
     hipLaunchKernelGGL(ccs_check_off, dim3(gv), dim3(256), 0, s, d_off, n, w.stat);
@@ -388,9 +405,9 @@ extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, u
         launch_fill_u32(reinterpret_cast<uint32_t *>(w.dropped), 0u, nwords, s);
         launch_fill_u32(w.ctr, 0u, 2, s);
         launch_fill_u64(reinterpret_cast<uint64_t *>(w.stat), 0ull, 3, s);
-        hipLaunchKernelGGL(ccs_compact, dim3(gv), dim3(256), 0, s, d_off, n, fc, w.short_list, w.items, w.ctr,
-                           w.stat);
-        hipLaunchKernelGGL(ccs_short, dim3(gv), dim3(256), 0, s, d_off, d_adj, n, d_labels, w.short_list, w.ctr, fn,
+        hipLaunchKernelGGL(ccs_compact, dim3(grid_for(n, 256 * 16, 1024)), dim3(256), 0, s, d_off, n, fc,
+                           w.short_list, w.items, w.ctr, w.stat);
+        hipLaunchKernelGGL(ccs_short, dim3(gl), dim3(256), 0, s, d_off, d_adj, n, d_labels, w.short_list, w.ctr, fn,
                            w.stat);
         hipLaunchKernelGGL(ccs_long_min, dim3(gw), dim3(256), 0, s, d_off, d_adj, n, d_labels, w.items, w.ctr,
                            w.dropped, w.stat);

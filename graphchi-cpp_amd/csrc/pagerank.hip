@@ -79,7 +79,7 @@ struct gchi_pr_plan {
     double *blk_stat = nullptr;
     double *hub_stat = nullptr;
     double *fin_stat = nullptr;
-    uint32_t *counters = nullptr;  // [0] finalize ticket, [1] tile counter
+    uint32_t *counters = nullptr;  // [0] finalize ticket, [kPrStripeGap...] tile counter stripes
     int hot = 0;           // tune value (0: vertex ids, -1: gather layout, > 0: layout + LDS cache)
     uint32_t A = 0;        // sources in the gather layout (out-degree > 0)
     uint32_t H = 0;        // layout values cached in LDS by the wavefront-tiled kernel
@@ -111,6 +111,14 @@ struct PrConsts {
     const uint32_t *seed_bits; // seed bitmap, or null
     double seed_tele;
 };
+
+constexpr uint32_t kPrStripes = 16;    // tile counters of the persistent sweep
+constexpr uint32_t kPrStripeGap = 32;  // uint32 words between counters (128 bytes)
+constexpr uint32_t kPrCounterWords = kPrStripeGap * (kPrStripes + 1);
+
+GCHI_DEV void pr_reset_stripes(uint32_t *counters) {
+    for (uint32_t i = threadIdx.x; i < kPrStripes; i += blockDim.x) counters[kPrStripeGap * (i + 1)] = 0u;
+}
 
 struct PrArgs {
     const uint32_t *reg_off, *reg_src, *blk_row, *blk_graph, *tile_e0, *tile_e1, *hub_bits;
@@ -293,21 +301,33 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_kernel(PrArgs a, PrConsts
     pr_tile<kPrThreads>(a, c, blockIdx.x, s_val, s_long, &s_nlong, s_red);
 }
 
-// Persistent form: workgroups pull tiles from the work counter until the queue
-// is drained (hub chunks first, then regular blocks).  Every workgroup exits
-// once the counter passes ntiles; the finalize kernel resets the counter.
+// Persistent form: workgroups pull tiles from the work counters until the queue
+// is drained (hub chunks first, then regular blocks).  Atomics on one address
+// complete one after another (about 10 ns each here), so one counter would cap
+// a sweep at ~10^8 tiles per second; kPrStripes counters, each in its own
+// 128-byte line, hand out the tiles s, s + kPrStripes, ... of stripe s.  A
+// workgroup starts on stripe blockIdx % kPrStripes and moves to the next one
+// when it is drained, so the tiles in flight stay a narrow window of the tile
+// order (the sources the gathers touch stay in cache).  Every workgroup exits
+// once all stripes are drained; the finalize kernels reset the counters.
 __global__ __launch_bounds__(kPrThreads) void pr_sweep_persistent_kernel(PrArgs a, PrConsts c) {
     extern __shared__ float s_val[];
     __shared__ uint32_t s_long[kMaxLong];
     __shared__ uint32_t s_nlong;
     __shared__ double s_red[2 * kPrWaves];
     __shared__ uint32_t s_tile;
+    uint32_t stripe = blockIdx.x % kPrStripes, spent = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+        if (threadIdx.x == 0) s_tile = atomicAdd(&a.tile_counter[stripe * kPrStripeGap], 1u);
         __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= a.ntiles) break;
-        pr_tile<kPrThreads>(a, c, tile, s_val, s_long, &s_nlong, s_red);
+        const uint64_t tile = stripe + (uint64_t)kPrStripes * s_tile;
+        __syncthreads();  // s_tile is rewritten by the next ticket
+        if (tile >= a.ntiles) {  // block-uniform
+            if (++spent == kPrStripes) break;
+            stripe = (stripe + 1) % kPrStripes;
+            continue;
+        }
+        pr_tile<kPrThreads>(a, c, (uint32_t)tile, s_val, s_long, &s_nlong, s_red);
         __syncthreads();
     }
 }
@@ -772,8 +792,8 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
             d_stats[0] = ta;
             d_stats[1] = ts;
             counters[0] = 0u;
-            counters[1] = 0u;
         }
+        pr_reset_stripes(counters);
     }
 }
 
@@ -837,7 +857,7 @@ __global__ __launch_bounds__(kFinThreads) void pr_graph_reduce_kernel(const uint
         }
         __syncthreads();
     }
-    if (blockIdx.x == 0 && tid == 0) counters[1] = 0u;
+    if (blockIdx.x == 0) pr_reset_stripes(counters);
 }
 
 __global__ void pr_init_kernel(const uint32_t *outdeg, uint32_t n, float init, const uint32_t *seed_bits,
@@ -1274,7 +1294,7 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
     zz.take<double>(2 * (size_t)p->nblk);
     zz.take<double>(2 * (size_t)nhub + 2);
     zz.take<double>(2 * (size_t)p->nfin);
-    zz.take<uint32_t>(2);
+    zz.take<uint32_t>(kPrCounterWords);
     zz.take<uint32_t>(std::max<uint32_t>(A, 1));
     zz.take<float>(std::max<uint32_t>(A, 1));
     zz.take<uint32_t>(p->nblk + 1);
@@ -1300,7 +1320,7 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
     p->blk_stat = c.take<double>(2 * (size_t)p->nblk);
     p->hub_stat = c.take<double>(2 * (size_t)nhub + 2);
     p->fin_stat = c.take<double>(2 * (size_t)p->nfin);
-    p->counters = c.take<uint32_t>(2);
+    p->counters = c.take<uint32_t>(kPrCounterWords);
     p->gorder = c.take<uint32_t>(std::max<uint32_t>(A, 1));
     p->xg = c.take<float>(std::max<uint32_t>(A, 1));
     p->tile_e0 = c.take<uint32_t>(p->nblk + 1);
@@ -1314,7 +1334,7 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
     GCHI_HIP(hipMemcpyAsync(p->hub_chunk_off, hub_choff, (size_t)(nhub + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->gblk, gblk, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, s));
     GCHI_HIP(hipMemcpyAsync(p->vbase, vbase, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, s));
-    launch_fill_u32(p->counters, 0u, 2, s);
+    launch_fill_u32(p->counters, 0u, kPrCounterWords, s);
     if (E)
         hipLaunchKernelGGL(plan_copy, dim3(grid_for((uint64_t)N * kWave, 256, 1u << 16)), dim3(256), 0, s, cv,
                            hub_flag, p->reg_off, p->ebase, grank, p->reg_src, bad);
@@ -1463,8 +1483,8 @@ int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, con
     const uint32_t nhubt = (p->nchunk + kPrWaves - 1) / kPrWaves;
     PrArgs a{p->reg_off, p->reg_src, p->blk_row, p->blk_graph, p->tile_e0, p->tile_e1, p->hub_bits, p->ebase,
              p->chunk_beg, p->chunk_len, p->hub_src,
-             p->outdeg, xg, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + 1, p->nchunk, nhubt,
-             p->nblk, nhubt + p->nblk, p->short_max};
+             p->outdeg, xg, r_in, x_out, r_out, p->chunk_part, p->blk_stat, p->counters + kPrStripeGap,
+             p->nchunk, nhubt, p->nblk, nhubt + p->nblk, p->short_max};
     const size_t shmem = (size_t)2 * p->W * sizeof(float);
     if (p->persistent == 3) {
         hipLaunchKernelGGL(pr_sweep_hotwg_kernel, dim3(p->nhot_wg), dim3(kPrThreads), p->hotwg_lds, s, a, c, p->H);
