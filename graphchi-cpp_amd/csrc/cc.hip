@@ -232,10 +232,15 @@ __global__ void ccs_long_min(const uint32_t *off, const uint32_t *adj, uint32_t 
         const uint32_t b = off[v] + c * kCcsChunk, e = min(off[v + 1], b + kCcsChunk);
         const uint32_t lv = label[v];
         uint32_t m = 0xffffffffu;
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            const uint32_t u = adj[k];
-            if (u < n) m = min(m, label[u]);
-            else bad = 1;
+        for (uint32_t k0 = b + lane; k0 < e; k0 += 4 * kWave) {  // four gathers in flight per lane
+            uint32_t u[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) u[q] = k0 + q * kWave < e ? adj[k0 + q * kWave] : 0xffffffffu;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (u[q] < n) m = min(m, label[u[q]]);
+                else if (k0 + q * kWave < e) bad = 1;
+            }
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, kWave));
@@ -262,9 +267,13 @@ __global__ void ccs_long_mark(const uint32_t *off, const uint32_t *adj, uint32_t
         if (!dropped_flag[v]) continue;
         if (c == 0) dropped++;
         const uint32_t b = off[v] + c * kCcsChunk, e = min(off[v + 1], b + kCcsChunk);
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            const uint32_t u = adj[k];
-            if (u < n) next[u] = 1;
+        for (uint32_t k0 = b + lane; k0 < e; k0 += 4 * kWave) {
+            uint32_t u[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) u[q] = k0 + q * kWave < e ? adj[k0 + q * kWave] : 0xffffffffu;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (u[q] < n) next[u[q]] = 1;
         }
     }
     if (lane == 0 && dropped) atomicAdd(&stat[0], (unsigned long long)dropped);
