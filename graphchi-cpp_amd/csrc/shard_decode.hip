@@ -1,0 +1,317 @@
+// Device decode of a shard's adjacency stream (the bytes of a .adj file).
+//
+// Reference: memory_shard::load_vertices (src/shards/memoryshard.hpp:448-497)
+// parses the stream record by record: a 0 byte and the byte nz after it skip
+// 1 + nz vertices; a count byte c (0xff: a uint32 count follows) is followed by
+// c uint32 dst ids of the current vertex, which then advances by one.  The
+// records have no markers, so where one starts depends on every record before
+// it.  Device form, for a stream of n bytes cut into 32 KB chunks:
+//   1. sd_exits: every byte position p is taken as if a record started there
+//      (next(p) = p + its length); one workgroup per chunk resolves, by pointer
+//      jumping in LDS, the first position at or beyond the chunk's end that the
+//      chain from p reaches (X[p]).  ~15 rounds over 32K entries.
+//   2. sd_chain: one thread walks the chunks from position 0 -- entry[c] is the
+//      first record start in chunk c, X[entry[c]] the first in the next chunk
+//      the chain reaches -- one dependent load per chunk instead of per record,
+//      and checks that the chain ends exactly at n.
+//   3. sd_count / sd_emit: each chunk parses its own records from its entry,
+//      from the chunk's bytes in LDS: counts (edges and vertex advance, scanned
+//      into bases), then (src, dst) pairs, written through an LDS queue so the
+//      workgroup copies many short records' dst words at once.
+// The output is the same edge sequence as the reference's parse: the shard's
+// edges in stream order, src ids counted from 0.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace gchi {
+
+constexpr uint32_t kSdChunk = 32768;  // bytes per chunk
+constexpr int kSdThreads = 1024;      // exit-table workgroup
+constexpr int kSdCopyThreads = 256;   // count / emit workgroups
+constexpr uint32_t kSdNone = 0xffffffffu;
+constexpr uint32_t kSdQueue = 1024;   // records per emit batch
+
+// Position after the record starting at p, or ~0 when it runs past n.
+GCHI_DEV uint64_t sd_next(const uint8_t *__restrict__ adj, uint64_t n, uint64_t p) {
+    const uint32_t b = adj[p];
+    uint64_t q;
+    if (b == 0) {
+        q = p + 2;
+    } else if (b != 0xff) {
+        q = p + 1 + 4ull * b;
+    } else {
+        if (p + 5 > n) return ~0ull;
+        const uint32_t c = (uint32_t)adj[p + 1] | ((uint32_t)adj[p + 2] << 8) | ((uint32_t)adj[p + 3] << 16) |
+                           ((uint32_t)adj[p + 4] << 24);
+        q = p + 5 + 4ull * c;
+    }
+    return q <= n ? q : ~0ull;
+}
+
+__global__ __launch_bounds__(kSdThreads) void sd_exits(const uint8_t *__restrict__ adj, uint64_t n,
+                                                       uint32_t *__restrict__ X) {
+    __shared__ uint32_t L[kSdChunk];  // chain position relative to the chunk; >= len once it has left
+    __shared__ int s_more[2];
+    const uint64_t start = (uint64_t)blockIdx.x * kSdChunk;
+    const uint32_t len = n - start < kSdChunk ? (uint32_t)(n - start) : kSdChunk;
+    for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
+        const uint64_t q = sd_next(adj, n, start + i);
+        L[i] = q == ~0ull ? kSdNone : (uint32_t)(q - start);
+    }
+    if (threadIdx.x < 2) s_more[threadIdx.x] = 1;
+    __syncthreads();
+    // Each round replaces an inside position by the one it leads to.  Values
+    // only move forward along the chain, so reading an entry another thread
+    // has already advanced this round is harmless.  Records are at least 2
+    // bytes long: at most log2(kSdChunk / 2) + 1 rounds.
+    for (int round = 0; round < 32 && s_more[round & 1]; round++) {
+        if (threadIdx.x == 0) s_more[(round + 1) & 1] = 0;
+        __syncthreads();
+        int more = 0;
+        for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
+            const uint32_t x = L[i];
+            if (x < len) {
+                const uint32_t y = L[x];
+                L[i] = y;
+                more |= y < len;
+            }
+        }
+        if (more) s_more[(round + 1) & 1] = 1;
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
+        const uint32_t x = L[i];
+        X[start + i] = x == kSdNone ? kSdNone : (uint32_t)(start + x);
+    }
+}
+
+// status: [0] malformed stream.
+__global__ void sd_chain(const uint32_t *__restrict__ X, uint64_t n, uint32_t nch, uint32_t *__restrict__ entry,
+                         uint32_t *__restrict__ status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t s = 0;
+    for (uint32_t c = 0; c < nch; c++) {
+        const uint64_t end = n < (uint64_t)(c + 1) * kSdChunk ? n : (uint64_t)(c + 1) * kSdChunk;
+        if (s >= end) continue;  // chunk c lies inside one record: no record starts there
+        entry[c] = (uint32_t)s;
+        const uint32_t x = X[s];
+        if (x == kSdNone) {
+            status[0] = 1;
+            return;
+        }
+        s = x;
+    }
+    if (s != n) status[0] = 1;
+}
+
+// Loads chunk c's bytes and the 5 after it (when inside the stream) into LDS.
+GCHI_DEV uint32_t sd_load_chunk(const uint8_t *__restrict__ adj, uint64_t n, uint32_t c, uint8_t *B) {
+    const uint64_t start = (uint64_t)c * kSdChunk;
+    const uint32_t len = n - start < kSdChunk ? (uint32_t)(n - start) : kSdChunk;
+    const uint32_t ext = n - start < kSdChunk + 5 ? (uint32_t)(n - start) : kSdChunk + 5;
+    for (uint32_t i = threadIdx.x; i < ext; i += blockDim.x) B[i] = adj[start + i];
+    return len;
+}
+
+GCHI_DEV uint32_t sd_u32(const uint8_t *B, uint32_t i) {
+    return (uint32_t)B[i] | ((uint32_t)B[i + 1] << 8) | ((uint32_t)B[i + 2] << 16) | ((uint32_t)B[i + 3] << 24);
+}
+
+// Per chunk: edges and vertex advance of the records starting in it.
+__global__ __launch_bounds__(kSdCopyThreads) void sd_count(const uint8_t *__restrict__ adj, uint64_t n,
+                                                           const uint32_t *__restrict__ entry,
+                                                           uint32_t *__restrict__ nedge, uint32_t *__restrict__ vadv,
+                                                           uint32_t *__restrict__ status) {
+    __shared__ uint8_t B[kSdChunk + 8];
+    const uint32_t c = blockIdx.x;
+    const uint32_t e0 = entry[c];
+    if (e0 == kSdNone) {
+        if (threadIdx.x == 0) nedge[c] = vadv[c] = 0u;
+        return;
+    }
+    const uint32_t len = sd_load_chunk(adj, n, c, B);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    // 64-bit position: a record that runs past the stream (the host then
+    // reports the stream as truncated) must not wrap the walk around
+    uint64_t p = e0 - (uint64_t)c * kSdChunk;
+    uint64_t ne = 0, va = 0;
+    while (p < len) {
+        const uint32_t b = B[p];
+        if (b == 0) {
+            va += 1 + (uint32_t)B[p + 1];
+            p += 2;
+            continue;
+        }
+        const uint32_t cnt = b != 0xff ? b : sd_u32(B, p + 1);
+        ne += cnt;
+        va += 1;
+        p += (b != 0xff ? 1u : 5u) + 4ull * cnt;
+    }
+    if (ne > 0xffffffffull || va > 0xffffffffull) status[1] = 1;
+    nedge[c] = (uint32_t)ne;
+    vadv[c] = (uint32_t)va;
+}
+
+// Per chunk: (src, dst) pairs of its records at their global edge positions.
+__global__ __launch_bounds__(kSdCopyThreads) void sd_emit(const uint8_t *__restrict__ adj, uint64_t n,
+                                                          const uint32_t *__restrict__ entry,
+                                                          const uint64_t *__restrict__ ebase,
+                                                          const uint64_t *__restrict__ vbase,
+                                                          uint32_t *__restrict__ out_src,
+                                                          uint32_t *__restrict__ out_dst) {
+    __shared__ uint8_t B[kSdChunk + 8];
+    __shared__ uint32_t q_src[kSdQueue];
+    __shared__ uint64_t q_pos[kSdQueue];   // byte position of the record's first dst word
+    __shared__ uint64_t q_eoff[kSdQueue + 1];  // edge offset of each record (exclusive), + the batch end
+    __shared__ uint32_t s_nq;
+    __shared__ uint64_t s_p;
+    __shared__ uint64_t s_e, s_v;
+    __shared__ int s_done;
+    const uint32_t c = blockIdx.x;
+    const uint32_t e0 = entry[c];
+    if (e0 == kSdNone) return;
+    const uint32_t len = sd_load_chunk(adj, n, c, B);
+    const uint64_t start = (uint64_t)c * kSdChunk;
+    if (threadIdx.x == 0) {
+        s_p = e0 - (uint64_t)c * kSdChunk;
+        s_e = ebase[c];
+        s_v = vbase[c];
+        s_done = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) {  // walk: queue up to kSdQueue records with edges
+            uint64_t p = s_p;
+            uint32_t nq = 0;
+            uint64_t e = s_e, v = s_v;
+            while (p < len && nq < kSdQueue) {
+                const uint32_t b = B[p];
+                if (b == 0) {
+                    v += 1 + (uint32_t)B[p + 1];
+                    p += 2;
+                    continue;
+                }
+                const uint32_t cnt = b != 0xff ? b : sd_u32(B, p + 1);
+                const uint32_t hdr = b != 0xff ? 1u : 5u;
+                if (cnt) {
+                    q_src[nq] = (uint32_t)v;
+                    q_pos[nq] = start + p + hdr;
+                    q_eoff[nq] = e;
+                    nq++;
+                }
+                e += cnt;
+                v += 1;
+                p += hdr + 4ull * cnt;
+            }
+            q_eoff[nq] = e;
+            s_nq = nq;
+            s_p = p;
+            s_e = e;
+            s_v = v;
+            s_done = p >= len;
+        }
+        __syncthreads();
+        const uint32_t nq = s_nq;
+        if (nq) {
+            const uint64_t b0 = q_eoff[0], total = q_eoff[nq] - b0;
+            for (uint64_t k = threadIdx.x; k < total; k += kSdCopyThreads) {
+                const uint64_t eo = b0 + k;
+                uint32_t lo = 0, hi = nq - 1;  // the record holding edge eo: largest r with q_eoff[r] <= eo
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (q_eoff[mid] <= eo) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint64_t bp = q_pos[lo] + 4ull * (eo - q_eoff[lo]);
+                const uint32_t d = (uint32_t)adj[bp] | ((uint32_t)adj[bp + 1] << 8) | ((uint32_t)adj[bp + 2] << 16) |
+                                   ((uint32_t)adj[bp + 3] << 24);
+                out_src[eo] = q_src[lo];
+                out_dst[eo] = d;
+            }
+        }
+        const int done = s_done;
+        __syncthreads();  // the queue is rewritten by the next walk
+        if (done) break;
+    }
+}
+
+struct SdWork {
+    uint32_t *X, *entry, *nedge, *vadv, *status;
+    uint64_t *ebase, *vbase;
+    void *scan;
+    size_t scan_bytes;
+};
+
+template <typename C>
+static void sd_layout(C &c, uint64_t nbytes, SdWork *w) {
+    const uint64_t nch = (nbytes + kSdChunk - 1) / kSdChunk;
+    auto *X = c.template take<uint32_t>(nbytes ? nbytes : 1);
+    auto *en = c.template take<uint32_t>(nch + 1);
+    auto *ne = c.template take<uint32_t>(nch + 1);
+    auto *va = c.template take<uint32_t>(nch + 1);
+    auto *st = c.template take<uint32_t>(2);
+    auto *eb = c.template take<uint64_t>(nch + 1);
+    auto *vb = c.template take<uint64_t>(nch + 1);
+    const size_t sb = scan_workspace((size_t)nch + 1);
+    void *sw = c.template take<char>(sb);
+    if (w) *w = SdWork{X, en, ne, va, st, eb, vb, sw, sb};
+}
+
+}  // namespace gchi
+
+using namespace gchi;
+
+extern "C" int gchi_shard_decode_workspace(uint64_t nbytes, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    SizerAdapter a;
+    sd_layout(a, nbytes, nullptr);
+    *bytes = a.z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_shard_decode(const uint8_t *d_adj, uint64_t nbytes, uint32_t *d_src, uint32_t *d_dst,
+                                 uint64_t max_edges, uint64_t *h_nedges, uint64_t *h_vend, void *d_work,
+                                 size_t work_bytes, void *stream) {
+    GCHI_CHECK(h_nedges, GCHI_ERR_INVALID, "shard_decode: null h_nedges");
+    GCHI_CHECK(nbytes == 0 || d_adj, GCHI_ERR_INVALID, "shard_decode: null stream");
+    GCHI_CHECK(nbytes < 0xffffffffull, GCHI_ERR_UNSUPPORTED, "shard_decode: streams of 4 GiB or more");
+    GCHI_CHECK((d_src == nullptr) == (d_dst == nullptr), GCHI_ERR_INVALID, "shard_decode: give both outputs or none");
+    *h_nedges = 0;
+    if (h_vend) *h_vend = 0;
+    if (nbytes == 0) return GCHI_OK;
+    size_t need = 0;
+    gchi_shard_decode_workspace(nbytes, &need);
+    GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "shard_decode: workspace too small");
+    hipStream_t s = as_stream(stream);
+    Carver c(d_work, work_bytes);
+    SdWork w;
+    sd_layout(c, nbytes, &w);
+    const uint32_t nch = (uint32_t)((nbytes + kSdChunk - 1) / kSdChunk);
+    launch_fill_u32(w.entry, kSdNone, nch, s);
+    launch_fill_u32(w.status, 0u, 2, s);
+    hipLaunchKernelGGL(sd_exits, dim3(nch), dim3(kSdThreads), 0, s, d_adj, nbytes, w.X);
+    hipLaunchKernelGGL(sd_chain, dim3(1), dim3(64), 0, s, w.X, nbytes, nch, w.entry, w.status);
+    hipLaunchKernelGGL(sd_count, dim3(nch), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.entry, w.nedge, w.vadv,
+                       w.status);
+    GCHI_LAUNCH_CHECK();
+    GCHI_TRY(exclusive_scan_u32_to_u64(w.nedge, w.ebase, nch, w.scan, w.scan_bytes, s));
+    GCHI_TRY(exclusive_scan_u32_to_u64(w.vadv, w.vbase, nch, w.scan, w.scan_bytes, s));
+    uint32_t st[2] = {0, 0};
+    uint64_t tot[2] = {0, 0};
+    GCHI_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipMemcpyAsync(&tot[0], w.ebase + nch, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipMemcpyAsync(&tot[1], w.vbase + nch, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(st[0] == 0, GCHI_ERR_FORMAT, "shard_decode: truncated stream (a record runs past its end)");
+    GCHI_CHECK(st[1] == 0 && tot[1] <= 0x100000000ull, GCHI_ERR_OVERFLOW,
+               "shard_decode: vertex ids or a chunk's edge count exceed 32 bits");
+    *h_nedges = tot[0];
+    if (h_vend) *h_vend = tot[1];
+    if (!d_src) return GCHI_OK;
+    GCHI_CHECK(tot[0] <= max_edges, GCHI_ERR_INVALID, "shard_decode: the stream holds more than max_edges edges");
+    hipLaunchKernelGGL(sd_emit, dim3(nch), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.entry, w.ebase, w.vbase,
+                       d_src, d_dst);
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}

@@ -104,6 +104,9 @@ _SIGS = {
     "gchi_cc_labelprop_init": (c_i32, [c_vp, c_u32, c_vp]),
     "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_scheduled_workspace": (c_i32, [c_u32, c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_shard_decode_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_shard_decode": (c_i32, [c_vp, c_u64, c_vp, c_vp, c_u64, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), c_vp,
+                                  c_sz, c_vp]),
     "gchi_cc_scheduled": (c_i32, [c_vp, c_vp, c_u32, c_u32, c_vp, ctypes.POINTER(c_u32), ctypes.POINTER(c_u64), c_vp,
                                   c_sz, c_vp]),
     "gchi_tc_prepare_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
@@ -449,6 +452,22 @@ def cc_scheduled(off, adj, n, max_rounds=0):
     check(L.gchi_cc_scheduled(dptr(off), dptr(adj), n, max_rounds, dptr(labels), ctypes.byref(rounds),
                               ctypes.byref(evals), dptr(ws), nb.value, stream()), "cc_scheduled")
     return labels, rounds.value, evals.value
+
+
+def shard_decode(d_adj, nbytes):
+    """Device decode of one shard's .adj stream (a uint8 device tensor);
+    returns (src, dst, vend) device tensors / int."""
+    L = lib()
+    nb = c_sz()
+    check(L.gchi_shard_decode_workspace(nbytes, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    ne, vend = c_u64(), c_u64()
+    check(L.gchi_shard_decode(dptr(d_adj), nbytes, None, None, 0, ctypes.byref(ne), ctypes.byref(vend), dptr(ws),
+                              nb.value, stream()), "shard_decode(count)")
+    src, dst = empty_u32(ne.value), empty_u32(ne.value)
+    check(L.gchi_shard_decode(dptr(d_adj), nbytes, dptr(src), dptr(dst), ne.value, ctypes.byref(ne),
+                              ctypes.byref(vend), dptr(ws), nb.value, stream()), "shard_decode")
+    return src, dst, vend.value
 
 
 # ---------------------------------------------------------------- triangle counting

@@ -370,6 +370,19 @@ int gchi_shard_encode(const gchi_shards *shards, int shard, uint8_t *h_adj, uint
  * uncompressed default-valued edge-data blocks + .size. */
 int gchi_shard_write_files(const gchi_shards *shards, const char *basefilename, uint32_t edge_value_bytes,
                            void *stream);
+/* Decodes one shard's .adj byte stream, already in HBM, into (src, dst) pairs
+ * in stream order -- the parse of memory_shard::load_vertices
+ * (src/shards/memoryshard.hpp:448-497): a 0 byte and the byte nz after it skip
+ * 1 + nz vertices; a count byte (0xff: a uint32 count follows) and that many
+ * uint32 dst ids belong to the current vertex, which then advances by one.
+ * src ids count from 0 as in the file.  Streams of less than 4 GiB.
+ * *h_nedges = the stream's edge count, *h_vend (optional) = the vertex id after
+ * the last record.  d_src == d_dst == NULL: count only.  A record that runs
+ * past the end of the stream is GCHI_ERR_FORMAT; more than max_edges edges is
+ * GCHI_ERR_INVALID.  Host-synchronous (reads the counts). */
+int gchi_shard_decode_workspace(uint64_t nbytes, size_t *bytes);
+int gchi_shard_decode(const uint8_t *d_adj, uint64_t nbytes, uint32_t *d_src, uint32_t *d_dst, uint64_t max_edges,
+                      uint64_t *h_nedges, uint64_t *h_vend, void *d_work, size_t work_bytes, void *stream);
 
 #ifdef __cplusplus
 }

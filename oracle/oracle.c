@@ -18,6 +18,9 @@
  *   orc_stable_sort_u64    - equals numpy stable argsort on random keys
  *   orc_shard_*            - golden .intervals / .adj / .adjidx / _degs.bin of the
  *                            reference sharder (several nshards, both edge sizes)
+ *   orc_shard_decode       - the golden .adj bytes of the reference sharder decode
+ *                            to the golden cases' edges; encode -> decode round
+ *                            trips (gaps, 0xff counts); truncations are refused
  *   orc_degree_order       - golden .vertexmap of the reference order_by_degree
  *   orc_triangles          - golden totals/per-vertex .vout of the reference
  *                            trianglecounting; brute force on tiny graphs; K_n
@@ -539,4 +542,40 @@ double orc_pr_psw_omp(uint32_t n, uint64_t e, const uint32_t *src, const uint32_
     free(cum);
     free(istart);
     return secs;
+}
+
+int64_t orc_shard_decode(uint64_t nbytes, const uint8_t *adj, uint32_t *src, uint32_t *dst, uint64_t max_edges) {
+    /* memory_shard::load_vertices' parse (src/shards/memoryshard.hpp:448-497):
+     * a 0 byte and the byte nz after it skip 1 + nz vertices; a count byte
+     * (0xff: a uint32 count follows) and that many uint32 dst ids belong to the
+     * current vertex, which then advances by one. */
+    uint64_t p = 0, ne = 0, vid = 0;
+    while (p < nbytes) {
+        uint8_t ns = adj[p++];
+        if (ns == 0) {
+            if (p >= nbytes) return -1;
+            vid += 1 + (uint64_t)adj[p++];
+            continue;
+        }
+        uint64_t cnt = ns;
+        if (ns == 0xff) {
+            uint32_t c32;
+            if (p + 4 > nbytes) return -1;
+            memcpy(&c32, adj + p, 4);
+            p += 4;
+            cnt = c32;
+        }
+        if (p + 4 * cnt > nbytes) return -1;
+        for (uint64_t k = 0; k < cnt; k++, p += 4) {
+            if (src && ne < max_edges) {
+                uint32_t d;
+                memcpy(&d, adj + p, 4);
+                src[ne] = (uint32_t)vid;
+                dst[ne] = d;
+            }
+            ne++;
+        }
+        vid++;
+    }
+    return (int64_t)ne;
 }

@@ -64,6 +64,11 @@ int orc_shard_intervals(uint64_t e, const uint32_t *dst_sorted, uint32_t max_id,
                         uint64_t *counts);
 int64_t orc_shard_encode(uint64_t e, const uint32_t *src, const uint32_t *dst, uint8_t *adj, uint64_t *idx_triples,
                          uint64_t *nidx);
+/* orc_shard_decode: memory_shard::load_vertices' parse of an adjacency stream
+ * (src/shards/memoryshard.hpp:448-497) into (src, dst) pairs in stream order.
+ * Returns the edge count (pairs beyond max_edges are counted, not written; src
+ * may be NULL to count only), or -1 for a truncated stream. */
+int64_t orc_shard_decode(uint64_t nbytes, const uint8_t *adj, uint32_t *src, uint32_t *dst, uint64_t max_edges);
 
 /* order_by_degree (src/preprocessing/util/orderbydegree.hpp:58-100): rank of
  * each vertex by (in+out degree, id); vertexmap[old] = new id. */

@@ -34,6 +34,7 @@ def lib():
             "orc_auto_nshards": (i32, [u64, ctypes.c_size_t, i32]),
             "orc_shard_intervals": (i32, [u64, vp, u32, i32, vp, vp]),
             "orc_shard_encode": (ctypes.c_int64, [u64, vp, vp, vp, vp, vp]),
+            "orc_shard_decode": (ctypes.c_int64, [u64, vp, vp, vp, u64]),
             "orc_degree_order": (i32, [u32, vp, vp]),
             "orc_triangles": (i32, [u32, u64, vp, vp, vp, vp, vp]),
             "orc_pr_psw_omp": (dbl, [u32, u64, vp, vp, i32, dbl, dbl, f32, i32, i32, vp]),
@@ -138,6 +139,16 @@ def shard_encode(src, dst):
     lib().orc_shard_encode(n, _p(src) if n else None, _p(dst) if n else None, _p(adj), _p(idx), ctypes.byref(nidx))
     return adj[:size].tobytes(), idx[:3 * nidx.value].reshape(-1, 3)
 
+
+def shard_decode(adj):
+    """(src, dst) pairs of an adjacency stream, or None when it is truncated."""
+    a = np.ascontiguousarray(np.frombuffer(bytes(adj), np.uint8))
+    n = lib().orc_shard_decode(len(a), _p(a) if len(a) else None, None, None, 0)
+    if n < 0:
+        return None
+    src, dst = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    lib().orc_shard_decode(len(a), _p(a) if len(a) else None, _p(src) if n else None, _p(dst) if n else None, n)
+    return src, dst
 
 def degree_order(deg):
     deg = u32(deg)

@@ -173,6 +173,44 @@ def test_sharder_oracle_bit_exact_vs_reference(case, manifest):
         assert g["edata_blocks"] == max(1, -(-8 * len(s) // 1048576))
 
 
+@pytest.mark.parametrize("case", SHARDED, ids=ids(SHARDED))
+def test_shard_decode_reads_reference_bytes(case, manifest):
+    """The decoder restatement reads the reference's .adj bytes back into the
+    shard's edges: the literal bytes where the manifest holds them, else the
+    oracle encoding, whose SHA-256 equals the reference file's (test above)."""
+    import hashlib
+    src, dst, n = graphs.make(case)
+    m = manifest[case["name"]]["shards"]
+    _, shards = reference_shards(src, dst, n, case.get("nshards", 1))
+    for (s, d), g in zip(shards, m["shards"]):
+        adj = bytes.fromhex(g["adj_hex"]) if g.get("adj_hex") else oracle.shard_encode(s, d)[0]
+        assert hashlib.sha256(adj).hexdigest() == g["adj_sha"]
+        got = oracle.shard_decode(adj)
+        assert got is not None
+        np.testing.assert_array_equal(got[0], s)
+        np.testing.assert_array_equal(got[1], d)
+
+
+def test_shard_decode_round_trip_and_truncation():
+    rng = np.random.default_rng(7)
+    # first source far from 0 (leading zero runs), gaps over 255 vertices, a
+    # 70000-edge source (0xff count) and many short rows
+    src = np.concatenate([np.full(3, 600), np.full(70000, 601), rng.integers(1000, 5000, 20000),
+                          np.full(5, 9000)]).astype(np.uint32)
+    dst = rng.integers(0, 1 << 32, len(src), dtype=np.uint64).astype(np.uint32)
+    o = np.lexsort((dst, src))
+    src, dst = src[o], dst[o]
+    adj, _ = oracle.shard_encode(src, dst)
+    s, d = oracle.shard_decode(adj)
+    np.testing.assert_array_equal(s, src)
+    np.testing.assert_array_equal(d, dst)
+    assert oracle.shard_decode(b"") is not None and len(oracle.shard_decode(b"")[0]) == 0
+    assert oracle.shard_decode(b"\x00") is None                # zero run without its count byte
+    assert oracle.shard_decode(b"\xff\x01\x00") is None        # 0xff without its 4-byte count
+    assert oracle.shard_decode(b"\x02\x01\x00\x00\x00") is None  # two dst words announced, one present
+    assert oracle.shard_decode(adj[:-1]) is None
+
+
 def test_auto_nshards_rule():
     # sharder.hpp:572-575 with membudget 800 (conf/graphchi.cnf): configs of the workloads
     assert oracle.auto_nshards(67_108_864, 8, 800) == 6
