@@ -120,8 +120,21 @@ def bench_load(gchi, torch, src, dst, E, N, tune):
     g, t_csc = timed(lambda: gchi.DeviceGraph(ds, dd, N), torch)
     plan, t_plan = timed(lambda: gchi.PageRankPlan(g, **tune), torch)
     del plan, g, ds, dd
+    # the reference's on-disk form: the graph's shards (.adj byte streams, from
+    # the device sharder) decoded in HBM back into edge lists
+    sh = gchi.Shards(src[:E], dst[:E], N - 1, gchi.shard_auto_count(E, 8))
+    streams = [sh.encode(p)[0] for p in range(sh.nshards)]
+    sh.close()
+    d_streams = [torch.from_numpy(np.frombuffer(b, np.uint8).copy()).to("cuda") for b in streams]
+    gchi.shard_decode(d_streams[0], len(streams[0]))
+    outs, t_dec = timed(lambda: [gchi.shard_decode(d, len(b)) for d, b in zip(d_streams, streams)], torch)
+    nbytes = sum(len(b) for b in streams)
+    assert sum(int(o[0].numel()) for o in outs) == E
+    del outs, d_streams
     return {"upload_ms": round(t_up * 1e3, 2), "upload_GBps": round(8 * E / t_up / 1e9, 1),
-            "csc_build_ms": round(t_csc * 1e3, 2), "plan_build_ms": round(t_plan * 1e3, 2)}
+            "csc_build_ms": round(t_csc * 1e3, 2), "plan_build_ms": round(t_plan * 1e3, 2),
+            "shard_decode_ms": round(t_dec * 1e3, 2), "shard_decode_streams": len(streams),
+            "shard_decode_bytes": nbytes, "shard_decode_GBps": round(nbytes / t_dec / 1e9, 1)}
 
 
 def sweep_kernels(info):

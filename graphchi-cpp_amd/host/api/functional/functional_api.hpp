@@ -192,10 +192,10 @@ void run_functional_device(std::string filename, int niters, metrics &m) {
     int nshards = convert_if_notexists<PairContainer<typename KERNEL::EdgeDataType>>(
         filename, get_option_string("nshards", "auto"));
     const size_t n = get_num_vertices(filename);
-    std::vector<vid_t> src, dst;
     metrics_entry ml = m.start_time();
-    load_shard_edges(filename, nshards, src, dst);
-    device::graph g(src, dst, (uint32_t)n, false);
+    device::buffer<uint32_t> src, dst;
+    const uint64_t ne = load_shard_edges_device(filename, nshards, src, dst);
+    device::graph g(std::move(src), std::move(dst), ne, (uint32_t)n, false);
     m.stop_time(ml, "load_graph");
     std::vector<uint32_t> indeg = g.indeg.to_host(n), outdeg = g.outdeg.to_host(n);
     graphchi_context ctx;

@@ -39,6 +39,20 @@ class buffer {
     ~buffer() { release(); }
     buffer(const buffer &) = delete;
     buffer &operator=(const buffer &) = delete;
+    buffer(buffer &&o) noexcept : p_(o.p_), n_(o.n_) {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    buffer &operator=(buffer &&o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
     void resize(size_t n) {
         release();
         n_ = n;
@@ -88,6 +102,17 @@ struct graph {
             src.upload(h_src.data(), e);
             dst.upload(h_dst.data(), e);
         }
+        build(keep_edges);
+    }
+    // From an edge list already in HBM (the device shard decode).
+    graph(buffer<uint32_t> &&d_src, buffer<uint32_t> &&d_dst, uint64_t nedges, uint32_t nvertices, bool keep_edges)
+        : n(nvertices), e(nedges), src(std::move(d_src)), dst(std::move(d_dst)) {
+        open();
+        build(keep_edges);
+    }
+
+  private:
+    void build(bool keep_edges) {
         in_off.resize((size_t)n + 1);
         in_src.resize(e);
         outdeg.resize(n);

@@ -97,12 +97,15 @@ class graphchi_engine {
     void set_reset_vertexdata(bool) {}
 
   private:
-    void load_edges(std::vector<vid_t> &src, std::vector<vid_t> &dst) {
+    // The shards' edges, decoded on the device into a graph resident in HBM.
+    device::graph load_graph() {
         metrics_entry me = m_.start_time();
-        load_shard_edges(base_, nshards_, src, dst);
-        nedges_ = src.size();
+        device::buffer<uint32_t> src, dst;
+        nedges_ = load_shard_edges_device(base_, nshards_, src, dst);
         ctx_.nedges = nedges_;
+        device::graph g(std::move(src), std::move(dst), nedges_, (uint32_t)nvertices_, true);
         m_.stop_time(me, "load_shards");
+        return g;
     }
 
     template <class ProgramT>
@@ -119,9 +122,7 @@ class graphchi_engine {
     template <class ProgramT>
     void run_impl(ProgramT &program, device::min_label_components) {
         static_assert(sizeof(VertexDataType) == 4, "min-label components writes 4-byte labels");
-        std::vector<vid_t> src, dst;
-        load_edges(src, dst);
-        device::graph g(src, dst, (uint32_t)nvertices_, true);
+        device::graph g = load_graph();
         ctx_.iteration = 0;
         program.before_iteration(0, ctx_);
         metrics_entry me = m_.start_time();
@@ -143,9 +144,7 @@ class graphchi_engine {
     template <class ProgramT>
     void run_impl(ProgramT &program, device::triangle_counting) {
         static_assert(sizeof(VertexDataType) == 4, "triangle counting writes uint32 counts");
-        std::vector<vid_t> src, dst;
-        load_edges(src, dst);
-        device::graph g(src, dst, (uint32_t)nvertices_, true);
+        device::graph g = load_graph();
         const uint32_t n = (uint32_t)nvertices_;
         ctx_.iteration = 0;
         program.before_iteration(0, ctx_);

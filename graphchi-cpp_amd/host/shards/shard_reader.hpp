@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "api/chifilenames.hpp"
+#include "engine/device.hpp"
 #include "logger/logger.hpp"
 
 namespace graphchi {
@@ -71,6 +72,46 @@ inline void load_shard_edges(const std::string &base, int nshards, std::vector<v
         if (!decode_adjacency(adj, src, dst))
             logstream(LOG_FATAL) << "malformed shard " << filename_shard_adj(base, (int)p, nshards) << std::endl;
     }
+}
+
+// The device load path: every shard's .adj bytes are uploaded once and decoded
+// in HBM (gchi_shard_decode), and the edges stay there for the device build.
+// Same edge sequence as load_shard_edges.  Returns the edge count.
+inline uint64_t load_shard_edges_device(const std::string &base, int nshards, device::buffer<uint32_t> &src,
+                                        device::buffer<uint32_t> &dst) {
+    device::open();
+    std::vector<vid_t> ends = load_interval_ends(base, nshards);
+    std::vector<device::buffer<uint8_t>> bytes(ends.size());
+    std::vector<uint64_t> nbytes(ends.size()), count(ends.size());
+    size_t ws_max = 0;
+    for (size_t p = 0; p < ends.size(); p++) {
+        std::vector<uint8_t> adj = read_file_bytes(filename_shard_adj(base, (int)p, nshards));
+        nbytes[p] = adj.size();
+        bytes[p].resize(adj.size());
+        if (!adj.empty()) bytes[p].upload(adj.data(), adj.size());
+        size_t ws = 0;
+        device::check(gchi_shard_decode_workspace(nbytes[p], &ws), "shard decode workspace");
+        ws_max = ws > ws_max ? ws : ws_max;
+    }
+    device::workspace w(ws_max);
+    uint64_t total = 0;
+    for (size_t p = 0; p < ends.size(); p++) {
+        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], nullptr, nullptr, 0, &count[p], nullptr, w.get(),
+                                        ws_max, nullptr),
+                      filename_shard_adj(base, (int)p, nshards).c_str());
+        total += count[p];
+    }
+    src.resize(total);
+    dst.resize(total);
+    uint64_t at = 0;
+    for (size_t p = 0; p < ends.size(); p++) {
+        uint64_t got = 0;
+        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], src.get() + at, dst.get() + at, count[p], &got,
+                                        nullptr, w.get(), ws_max, nullptr),
+                      filename_shard_adj(base, (int)p, nshards).c_str());
+        at += got;
+    }
+    return total;
 }
 
 }  // namespace graphchi
