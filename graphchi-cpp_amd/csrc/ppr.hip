@@ -15,7 +15,10 @@
 // query block instead of once per query.  Rows come from the PageRank plan
 // (regular blocks of < 2W edges, hub rows cut into 512-edge chunks whose
 // 64-wide partials are combined in chunk order).  A persistent kernel pulls
-// (query block, tile) items, query-block-major, from a device work counter.
+// (query block, tile) items, query-block-major, from kPpStripes device work
+// counters (stripe s hands out items s, s + kPpStripes, ...; each counter in
+// its own 128-byte line, since atomics on one address complete one after
+// another).
 //
 // Precision: phase 1 stores ranks in BF16 with FP32 accumulation; phase 2
 // converts to FP32 and refines with FP32 sweeps until every query meets the
@@ -64,6 +67,8 @@ namespace gchi {
 constexpr int kPpThreads = 256;
 constexpr int kPpWaves = kPpThreads / kWave;
 constexpr int kQ = 64;                 // queries per block (one per lane)
+constexpr uint32_t kPpStripes = 16;    // work counters of the persistent sweep
+constexpr uint32_t kPpStripeGap = 32;  // uint32 words between counters
 constexpr int kMaxSeeds = 64;          // restart-set size limit per query
 constexpr int kSeedTable = kQ * kMaxSeeds;
 
@@ -181,10 +186,21 @@ __global__ __launch_bounds__(kPpThreads) void ppr_sweep_kernel(PpArgs a, const T
     uint32_t cur_qb = 0xffffffffu;
     double res = 0.0;  // this lane's query, accumulated by this wave for cur_qb
     if (threadIdx.x < kQ) s_res[threadIdx.x] = 0.0;
+    uint32_t stripe = blockIdx.x % kPpStripes, spent = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const uint32_t t = s_tile;
+        uint32_t t = ntiles;  // the next tile from the stripes; ntiles once all are drained
+        while (spent < kPpStripes) {  // block-uniform
+            if (threadIdx.x == 0) s_tile = atomicAdd(&a.counter[stripe * kPpStripeGap], 1u);
+            __syncthreads();
+            const uint64_t t64 = stripe + (uint64_t)kPpStripes * s_tile;
+            __syncthreads();  // s_tile is rewritten by the next ticket
+            if (t64 < ntiles) {
+                t = (uint32_t)t64;
+                break;
+            }
+            spent++;
+            stripe = (stripe + 1) % kPpStripes;
+        }
         const uint32_t qb = t < ntiles ? t / T_per_qb : 0xffffffffu;
         if (qb != cur_qb) {  // block-uniform: every thread read the same tile
             if (a.residual && cur_qb != 0xffffffffu) {
@@ -259,7 +275,8 @@ __global__ __launch_bounds__(kPpThreads) void ppr_hub_kernel(PpArgs a, const T *
         }
         Store<T>::st(Rout + (size_t)v * kQ + lane, rn);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;  // work queue ready for the next sweep
+    if (blockIdx.x == 0)  // work queue ready for the next sweep
+        for (uint32_t i = threadIdx.x; i < kPpStripes; i += blockDim.x) a.counter[i * kPpStripeGap] = 0u;
 }
 
 template <typename T>
@@ -366,7 +383,7 @@ extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries
     z.take<uint32_t>((size_t)b->nqb * kSeedTable);
     z.take<uint32_t>(b->nqb);
     z.take<float>((size_t)b->nqb * kQ);
-    z.take<uint32_t>(1);
+    z.take<uint32_t>(kPpStripes * kPpStripeGap);
     if (hipMalloc(&b->mem, z.off) != hipSuccess) {
         delete b;
         return fail(GCHI_ERR_HIP, "ppr_batch_create: hipMalloc failed (state " + std::to_string(z.off) + " bytes)");
@@ -384,10 +401,10 @@ extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries
     b->seed_q = c.take<uint32_t>((size_t)b->nqb * kSeedTable);
     b->seed_n = c.take<uint32_t>(b->nqb);
     b->tele = c.take<float>((size_t)b->nqb * kQ);
-    b->counter = c.take<uint32_t>(1);
+    b->counter = c.take<uint32_t>(kPpStripes * kPpStripeGap);
     hipLaunchKernelGGL(inv_outdeg_kernel, dim3(grid_for(b->pv.N, 256, 65536)), dim3(256), 0, 0, b->pv.outdeg, b->pv.N,
                        b->inv_outdeg);
-    launch_fill_u32(b->counter, 0u, 1, 0);
+    launch_fill_u32(b->counter, 0u, kPpStripes * kPpStripeGap, 0);
     GCHI_LAUNCH_CHECK();
     GCHI_HIP(hipDeviceSynchronize());
     *out = b;
