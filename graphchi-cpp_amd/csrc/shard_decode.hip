@@ -5,17 +5,19 @@
 // 1 + nz vertices; a count byte c (0xff: a uint32 count follows) is followed by
 // c uint32 dst ids of the current vertex, which then advances by one.  The
 // records have no markers, so where one starts depends on every record before
-// it.  Device form, for a stream of n bytes cut into 32 KB chunks:
+// it.  Device form, for a stream of n bytes cut into 32 KB chunks of four 8 KB
+// pieces:
 //   1. sd_exits: every byte position p is taken as if a record started there
 //      (next(p) = p + its length); one workgroup per chunk resolves, by pointer
-//      jumping in LDS, the first position at or beyond the chunk's end that the
-//      chain from p reaches (X[p]).  ~15 rounds over 32K entries.
+//      jumping in LDS, the first position at or beyond the end of p's piece
+//      (X8[p]), then of p's chunk (X[p]), that the chain from p reaches.
 //   2. sd_chain: one thread walks the chunks from position 0 -- entry[c] is the
 //      first record start in chunk c, X[entry[c]] the first in the next chunk
 //      the chain reaches -- one dependent load per chunk instead of per record,
-//      and checks that the chain ends exactly at n.
-//   3. sd_count / sd_emit: each chunk parses its own records from its entry,
-//      from the chunk's bytes in LDS: counts (edges and vertex advance, scanned
+//      and checks that the chain ends exactly at n.  sd_pieces then finds each
+//      piece's first record start from its chunk's (up to four X8 loads).
+//   3. sd_count / sd_emit: each piece parses its own records from its entry,
+//      from the piece's bytes in LDS: counts (edges and vertex advance, scanned
 //      into bases), then (src, dst) pairs, written through an LDS queue so the
 //      workgroup copies many short records' dst words at once.
 // The output is the same edge sequence as the reference's parse: the shard's
@@ -25,7 +27,9 @@
 
 namespace gchi {
 
-constexpr uint32_t kSdChunk = 32768;  // bytes per chunk
+constexpr uint32_t kSdChunk = 32768;  // bytes per chunk (the chain walk's step)
+constexpr uint32_t kSdPiece = 8192;   // bytes per piece (one record walk each)
+constexpr uint32_t kSdPieces = kSdChunk / kSdPiece;
 constexpr int kSdThreads = 1024;      // exit-table workgroup
 constexpr int kSdCopyThreads = 256;   // count / emit workgroups
 constexpr uint32_t kSdNone = 0xffffffffu;
@@ -49,8 +53,8 @@ GCHI_DEV uint64_t sd_next(const uint8_t *__restrict__ adj, uint64_t n, uint64_t 
 }
 
 __global__ __launch_bounds__(kSdThreads) void sd_exits(const uint8_t *__restrict__ adj, uint64_t n,
-                                                       uint32_t *__restrict__ X) {
-    __shared__ uint32_t L[kSdChunk];  // chain position relative to the chunk; >= len once it has left
+                                                       uint32_t *__restrict__ X8, uint32_t *__restrict__ X) {
+    __shared__ uint32_t L[kSdChunk];  // chain position relative to the chunk
     __shared__ int s_more[2];
     const uint64_t start = (uint64_t)blockIdx.x * kSdChunk;
     const uint32_t len = n - start < kSdChunk ? (uint32_t)(n - start) : kSdChunk;
@@ -58,30 +62,36 @@ __global__ __launch_bounds__(kSdThreads) void sd_exits(const uint8_t *__restrict
         const uint64_t q = sd_next(adj, n, start + i);
         L[i] = q == ~0ull ? kSdNone : (uint32_t)(q - start);
     }
-    if (threadIdx.x < 2) s_more[threadIdx.x] = 1;
-    __syncthreads();
-    // Each round replaces an inside position by the one it leads to.  Values
-    // only move forward along the chain, so reading an entry another thread
-    // has already advanced this round is harmless.  Records are at least 2
-    // bytes long: at most log2(kSdChunk / 2) + 1 rounds.
-    for (int round = 0; round < 32 && s_more[round & 1]; round++) {
-        if (threadIdx.x == 0) s_more[(round + 1) & 1] = 0;
+    // Phase 0 stops at the end of each position's piece, phase 1 at the end of
+    // the chunk.  Each round replaces an inside position by the one it leads
+    // to; values only move forward along the chain, so reading an entry that
+    // another thread has already advanced this round is harmless.  Records are
+    // at least 2 bytes long: at most log2(kSdChunk / 2) + 1 rounds per phase.
+    for (int phase = 0; phase < 2; phase++) {
+        if (threadIdx.x < 2) s_more[threadIdx.x] = 1;
         __syncthreads();
-        int more = 0;
+        for (int round = 0; round < 32 && s_more[round & 1]; round++) {
+            if (threadIdx.x == 0) s_more[(round + 1) & 1] = 0;
+            __syncthreads();
+            int more = 0;
+            for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
+                const uint32_t lim = phase ? len : min(len, (i / kSdPiece + 1) * kSdPiece);
+                const uint32_t x = L[i];
+                if (x < lim) {
+                    const uint32_t y = L[x];
+                    L[i] = y;
+                    more |= y < lim;
+                }
+            }
+            if (more) s_more[(round + 1) & 1] = 1;
+            __syncthreads();
+        }
+        uint32_t *out = phase ? X : X8;
         for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
             const uint32_t x = L[i];
-            if (x < len) {
-                const uint32_t y = L[x];
-                L[i] = y;
-                more |= y < len;
-            }
+            out[start + i] = x == kSdNone ? kSdNone : (uint32_t)(start + x);
         }
-        if (more) s_more[(round + 1) & 1] = 1;
         __syncthreads();
-    }
-    for (uint32_t i = threadIdx.x; i < len; i += kSdThreads) {
-        const uint32_t x = L[i];
-        X[start + i] = x == kSdNone ? kSdNone : (uint32_t)(start + x);
     }
 }
 
@@ -104,11 +114,25 @@ __global__ void sd_chain(const uint32_t *__restrict__ X, uint64_t n, uint32_t nc
     if (s != n) status[0] = 1;
 }
 
-// Loads chunk c's bytes and the 5 after it (when inside the stream) into LDS.
-GCHI_DEV uint32_t sd_load_chunk(const uint8_t *__restrict__ adj, uint64_t n, uint32_t c, uint8_t *B) {
-    const uint64_t start = (uint64_t)c * kSdChunk;
-    const uint32_t len = n - start < kSdChunk ? (uint32_t)(n - start) : kSdChunk;
-    const uint32_t ext = n - start < kSdChunk + 5 ? (uint32_t)(n - start) : kSdChunk + 5;
+// First record start of every piece, from its chunk's entry.
+__global__ void sd_pieces(const uint32_t *__restrict__ X8, uint64_t n, uint32_t nch,
+                          const uint32_t *__restrict__ entry, uint32_t *__restrict__ pentry) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    uint32_t s = entry[c];
+    for (uint32_t j = 0; j < kSdPieces; j++) {
+        const uint64_t pend = (uint64_t)c * kSdChunk + (uint64_t)(j + 1) * kSdPiece;
+        const bool here = s != kSdNone && (uint64_t)s < pend && (uint64_t)s < n;
+        pentry[c * kSdPieces + j] = here ? s : kSdNone;
+        if (here) s = X8[s];  // a valid chain: X8 is never kSdNone on it
+    }
+}
+
+// Loads piece c's bytes and the 5 after it (when inside the stream) into LDS.
+GCHI_DEV uint32_t sd_load_piece(const uint8_t *__restrict__ adj, uint64_t n, uint32_t c, uint8_t *B) {
+    const uint64_t start = (uint64_t)c * kSdPiece;
+    const uint32_t len = n - start < kSdPiece ? (uint32_t)(n - start) : kSdPiece;
+    const uint32_t ext = n - start < kSdPiece + 5 ? (uint32_t)(n - start) : kSdPiece + 5;
     for (uint32_t i = threadIdx.x; i < ext; i += blockDim.x) B[i] = adj[start + i];
     return len;
 }
@@ -117,24 +141,24 @@ GCHI_DEV uint32_t sd_u32(const uint8_t *B, uint32_t i) {
     return (uint32_t)B[i] | ((uint32_t)B[i + 1] << 8) | ((uint32_t)B[i + 2] << 16) | ((uint32_t)B[i + 3] << 24);
 }
 
-// Per chunk: edges and vertex advance of the records starting in it.
+// Per piece: edges and vertex advance of the records starting in it.
 __global__ __launch_bounds__(kSdCopyThreads) void sd_count(const uint8_t *__restrict__ adj, uint64_t n,
                                                            const uint32_t *__restrict__ entry,
                                                            uint32_t *__restrict__ nedge, uint32_t *__restrict__ vadv,
                                                            uint32_t *__restrict__ status) {
-    __shared__ uint8_t B[kSdChunk + 8];
+    __shared__ uint8_t B[kSdPiece + 8];
     const uint32_t c = blockIdx.x;
     const uint32_t e0 = entry[c];
     if (e0 == kSdNone) {
         if (threadIdx.x == 0) nedge[c] = vadv[c] = 0u;
         return;
     }
-    const uint32_t len = sd_load_chunk(adj, n, c, B);
+    const uint32_t len = sd_load_piece(adj, n, c, B);
     __syncthreads();
     if (threadIdx.x != 0) return;
     // 64-bit position: a record that runs past the stream (the host then
     // reports the stream as truncated) must not wrap the walk around
-    uint64_t p = e0 - (uint64_t)c * kSdChunk;
+    uint64_t p = e0 - (uint64_t)c * kSdPiece;
     uint64_t ne = 0, va = 0;
     while (p < len) {
         const uint32_t b = B[p];
@@ -153,14 +177,14 @@ __global__ __launch_bounds__(kSdCopyThreads) void sd_count(const uint8_t *__rest
     vadv[c] = (uint32_t)va;
 }
 
-// Per chunk: (src, dst) pairs of its records at their global edge positions.
+// Per piece: (src, dst) pairs of its records at their global edge positions.
 __global__ __launch_bounds__(kSdCopyThreads) void sd_emit(const uint8_t *__restrict__ adj, uint64_t n,
                                                           const uint32_t *__restrict__ entry,
                                                           const uint64_t *__restrict__ ebase,
                                                           const uint64_t *__restrict__ vbase,
                                                           uint32_t *__restrict__ out_src,
                                                           uint32_t *__restrict__ out_dst) {
-    __shared__ uint8_t B[kSdChunk + 8];
+    __shared__ uint8_t B[kSdPiece + 8];
     __shared__ uint32_t q_src[kSdQueue];
     __shared__ uint64_t q_pos[kSdQueue];   // byte position of the record's first dst word
     __shared__ uint64_t q_eoff[kSdQueue + 1];  // edge offset of each record (exclusive), + the batch end
@@ -171,10 +195,10 @@ __global__ __launch_bounds__(kSdCopyThreads) void sd_emit(const uint8_t *__restr
     const uint32_t c = blockIdx.x;
     const uint32_t e0 = entry[c];
     if (e0 == kSdNone) return;
-    const uint32_t len = sd_load_chunk(adj, n, c, B);
-    const uint64_t start = (uint64_t)c * kSdChunk;
+    const uint32_t len = sd_load_piece(adj, n, c, B);
+    const uint64_t start = (uint64_t)c * kSdPiece;
     if (threadIdx.x == 0) {
-        s_p = e0 - (uint64_t)c * kSdChunk;
+        s_p = e0 - (uint64_t)c * kSdPiece;
         s_e = ebase[c];
         s_v = vbase[c];
         s_done = 0;
@@ -237,7 +261,7 @@ __global__ __launch_bounds__(kSdCopyThreads) void sd_emit(const uint8_t *__restr
 }
 
 struct SdWork {
-    uint32_t *X, *entry, *nedge, *vadv, *status;
+    uint32_t *X8, *X, *entry, *pentry, *nedge, *vadv, *status;
     uint64_t *ebase, *vbase;
     void *scan;
     size_t scan_bytes;
@@ -245,17 +269,19 @@ struct SdWork {
 
 template <typename C>
 static void sd_layout(C &c, uint64_t nbytes, SdWork *w) {
-    const uint64_t nch = (nbytes + kSdChunk - 1) / kSdChunk;
+    const uint64_t nch = (nbytes + kSdChunk - 1) / kSdChunk, npc = nch * kSdPieces;
+    auto *X8 = c.template take<uint32_t>(nbytes ? nbytes : 1);
     auto *X = c.template take<uint32_t>(nbytes ? nbytes : 1);
     auto *en = c.template take<uint32_t>(nch + 1);
-    auto *ne = c.template take<uint32_t>(nch + 1);
-    auto *va = c.template take<uint32_t>(nch + 1);
+    auto *pe = c.template take<uint32_t>(npc + 1);
+    auto *ne = c.template take<uint32_t>(npc + 1);
+    auto *va = c.template take<uint32_t>(npc + 1);
     auto *st = c.template take<uint32_t>(2);
-    auto *eb = c.template take<uint64_t>(nch + 1);
-    auto *vb = c.template take<uint64_t>(nch + 1);
-    const size_t sb = scan_workspace((size_t)nch + 1);
+    auto *eb = c.template take<uint64_t>(npc + 1);
+    auto *vb = c.template take<uint64_t>(npc + 1);
+    const size_t sb = scan_workspace((size_t)npc + 1);
     void *sw = c.template take<char>(sb);
-    if (w) *w = SdWork{X, en, ne, va, st, eb, vb, sw, sb};
+    if (w) *w = SdWork{X8, X, en, pe, ne, va, st, eb, vb, sw, sb};
 }
 
 }  // namespace gchi
@@ -287,30 +313,31 @@ extern "C" int gchi_shard_decode(const uint8_t *d_adj, uint64_t nbytes, uint32_t
     Carver c(d_work, work_bytes);
     SdWork w;
     sd_layout(c, nbytes, &w);
-    const uint32_t nch = (uint32_t)((nbytes + kSdChunk - 1) / kSdChunk);
+    const uint32_t nch = (uint32_t)((nbytes + kSdChunk - 1) / kSdChunk), npc = nch * kSdPieces;
     launch_fill_u32(w.entry, kSdNone, nch, s);
     launch_fill_u32(w.status, 0u, 2, s);
-    hipLaunchKernelGGL(sd_exits, dim3(nch), dim3(kSdThreads), 0, s, d_adj, nbytes, w.X);
+    hipLaunchKernelGGL(sd_exits, dim3(nch), dim3(kSdThreads), 0, s, d_adj, nbytes, w.X8, w.X);
     hipLaunchKernelGGL(sd_chain, dim3(1), dim3(64), 0, s, w.X, nbytes, nch, w.entry, w.status);
-    hipLaunchKernelGGL(sd_count, dim3(nch), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.entry, w.nedge, w.vadv,
+    hipLaunchKernelGGL(sd_pieces, dim3((nch + 255) / 256), dim3(256), 0, s, w.X8, nbytes, nch, w.entry, w.pentry);
+    hipLaunchKernelGGL(sd_count, dim3(npc), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.pentry, w.nedge, w.vadv,
                        w.status);
     GCHI_LAUNCH_CHECK();
-    GCHI_TRY(exclusive_scan_u32_to_u64(w.nedge, w.ebase, nch, w.scan, w.scan_bytes, s));
-    GCHI_TRY(exclusive_scan_u32_to_u64(w.vadv, w.vbase, nch, w.scan, w.scan_bytes, s));
+    GCHI_TRY(exclusive_scan_u32_to_u64(w.nedge, w.ebase, npc, w.scan, w.scan_bytes, s));
+    GCHI_TRY(exclusive_scan_u32_to_u64(w.vadv, w.vbase, npc, w.scan, w.scan_bytes, s));
     uint32_t st[2] = {0, 0};
     uint64_t tot[2] = {0, 0};
     GCHI_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
-    GCHI_HIP(hipMemcpyAsync(&tot[0], w.ebase + nch, 8, hipMemcpyDeviceToHost, s));
-    GCHI_HIP(hipMemcpyAsync(&tot[1], w.vbase + nch, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipMemcpyAsync(&tot[0], w.ebase + npc, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipMemcpyAsync(&tot[1], w.vbase + npc, 8, hipMemcpyDeviceToHost, s));
     GCHI_HIP(hipStreamSynchronize(s));
     GCHI_CHECK(st[0] == 0, GCHI_ERR_FORMAT, "shard_decode: truncated stream (a record runs past its end)");
     GCHI_CHECK(st[1] == 0 && tot[1] <= 0x100000000ull, GCHI_ERR_OVERFLOW,
-               "shard_decode: vertex ids or a chunk's edge count exceed 32 bits");
+               "shard_decode: vertex ids or a piece's edge count exceed 32 bits");
     *h_nedges = tot[0];
     if (h_vend) *h_vend = tot[1];
     if (!d_src) return GCHI_OK;
     GCHI_CHECK(tot[0] <= max_edges, GCHI_ERR_INVALID, "shard_decode: the stream holds more than max_edges edges");
-    hipLaunchKernelGGL(sd_emit, dim3(nch), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.entry, w.ebase, w.vbase,
+    hipLaunchKernelGGL(sd_emit, dim3(npc), dim3(kSdCopyThreads), 0, s, d_adj, nbytes, w.pentry, w.ebase, w.vbase,
                        d_src, d_dst);
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;

@@ -462,12 +462,11 @@ def shard_decode(d_adj, nbytes):
     check(L.gchi_shard_decode_workspace(nbytes, ctypes.byref(nb)))
     ws = workspace(nb.value)
     ne, vend = c_u64(), c_u64()
-    check(L.gchi_shard_decode(dptr(d_adj), nbytes, None, None, 0, ctypes.byref(ne), ctypes.byref(vend), dptr(ws),
-                              nb.value, stream()), "shard_decode(count)")
-    src, dst = empty_u32(ne.value), empty_u32(ne.value)
-    check(L.gchi_shard_decode(dptr(d_adj), nbytes, dptr(src), dptr(dst), ne.value, ctypes.byref(ne),
-                              ctypes.byref(vend), dptr(ws), nb.value, stream()), "shard_decode")
-    return src, dst, vend.value
+    cap = nbytes // 4  # every edge takes 4 bytes of the stream
+    src, dst = empty_u32(cap), empty_u32(cap)
+    check(L.gchi_shard_decode(dptr(d_adj), nbytes, dptr(src), dptr(dst), cap, ctypes.byref(ne), ctypes.byref(vend),
+                              dptr(ws), nb.value, stream()), "shard_decode")
+    return src[:ne.value], dst[:ne.value], vend.value
 
 
 # ---------------------------------------------------------------- triangle counting

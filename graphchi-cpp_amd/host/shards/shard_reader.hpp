@@ -94,22 +94,16 @@ inline uint64_t load_shard_edges_device(const std::string &base, int nshards, de
         ws_max = ws > ws_max ? ws : ws_max;
     }
     device::workspace w(ws_max);
+    uint64_t cap = 0;  // every edge takes 4 bytes of its stream
+    for (uint64_t b : nbytes) cap += b / 4;
+    src.resize(cap);
+    dst.resize(cap);
     uint64_t total = 0;
     for (size_t p = 0; p < ends.size(); p++) {
-        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], nullptr, nullptr, 0, &count[p], nullptr, w.get(),
-                                        ws_max, nullptr),
+        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], src.get() + total, dst.get() + total,
+                                        cap - total, &count[p], nullptr, w.get(), ws_max, nullptr),
                       filename_shard_adj(base, (int)p, nshards).c_str());
         total += count[p];
-    }
-    src.resize(total);
-    dst.resize(total);
-    uint64_t at = 0;
-    for (size_t p = 0; p < ends.size(); p++) {
-        uint64_t got = 0;
-        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], src.get() + at, dst.get() + at, count[p], &got,
-                                        nullptr, w.get(), ws_max, nullptr),
-                      filename_shard_adj(base, (int)p, nshards).c_str());
-        at += got;
     }
     return total;
 }
