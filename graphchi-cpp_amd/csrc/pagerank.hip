@@ -722,6 +722,14 @@ __global__ void pr_gather_layout_kernel(const uint32_t *__restrict__ order, uint
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += stride) xg[i] = x[order[i]];
 }
 
+// A hub row's in-edge sum: its 512-edge chunk partials added one after another
+// in chunk order (every launch form and both finalize forms use this order).
+GCHI_DEV float hub_chunk_sum_seq(const float *__restrict__ chunk_part, uint32_t c0, uint32_t c1) {
+    float s = 0.0f;
+    for (uint32_t k = c0; k < c1; k++) s += chunk_part[k];
+    return s;
+}
+
 // Single-graph finalize: hub rows (sum of chunk partials, apply) + the
 // deterministic reduction of per-block statistics; the last workgroup writes
 // d_stats and resets the counters for the next sweep / graph replay.
@@ -733,18 +741,12 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     __shared__ double s_red[2 * (kFinThreads / kWave)];
     __shared__ bool s_last;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    const uint32_t nwaves_total = gridDim.x * (kFinThreads / kWave);
-    const uint32_t gw = blockIdx.x * (kFinThreads / kWave) + wave;
     double res = 0.0, sum = 0.0;
-    for (uint32_t h = gw; h < nhub; h += nwaves_total) {
-        uint32_t c0 = hub_chunk_off[h], c1 = hub_chunk_off[h + 1];
-        float s = 0.0f;
-        for (uint32_t k = c0 + lane; k < c1; k += kWave) s += chunk_part[k];
-        s = wave_sum(s);
-        if (lane == 0) pr_finish_row(hub_v[h], 0, s, c, r_in, outdeg, r_out, x_out, res, sum);
-    }
     const uint32_t gt = blockIdx.x * kFinThreads + tid;
     const uint32_t nthreads_total = gridDim.x * kFinThreads;
+    for (uint32_t h = gt; h < nhub; h += nthreads_total)  // one lane per hub row
+        pr_finish_row(hub_v[h], 0, hub_chunk_sum_seq(chunk_part, hub_chunk_off[h], hub_chunk_off[h + 1]), c, r_in,
+                      outdeg, r_out, x_out, res, sum);
     for (uint32_t i = gt; i < nblk; i += nthreads_total) {
         res += blk_stat[2 * i];
         sum += blk_stat[2 * i + 1];
@@ -797,25 +799,19 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     }
 }
 
-// Multi-graph finalize, step 1: hub rows (wave per hub), per-hub statistics.
+// Multi-graph finalize, step 1: hub rows (lane per hub), per-hub statistics.
 __global__ __launch_bounds__(kFinThreads) void pr_hub_apply_kernel(
     const uint32_t *__restrict__ hub_v, const uint32_t *__restrict__ hub_graph,
     const uint32_t *__restrict__ hub_chunk_off, const float *__restrict__ chunk_part,
     const uint32_t *__restrict__ outdeg, const float *__restrict__ r_in, float *__restrict__ r_out,
     float *__restrict__ x_out, uint32_t nhub, double *hub_stat, PrConsts c) {
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t nw = gridDim.x * (kFinThreads / kWave);
-    for (uint32_t h = blockIdx.x * (kFinThreads / kWave) + wave; h < nhub; h += nw) {
-        uint32_t c0 = hub_chunk_off[h], c1 = hub_chunk_off[h + 1];
-        float s = 0.0f;
-        for (uint32_t k = c0 + lane; k < c1; k += kWave) s += chunk_part[k];
-        s = wave_sum(s);
-        if (lane == 0) {
-            double res = 0.0, sum = 0.0;
-            pr_finish_row(hub_v[h], hub_graph[h], s, c, r_in, outdeg, r_out, x_out, res, sum);
-            hub_stat[2 * h] = res;
-            hub_stat[2 * h + 1] = sum;
-        }
+    const uint32_t stride = gridDim.x * kFinThreads;
+    for (uint32_t h = blockIdx.x * kFinThreads + threadIdx.x; h < nhub; h += stride) {  // one lane per hub row
+        double res = 0.0, sum = 0.0;
+        pr_finish_row(hub_v[h], hub_graph[h], hub_chunk_sum_seq(chunk_part, hub_chunk_off[h], hub_chunk_off[h + 1]),
+                      c, r_in, outdeg, r_out, x_out, res, sum);
+        hub_stat[2 * h] = res;
+        hub_stat[2 * h + 1] = sum;
     }
 }
 
@@ -1246,8 +1242,9 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
     p->e_hub = e_hub;
     p->nchunk = nchunk;
     GCHI_CHECK(e_reg + e_hub == E, GCHI_ERR_INVALID, "pr_plan_create: in-edge offsets do not add up to nedges");
-    // one wavefront per hub row; at least 64 workgroups for the statistics
-    p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(64, (nhub + kFinThreads / kWave - 1) / (kFinThreads / kWave)));
+    // one lane per hub row; at least 64 workgroups for the statistics.  Each
+    // workgroup takes a ticket on one counter at its end, so fewer is better.
+    p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(64, (nhub + kFinThreads - 1) / kFinThreads));
 
     // gather layout: sources ranked by out-degree (descending, ties by id)
     uint32_t *grank = nullptr, *gorder_tmp = nullptr, *bad = tp.alloc<uint32_t>(1);
@@ -1505,7 +1502,7 @@ int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, con
                            p->counters, d_stats, c);
     } else {
         if (p->nhub)
-            hipLaunchKernelGGL(pr_hub_apply_kernel, dim3(grid_for((uint64_t)p->nhub * kWave, kFinThreads, 1024)),
+            hipLaunchKernelGGL(pr_hub_apply_kernel, dim3(grid_for((uint64_t)p->nhub, kFinThreads, 1024)),
                                dim3(kFinThreads), 0, s, p->hub_v, p->hub_graph, p->hub_chunk_off, p->chunk_part,
                                p->outdeg, r_in, r_out, x_out, p->nhub, p->hub_stat, c);
         hipLaunchKernelGGL(pr_graph_reduce_kernel, dim3(std::min<uint32_t>(p->G, 4096)), dim3(kFinThreads), 0, s,
