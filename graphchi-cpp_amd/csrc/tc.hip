@@ -224,12 +224,12 @@ __global__ __launch_bounds__(kTcWaves *kWave) void tc_count_kernel(const uint32_
 // graph with up to ~500K edges, and nearly all of larger ones).  Vertices are
 // taken from the highest rank down: the work per vertex grows with its rank,
 // so the longest ones start first.  One wavefront per vertex a puts N+(a) into
-// an LDS hash table (open addressing, linear
-// probing, at most half full, a quarter when |N+(a)| <= kTcHashSlots / 4), then
-// walks b in N+(a) one at a time and looks up every w in N+(b) lane-wide, four
-// w per lane in flight.  A hit is the triangle a < b < w.  Counts: w's and b's
-// credits go to their slots (16-bit halves of LDS words: at most 2|N+(a)| per
-// slot) and are added to the global counts once per vertex a.
+// an LDS hash table (open addressing, linear probing, at most half full, a
+// quarter when |N+(a)| <= kTcHashSlots / 4), then walks b in N+(a) one at a
+// time and looks up every w in N+(b) lane-wide, kTcInflight w per lane in
+// flight.  A hit is the triangle a < b < w.  Counts: w's and b's credits go to
+// their slots (16-bit halves of LDS words: at most 2|N+(a)| per slot) and are
+// added to the global counts once per vertex a.
 constexpr int kTcHashWaves = 4;
 constexpr uint32_t kTcHashSlots = 2048;
 constexpr uint32_t kTcHashMax = kTcHashSlots / 2;
@@ -239,6 +239,7 @@ constexpr uint32_t kTcEmpty = 0xffffffffu;
 // address complete one after another (about 10 ns each here), so a single
 // counter would cap the kernel at ~10^8 vertices per second.
 constexpr uint32_t kTcStripes = 32;
+constexpr int kTcInflight = 8;  // w loads per lane in flight (one memory round trip per 512 w)
 constexpr uint32_t kTcStripeGap = 32;
 
 GCHI_DEV uint32_t tc_slot(uint32_t w, int sh) { return (w * 2654435761u) >> sh; }
@@ -311,15 +312,15 @@ __global__ __launch_bounds__(kTcHashWaves *kWave) void tc_count_hash_kernel(cons
                 const uint32_t lbj = (uint32_t)__builtin_amdgcn_readlane((int)lb, (int)j);
                 const uint32_t *B = adj + bbj;
                 uint32_t hits = 0;
-                for (uint32_t t = lane; t < lbj; t += 4 * kWave) {
-                    uint32_t w[4];
+                for (uint32_t t = lane; t < lbj; t += kTcInflight * kWave) {
+                    uint32_t w[kTcInflight];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
+                    for (int q = 0; q < kTcInflight; q++) {
                         const uint32_t tq = t + q * kWave;
                         w[q] = tq < lbj ? B[tq] : kTcEmpty;
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
+                    for (int q = 0; q < kTcInflight; q++) {
                         if (w[q] == kTcEmpty) continue;
                         const int h = tc_find(keys, mask, sh, w[q]);
                         if (h >= 0) {
