@@ -722,12 +722,48 @@ __global__ void pr_gather_layout_kernel(const uint32_t *__restrict__ order, uint
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += stride) xg[i] = x[order[i]];
 }
 
-// A hub row's in-edge sum: its 512-edge chunk partials added one after another
-// in chunk order (every launch form and both finalize forms use this order).
-GCHI_DEV float hub_chunk_sum_seq(const float *__restrict__ chunk_part, uint32_t c0, uint32_t c1) {
-    float s = 0.0f;
-    for (uint32_t k = c0; k < c1; k++) s += chunk_part[k];
-    return s;
+// Hub rows' in-edge sums: each row's 512-edge chunk partials added one after
+// another in chunk order (every launch form and both finalize forms use this
+// order).  Lane slot i of the finalize handles hub hub_of_slot(i): a bijection
+// of [0, nhub) (multiplication by a prime above nhub, mod nhub) that scatters
+// the long rows -- R-MAT's hubs have the lowest ids, so consecutive hubs are
+// the longest -- over many wavefronts.  Rows of up to kHubLaneChunks chunks
+// are summed by their own lane; longer ones (a row of 10^5 in-edges has
+// hundreds of chunks) by the whole wavefront, whose lanes load 64 partials at
+// a time while every lane adds them in order: the same float additions in the
+// same order.  Call hub_sums_wave with all lanes of the wavefront.
+constexpr uint32_t kHubLaneChunks = 16;
+constexpr uint64_t kHubSlotPrime = 4294967291ull;  // largest prime below 2^32
+
+GCHI_DEV uint32_t hub_of_slot(uint32_t i, uint32_t nhub) { return (uint32_t)(((uint64_t)i * kHubSlotPrime) % nhub); }
+
+GCHI_DEV float hub_sums_wave(const float *__restrict__ chunk_part, const uint32_t *__restrict__ hub_chunk_off,
+                             uint32_t h, bool valid, int lane) {
+    uint32_t c0 = 0, c1 = 0;
+    if (valid) {
+        c0 = hub_chunk_off[h];
+        c1 = hub_chunk_off[h + 1];
+    }
+    const bool big = c1 - c0 > kHubLaneChunks;
+    float mine = 0.0f;
+    if (!big)
+        for (uint32_t k = c0; k < c1; k++) mine += chunk_part[k];
+    uint64_t m = __ballot(big);
+    while (m) {
+        const int j = __builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)c0, j);
+        const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)c1, j);
+        float s = 0.0f;
+        for (uint32_t k0 = b0; k0 < b1; k0 += kWave) {
+            const uint32_t k = k0 + lane;
+            const float v = k < b1 ? chunk_part[k] : 0.0f;
+            const uint32_t cnt = min((uint32_t)kWave, b1 - k0);
+            for (uint32_t q = 0; q < cnt; q++) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)q));
+        }
+        if (lane == j) mine = s;
+    }
+    return mine;
 }
 
 // Single-graph finalize: hub rows (sum of chunk partials, apply) + the
@@ -744,9 +780,12 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     double res = 0.0, sum = 0.0;
     const uint32_t gt = blockIdx.x * kFinThreads + tid;
     const uint32_t nthreads_total = gridDim.x * kFinThreads;
-    for (uint32_t h = gt; h < nhub; h += nthreads_total)  // one lane per hub row
-        pr_finish_row(hub_v[h], 0, hub_chunk_sum_seq(chunk_part, hub_chunk_off[h], hub_chunk_off[h + 1]), c, r_in,
-                      outdeg, r_out, x_out, res, sum);
+    for (uint32_t base = gt - lane; base < nhub; base += nthreads_total) {  // a lane per hub row
+        const bool valid = base + lane < nhub;
+        const uint32_t h = valid ? hub_of_slot(base + lane, nhub) : 0u;
+        const float hs = hub_sums_wave(chunk_part, hub_chunk_off, h, valid, lane);
+        if (valid) pr_finish_row(hub_v[h], 0, hs, c, r_in, outdeg, r_out, x_out, res, sum);
+    }
     for (uint32_t i = gt; i < nblk; i += nthreads_total) {
         res += blk_stat[2 * i];
         sum += blk_stat[2 * i + 1];
@@ -805,13 +844,19 @@ __global__ __launch_bounds__(kFinThreads) void pr_hub_apply_kernel(
     const uint32_t *__restrict__ hub_chunk_off, const float *__restrict__ chunk_part,
     const uint32_t *__restrict__ outdeg, const float *__restrict__ r_in, float *__restrict__ r_out,
     float *__restrict__ x_out, uint32_t nhub, double *hub_stat, PrConsts c) {
+    const int lane = threadIdx.x & (kWave - 1);
     const uint32_t stride = gridDim.x * kFinThreads;
-    for (uint32_t h = blockIdx.x * kFinThreads + threadIdx.x; h < nhub; h += stride) {  // one lane per hub row
-        double res = 0.0, sum = 0.0;
-        pr_finish_row(hub_v[h], hub_graph[h], hub_chunk_sum_seq(chunk_part, hub_chunk_off[h], hub_chunk_off[h + 1]),
-                      c, r_in, outdeg, r_out, x_out, res, sum);
-        hub_stat[2 * h] = res;
-        hub_stat[2 * h + 1] = sum;
+    // a lane per hub row
+    for (uint32_t base = blockIdx.x * kFinThreads + threadIdx.x - lane; base < nhub; base += stride) {
+        const bool valid = base + lane < nhub;
+        const uint32_t h = valid ? hub_of_slot(base + lane, nhub) : 0u;
+        const float hs = hub_sums_wave(chunk_part, hub_chunk_off, h, valid, lane);
+        if (valid) {
+            double res = 0.0, sum = 0.0;
+            pr_finish_row(hub_v[h], hub_graph[h], hs, c, r_in, outdeg, r_out, x_out, res, sum);
+            hub_stat[2 * h] = res;
+            hub_stat[2 * h + 1] = sum;
+        }
     }
 }
 
