@@ -126,11 +126,12 @@ def bench_load(gchi, torch, src, dst, E, N, tune):
     streams = [sh.encode(p)[0] for p in range(sh.nshards)]
     sh.close()
     d_streams = [torch.from_numpy(np.frombuffer(b, np.uint8).copy()).to("cuda") for b in streams]
-    gchi.shard_decode(d_streams[0], len(streams[0]))
-    outs, t_dec = timed(lambda: [gchi.shard_decode(d, len(b)) for d, b in zip(d_streams, streams)], torch)
+    pairs = [(d, len(b)) for d, b in zip(d_streams, streams)]
+    gchi.shard_decode_many(pairs)
+    out, t_dec = timed(lambda: gchi.shard_decode_many(pairs), torch)
     nbytes = sum(len(b) for b in streams)
-    assert sum(int(o[0].numel()) for o in outs) == E
-    del outs, d_streams
+    assert int(out[0].numel()) == E
+    del out, d_streams, pairs
     return {"upload_ms": round(t_up * 1e3, 2), "upload_GBps": round(8 * E / t_up / 1e9, 1),
             "csc_build_ms": round(t_csc * 1e3, 2), "plan_build_ms": round(t_plan * 1e3, 2),
             "shard_decode_ms": round(t_dec * 1e3, 2), "shard_decode_streams": len(streams),

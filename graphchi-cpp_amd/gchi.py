@@ -105,6 +105,8 @@ _SIGS = {
     "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_scheduled_workspace": (c_i32, [c_u32, c_u64, ctypes.POINTER(c_sz)]),
     "gchi_shard_decode_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_shard_decode_many_workspace": (c_i32, [c_vp, c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_shard_decode_many": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "gchi_shard_decode": (c_i32, [c_vp, c_u64, c_vp, c_vp, c_u64, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), c_vp,
                                   c_sz, c_vp]),
     "gchi_cc_scheduled": (c_i32, [c_vp, c_vp, c_u32, c_u32, c_vp, ctypes.POINTER(c_u32), ctypes.POINTER(c_u64), c_vp,
@@ -467,6 +469,26 @@ def shard_decode(d_adj, nbytes):
     check(L.gchi_shard_decode(dptr(d_adj), nbytes, dptr(src), dptr(dst), cap, ctypes.byref(ne), ctypes.byref(vend),
                               dptr(ws), nb.value, stream()), "shard_decode")
     return src[:ne.value], dst[:ne.value], vend.value
+
+
+def shard_decode_many(streams):
+    """Device decode of several .adj streams [(uint8 device tensor, nbytes), ...]
+    in one call; returns (src, dst, per-stream edge counts, per-stream vend)."""
+    L = lib()
+    k = len(streams)
+    nb = np.array([int(n) for _, n in streams], np.uint64)
+    ptrs = np.array([t.data_ptr() if n else 0 for t, n in streams], np.uint64)
+    wsz = c_sz()
+    check(L.gchi_shard_decode_many_workspace(c_vp(nb.ctypes.data), k, ctypes.byref(wsz)))
+    ws = workspace(wsz.value)
+    cap = int(sum(int(n) // 4 for n in nb))
+    src, dst = empty_u32(cap), empty_u32(cap)
+    ne, vend = np.zeros(max(k, 1), np.uint64), np.zeros(max(k, 1), np.uint64)
+    check(L.gchi_shard_decode_many(c_vp(ptrs.ctypes.data), c_vp(nb.ctypes.data), k, dptr(src), dptr(dst), cap,
+                                   c_vp(ne.ctypes.data), c_vp(vend.ctypes.data), dptr(ws), wsz.value, stream()),
+          "shard_decode_many")
+    tot = int(ne[:k].sum())
+    return src[:tot], dst[:tot], ne[:k].astype(np.int64), vend[:k].astype(np.int64)
 
 
 # ---------------------------------------------------------------- triangle counting

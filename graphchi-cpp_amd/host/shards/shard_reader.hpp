@@ -75,7 +75,8 @@ inline void load_shard_edges(const std::string &base, int nshards, std::vector<v
 }
 
 // The device load path: every shard's .adj bytes are uploaded once and decoded
-// in HBM (gchi_shard_decode), and the edges stay there for the device build.
+// in HBM, all shards in one call (gchi_shard_decode_many), and the edges stay
+// there for the device build.
 // Same edge sequence as load_shard_edges.  Returns the edge count.
 inline uint64_t load_shard_edges_device(const std::string &base, int nshards, device::buffer<uint32_t> &src,
                                         device::buffer<uint32_t> &dst) {
@@ -83,28 +84,27 @@ inline uint64_t load_shard_edges_device(const std::string &base, int nshards, de
     std::vector<vid_t> ends = load_interval_ends(base, nshards);
     std::vector<device::buffer<uint8_t>> bytes(ends.size());
     std::vector<uint64_t> nbytes(ends.size()), count(ends.size());
-    size_t ws_max = 0;
     for (size_t p = 0; p < ends.size(); p++) {
         std::vector<uint8_t> adj = read_file_bytes(filename_shard_adj(base, (int)p, nshards));
         nbytes[p] = adj.size();
         bytes[p].resize(adj.size());
         if (!adj.empty()) bytes[p].upload(adj.data(), adj.size());
-        size_t ws = 0;
-        device::check(gchi_shard_decode_workspace(nbytes[p], &ws), "shard decode workspace");
-        ws_max = ws > ws_max ? ws : ws_max;
     }
-    device::workspace w(ws_max);
+    std::vector<const uint8_t *> ptrs(ends.size());
+    for (size_t p = 0; p < ends.size(); p++) ptrs[p] = bytes[p].get();
+    size_t ws = 0;
+    device::check(gchi_shard_decode_many_workspace(nbytes.data(), (uint32_t)ends.size(), &ws),
+                  "shard decode workspace");
+    device::workspace w(ws);
     uint64_t cap = 0;  // every edge takes 4 bytes of its stream
     for (uint64_t b : nbytes) cap += b / 4;
     src.resize(cap);
     dst.resize(cap);
+    device::check(gchi_shard_decode_many(ptrs.data(), nbytes.data(), (uint32_t)ends.size(), src.get(), dst.get(), cap,
+                                         count.data(), nullptr, w.get(), ws, nullptr),
+                  ("decoding the shards of " + base).c_str());
     uint64_t total = 0;
-    for (size_t p = 0; p < ends.size(); p++) {
-        device::check(gchi_shard_decode(bytes[p].get(), nbytes[p], src.get() + total, dst.get() + total,
-                                        cap - total, &count[p], nullptr, w.get(), ws_max, nullptr),
-                      filename_shard_adj(base, (int)p, nshards).c_str());
-        total += count[p];
-    }
+    for (uint64_t c : count) total += c;
     return total;
 }
 

@@ -383,6 +383,14 @@ int gchi_shard_write_files(const gchi_shards *shards, const char *basefilename, 
 int gchi_shard_decode_workspace(uint64_t nbytes, size_t *bytes);
 int gchi_shard_decode(const uint8_t *d_adj, uint64_t nbytes, uint32_t *d_src, uint32_t *d_dst, uint64_t max_edges,
                       uint64_t *h_nedges, uint64_t *h_vend, void *d_work, size_t work_bytes, void *stream);
+/* Several streams (a graph's shards) in one call: h_adj[i] (device pointers, in
+ * a host array) of h_nbytes[i] bytes.  The edges land back to back in stream
+ * order in d_src/d_dst, each stream's src ids counting from 0 as in its file;
+ * h_nedges[i] / h_vend[i] per stream. */
+int gchi_shard_decode_many_workspace(const uint64_t *h_nbytes, uint32_t nstreams, size_t *bytes);
+int gchi_shard_decode_many(const uint8_t *const *h_adj, const uint64_t *h_nbytes, uint32_t nstreams, uint32_t *d_src,
+                           uint32_t *d_dst, uint64_t max_edges, uint64_t *h_nedges, uint64_t *h_vend, void *d_work,
+                           size_t work_bytes, void *stream);
 
 #ifdef __cplusplus
 }

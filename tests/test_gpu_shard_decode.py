@@ -95,3 +95,29 @@ def test_decode_small_and_malformed(dev):
     for bad in (b"\x00", b"\xff\x01\x00", b"\x02\x01\x00\x00\x00", b"\x01\x00\x00\x00"):
         with pytest.raises(dev.GchiError):
             dev.shard_decode(to_dev_u8(dev, bad), len(bad))
+
+
+def test_decode_many_streams_in_one_call(dev):
+    """All shards of a scale-18 graph, plus an empty stream and a golden hand
+    stream, decoded in one call: the edges land back to back in stream order,
+    each stream's src ids counting from 0."""
+    src, dst = graphs.rmat_pairs(18, 16, 5)
+    src, dst = graphs.preprocess(src, dst)
+    maxid = int(max(src.max(), dst.max()))
+    sh = dev.Shards(dev.to_dev_u32(src), dev.to_dev_u32(dst), maxid, 4)
+    streams = [sh.encode(p)[0] for p in range(sh.nshards)]
+    sh.close()
+    streams.insert(2, b"")
+    streams.append(bytes.fromhex("00040200000000070000000101000000"))
+    d = [(to_dev_u8(dev, b), len(b)) for b in streams]
+    gs, gd, counts, vends = dev.shard_decode_many(d)
+    want = [oracle.shard_decode(b) for b in streams]
+    assert counts.tolist() == [len(w[0]) for w in want]
+    tot = int(counts.sum())
+    np.testing.assert_array_equal(dev.to_host_u32(gs, tot), np.concatenate([w[0] for w in want]))
+    np.testing.assert_array_equal(dev.to_host_u32(gd, tot), np.concatenate([w[1] for w in want]))
+    assert vends.tolist() == [int(w[0][-1]) + 1 if len(w[0]) else 0 for w in want]
+    # one truncated stream among good ones fails the call
+    bad = d[:2] + [(to_dev_u8(dev, streams[0][:-1]), len(streams[0]) - 1)]
+    with pytest.raises(dev.GchiError):
+        dev.shard_decode_many(bad)
