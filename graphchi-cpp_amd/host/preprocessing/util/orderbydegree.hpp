@@ -1,9 +1,9 @@
 // order_by_degree (src/preprocessing/util/orderbydegree.hpp:58-143): ranks the
 // vertices of a sharded graph by (in+out degree, id), writes the translation
 // table <base>.vertexmap (vertexmap[old] = new id), relabels every edge and
-// shards the result as <base>_degord with the same number of shards.  Degree
-// ranking and relabelling run on the device (gchi_degree_order,
-// gchi_relabel_edges), then the device sharder.
+// shards the result as <base>_degord with the same number of shards.  The
+// shards are decoded, ranked and relabelled on the device (gchi_shard_decode_many,
+// gchi_degree_order, gchi_relabel_edges), then sharded by the device sharder.
 #pragma once
 #include <cstdio>
 #include <string>
@@ -21,14 +21,9 @@ int order_by_degree(std::string &base_filename, int nshards, metrics &m) {
     metrics_entry me = m.start_time();
     device::open();
     const size_t n = get_num_vertices(base_filename);
-    std::vector<vid_t> src, dst;
-    load_shard_edges(base_filename, nshards, src, dst);
-    const uint64_t e = src.size();
-    device::buffer<uint32_t> s(e), d(e), s2(e), d2(e), vmap(n);
-    if (e) {
-        s.upload(src.data(), e);
-        d.upload(dst.data(), e);
-    }
+    device::buffer<uint32_t> s, d;
+    const uint64_t e = load_shard_edges_device(base_filename, nshards, s, d);
+    device::buffer<uint32_t> s2(e), d2(e), vmap(n);
     size_t ws = 0;
     device::check(gchi_degree_order_workspace(e, (uint32_t)n, &ws), "degree order workspace");
     {
