@@ -728,11 +728,13 @@ __global__ void pr_gather_layout_kernel(const uint32_t *__restrict__ order, uint
 // of [0, nhub) (multiplication by a prime above nhub, mod nhub) that scatters
 // the long rows -- R-MAT's hubs have the lowest ids, so consecutive hubs are
 // the longest -- over many wavefronts.  Rows of up to kHubLaneChunks chunks
-// are summed by their own lane; longer ones (a row of 10^5 in-edges has
-// hundreds of chunks) by the whole wavefront, whose lanes load 64 partials at
-// a time while every lane adds them in order: the same float additions in the
-// same order.  Call hub_sums_wave with all lanes of the wavefront.
-constexpr uint32_t kHubLaneChunks = 16;
+// are summed by their own lane, 16 partial loads in flight; longer ones (a
+// row of 10^5 in-edges has hundreds of chunks) by the whole wavefront, whose
+// lanes load 64 partials at a time while every lane adds them in order: the
+// same float additions in the same order.  Call hub_sums_wave with all lanes
+// of the wavefront.
+constexpr uint32_t kHubLaneChunks = 32;
+constexpr uint32_t kHubLaneBatch = 16;
 constexpr uint64_t kHubSlotPrime = 4294967291ull;  // largest prime below 2^32
 
 GCHI_DEV uint32_t hub_of_slot(uint32_t i, uint32_t nhub) { return (uint32_t)(((uint64_t)i * kHubSlotPrime) % nhub); }
@@ -747,7 +749,14 @@ GCHI_DEV float hub_sums_wave(const float *__restrict__ chunk_part, const uint32_
     const bool big = c1 - c0 > kHubLaneChunks;
     float mine = 0.0f;
     if (!big)
-        for (uint32_t k = c0; k < c1; k++) mine += chunk_part[k];
+        for (uint32_t k0 = c0; k0 < c1; k0 += kHubLaneBatch) {  // loads in flight, then the adds in order
+            float v[kHubLaneBatch];
+#pragma unroll
+            for (uint32_t q = 0; q < kHubLaneBatch; q++) v[q] = k0 + q < c1 ? chunk_part[k0 + q] : 0.0f;
+#pragma unroll
+            for (uint32_t q = 0; q < kHubLaneBatch; q++)
+                if (k0 + q < c1) mine += v[q];
+        }
     uint64_t m = __ballot(big);
     while (m) {
         const int j = __builtin_ctzll(m);
@@ -780,9 +789,13 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     double res = 0.0, sum = 0.0;
     const uint32_t gt = blockIdx.x * kFinThreads + tid;
     const uint32_t nthreads_total = gridDim.x * kFinThreads;
-    for (uint32_t base = gt - lane; base < nhub; base += nthreads_total) {  // a lane per hub row
-        const bool valid = base + lane < nhub;
-        const uint32_t h = valid ? hub_of_slot(base + lane, nhub) : 0u;
+    // a lane per hub row: slot k*T + lane*(T/64) + w for wavefront w, so that
+    // even a few hundred hubs spread over all wavefronts
+    const uint32_t nwv = nthreads_total / kWave, wv = gt / kWave;
+    for (uint32_t base = wv; base < nhub; base += nthreads_total) {
+        const uint32_t slot = base + lane * nwv;
+        const bool valid = slot < nhub;
+        const uint32_t h = valid ? hub_of_slot(slot, nhub) : 0u;
         const float hs = hub_sums_wave(chunk_part, hub_chunk_off, h, valid, lane);
         if (valid) pr_finish_row(hub_v[h], 0, hs, c, r_in, outdeg, r_out, x_out, res, sum);
     }
@@ -846,10 +859,12 @@ __global__ __launch_bounds__(kFinThreads) void pr_hub_apply_kernel(
     float *__restrict__ x_out, uint32_t nhub, double *hub_stat, PrConsts c) {
     const int lane = threadIdx.x & (kWave - 1);
     const uint32_t stride = gridDim.x * kFinThreads;
-    // a lane per hub row
-    for (uint32_t base = blockIdx.x * kFinThreads + threadIdx.x - lane; base < nhub; base += stride) {
-        const bool valid = base + lane < nhub;
-        const uint32_t h = valid ? hub_of_slot(base + lane, nhub) : 0u;
+    // a lane per hub row, slots spread over all wavefronts as in the single-graph finalize
+    const uint32_t nwv = stride / kWave, wv = (blockIdx.x * kFinThreads + threadIdx.x) / kWave;
+    for (uint32_t base = wv; base < nhub; base += stride) {
+        const uint32_t slot = base + lane * nwv;
+        const bool valid = slot < nhub;
+        const uint32_t h = valid ? hub_of_slot(slot, nhub) : 0u;
         const float hs = hub_sums_wave(chunk_part, hub_chunk_off, h, valid, lane);
         if (valid) {
             double res = 0.0, sum = 0.0;
