@@ -8,9 +8,10 @@
  * Streams are hipStream_t passed as void* (NULL == default stream).
  *
  * Workspace convention: entry points that need scratch device memory take
- * (d_work, work_bytes); the matching *_workspace() call reports the size.  The
- * library never allocates device memory inside a compute call, so every
- * compute call can be captured into a HIP graph.
+ * (d_work, work_bytes); the matching *_workspace() call reports the size.
+ * Asynchronous compute calls (sweeps, sorts, CC, TC counting, CSC builds)
+ * never allocate device memory, so they can be captured into a HIP graph.
+ * Calls documented as host-synchronous read results back to the host.
  *
  * Each declaration names the reference interface it replaces (paths relative
  * to the GraphChi source tree).
@@ -333,7 +334,8 @@ int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedge
                     void *d_work, size_t work_bytes, void *stream);
 /* The translation table of order_by_degree alone, and an edge relabel
  * (src' = map[src], dst' = map[dst]) -- the re-sharding input of
- * orderbydegree.hpp:112-142. */
+ * orderbydegree.hpp:112-142.  Both read back an id-range check, so they are
+ * host-synchronous; gchi_relabel_edges allocates its 4-byte check flag. */
 int gchi_degree_order_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
 int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                       uint32_t *d_vertexmap, void *d_work, size_t work_bytes, void *stream);
