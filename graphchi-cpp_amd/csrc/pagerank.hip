@@ -554,8 +554,22 @@ GCHI_DEV void chunk_step(const PrArgs &a, const float *hot, uint32_t H, int lane
     if (lane == 0) a.chunk_part[ch] = t;
 }
 
+// Issue priority from the wavefront's remaining pipeline steps.  Every
+// wavefront of the sweep gets about the same number of steps, and a SIMD issues
+// the oldest ready wavefront first, so without this the oldest of a
+// workgroup's 16 finish well before the youngest, which then run on at low
+// occupancy.  A wavefront that falls behind keeps a higher priority than one
+// ahead of it (four levels: more than 3/4, 1/2, 1/4 of its steps left).
+GCHI_DEV void wave_prio(uint32_t left, uint32_t total) {
+    if (4 * (uint64_t)left > 3 * (uint64_t)total) __builtin_amdgcn_s_setprio(3);
+    else if (2 * (uint64_t)left > total) __builtin_amdgcn_s_setprio(2);
+    else if (4 * (uint64_t)left > total) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <bool HOT>
-GCHI_DEV void wave_chunks(const PrArgs &a, const float *hot, uint32_t H, int lane, uint32_t ch0, uint32_t step) {
+GCHI_DEV void wave_chunks(const PrArgs &a, const float *hot, uint32_t H, int lane, uint32_t ch0, uint32_t step,
+                          uint32_t after, uint32_t total) {
     if (ch0 >= a.nchunk) return;
     uint32_t idx[2][kChunkPer];
     ChunkVals<HOT> cv[2];
@@ -563,7 +577,9 @@ GCHI_DEV void wave_chunks(const PrArgs &a, const float *hot, uint32_t H, int lan
     if (ch0 + step < a.nchunk) chunk_ids(a, ch0 + step, lane, idx[1]);
     chunk_gather<HOT>(a, ch0, hot, H, idx[0], cv[0]);
     uint32_t ch = ch0;
-    for (; ch + step < a.nchunk; ch += 2 * step) {
+    uint32_t left = (a.nchunk - ch0 + step - 1) / step + after;
+    for (; ch + step < a.nchunk; ch += 2 * step, left -= 2) {
+        wave_prio(left, total);
         chunk_step<HOT, 0>(a, hot, H, lane, ch, step, idx, cv);
         chunk_step<HOT, 1>(a, hot, H, lane, ch + step, step, idx, cv);
     }
@@ -674,7 +690,7 @@ GCHI_DEV void wave_step(const PrArgs &a, const PrConsts &c, const float *hot, ui
 // then the pipeline (prologue, steps in slot pairs).
 template <int PER, bool HOT>
 GCHI_DEV void wave_blocks(const PrArgs &a, const PrConsts &c, const float *hot, uint32_t H, float *st, int lane,
-                          uint32_t b0, uint32_t b1) {
+                          uint32_t b0, uint32_t b1, uint32_t bend, uint32_t total) {
     DescBatch db;
     load_desc_batch(a, c, b0, b1, lane, db);
     uint32_t idx[2][PER];
@@ -687,6 +703,7 @@ GCHI_DEV void wave_blocks(const PrArgs &a, const PrConsts &c, const float *hot, 
     gather_ids<PER, HOT>(a, t0, hot, H, lane, idx[0], gd[0]);
     uint32_t b = b0;
     for (; b + 1 < b1; b += 2) {
+        wave_prio(bend - b, total);
         wave_step<PER, HOT, 0>(a, c, hot, H, st, lane, b, b1, db, idx, gd, meta);
         wave_step<PER, HOT, 1>(a, c, hot, H, st, lane, b + 1, b1, db, idx, gd, meta);
     }
@@ -707,12 +724,13 @@ __global__ __launch_bounds__(T) void pr_sweep_wave_kernel(PrArgs a, PrConsts c, 
     __syncthreads();
     const uint32_t nwv = gridDim.x * NW;
     const uint32_t gw = blockIdx.x * NW + wave;
-    wave_chunks<HOT>(a, hot, H, lane, gw, nwv);
     const uint64_t per = ((uint64_t)a.nblk + nwv - 1) / nwv;
     const uint32_t b0 = (uint32_t)min<uint64_t>(a.nblk, per * gw);
     const uint32_t b1 = (uint32_t)min<uint64_t>(a.nblk, per * gw + per);
+    const uint32_t nch = gw < a.nchunk ? (a.nchunk - gw + nwv - 1) / nwv : 0u;
+    wave_chunks<HOT>(a, hot, H, lane, gw, nwv, b1 - b0, nch + (b1 - b0));
     for (uint32_t p = b0; p < b1; p += kWave)
-        wave_blocks<PER, HOT>(a, c, hot, H, st, lane, p, min(b1, p + (uint32_t)kWave));
+        wave_blocks<PER, HOT>(a, c, hot, H, st, lane, p, min(b1, p + (uint32_t)kWave), b1, nch + (b1 - b0));
 }
 
 // Gather-layout refresh: xg[i] = x[order[i]] for the A layout positions.
