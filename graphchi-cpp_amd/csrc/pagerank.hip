@@ -752,6 +752,7 @@ __global__ void pr_gather_layout_kernel(const uint32_t *__restrict__ order, uint
 // same float additions in the same order.  Call hub_sums_wave with all lanes
 // of the wavefront.
 constexpr uint32_t kHubLaneChunks = 32;
+constexpr uint32_t kFinStatAhead = 4;  // block statistics per finalize thread loaded up front
 constexpr uint32_t kHubLaneBatch = 16;
 constexpr uint64_t kHubSlotPrime = 4294967291ull;  // largest prime below 2^32
 
@@ -809,15 +810,42 @@ __global__ __launch_bounds__(kFinThreads) void pr_finalize_kernel(
     const uint32_t nthreads_total = gridDim.x * kFinThreads;
     // a lane per hub row: slot k*T + lane*(T/64) + w for wavefront w, so that
     // even a few hundred hubs spread over all wavefronts
+    // The first kFinStatAhead block statistics of this thread are loaded before
+    // the hub rows and added after them, as the loop below adds the rest: the
+    // order of additions is the same either way.
+    double bs[2 * kFinStatAhead];
+#pragma unroll
+    for (uint32_t q = 0; q < kFinStatAhead; q++) {
+        const uint32_t i = min(gt + q * nthreads_total, nblk ? nblk - 1 : 0u);
+        bs[2 * q] = nblk ? blk_stat[2 * i] : 0.0;
+        bs[2 * q + 1] = nblk ? blk_stat[2 * i + 1] : 0.0;
+    }
     const uint32_t nwv = nthreads_total / kWave, wv = gt / kWave;
     for (uint32_t base = wv; base < nhub; base += nthreads_total) {
         const uint32_t slot = base + lane * nwv;
         const bool valid = slot < nhub;
         const uint32_t h = valid ? hub_of_slot(slot, nhub) : 0u;
+        // the row's own inputs are loaded beside its chunk partials
+        const uint32_t v = hub_v[h];
+        const float ro = r_in[v];
+        const uint32_t od = outdeg[v];
         const float hs = hub_sums_wave(chunk_part, hub_chunk_off, h, valid, lane);
-        if (valid) pr_finish_row(hub_v[h], 0, hs, c, r_in, outdeg, r_out, x_out, res, sum);
+        if (valid) {
+            const float r = pr_apply(row_teleport(c, 0, v), c.damping, hs);
+            r_out[v] = r;
+            x_out[v] = pr_scatter(r, od);
+            res += fabs((double)r - (double)ro);
+            sum += (double)r;
+        }
     }
-    for (uint32_t i = gt; i < nblk; i += nthreads_total) {
+#pragma unroll
+    for (uint32_t q = 0; q < kFinStatAhead; q++) {
+        if (gt + q * nthreads_total < nblk) {
+            res += bs[2 * q];
+            sum += bs[2 * q + 1];
+        }
+    }
+    for (uint32_t i = gt + kFinStatAhead * nthreads_total; i < nblk; i += nthreads_total) {
         res += blk_stat[2 * i];
         sum += blk_stat[2 * i + 1];
     }
