@@ -25,12 +25,17 @@ constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / kWave;
 constexpr int kRsItems = 16;
 constexpr int kRsTile = kRsThreads * kRsItems;  // 4096 keys
+// Key-only passes use 512-thread workgroups (8192-key tiles): half as many
+// tiles, so half as many tile tickets and look-back chains per pass.
+constexpr int kRsThreadsK = 512;
+constexpr int kRsTileK = kRsThreadsK * kRsItems;
 constexpr int kRsRadix = 256;
 constexpr int kRsMaxPasses = 8;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 26;
+constexpr int kRsLookback = 16;  // predecessor tiles read per look-back round trip
 
 struct PassDesc {
     int shift[kRsMaxPasses];
@@ -79,32 +84,35 @@ __global__ void rs_hist_scan(uint32_t *ghist) {
     h[threadIdx.x] = s[threadIdx.x];
 }
 
-template <bool kVals>
-__global__ __launch_bounds__(kRsThreads) void rs_onesweep_pass(const uint64_t *__restrict__ kin,
+template <bool kVals, int T>
+__global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict__ kin,
                                                                const uint32_t *__restrict__ vin,
                                                                uint64_t *__restrict__ kout,
                                                                uint32_t *__restrict__ vout, uint64_t n, int shift,
                                                                int nbits, const uint32_t *__restrict__ gbase,
                                                                uint32_t *status, uint32_t *tile_counter,
                                                                uint32_t *err) {
-    __shared__ uint32_t s_wh[kRsWaves][kRsRadix];
+    constexpr int NW = T / kWave;
+    constexpr int TILE = T * kRsItems;
+    static_assert(T >= kRsRadix && T % kRsRadix == 0, "a thread per digit");
+    __shared__ uint32_t s_wh[NW][kRsRadix];
     __shared__ uint32_t s_toff[kRsRadix];
     __shared__ uint32_t s_gb[kRsRadix];
-    __shared__ uint64_t s_keys[kRsTile];
-    __shared__ uint32_t s_vals[kVals ? kRsTile : 1];
-    __shared__ uint32_t s_scan[kRsWaves];
+    __shared__ uint64_t s_keys[TILE];
+    __shared__ uint32_t s_vals[kVals ? TILE : 1];
+    __shared__ uint32_t s_scan[kRsRadix / kWave];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
+    const bool dig = tid < kRsRadix;  // threads 0..255 own digit tid (whole wavefronts)
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-#pragma unroll
-    for (int w = 0; w < kRsWaves; w++) s_wh[w][tid] = 0;
+    for (int i = tid; i < NW * kRsRadix; i += T) (&s_wh[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint64_t tbase = (uint64_t)tile * kRsTile;
-    const uint32_t tile_n = (uint32_t)min<uint64_t>(kRsTile, n - tbase);
+    const uint64_t tbase = (uint64_t)tile * TILE;
+    const uint32_t tile_n = (uint32_t)min<uint64_t>(TILE, n - tbase);
     const uint32_t dmask = (1u << nbits) - 1u;
 
     uint64_t k[kRsItems];
@@ -140,57 +148,79 @@ __global__ __launch_bounds__(kRsThreads) void rs_onesweep_pass(const uint64_t *_
     __syncthreads();
 
     // per-digit: wave exclusive offsets and tile count (thread tid owns digit tid)
-    uint32_t run = 0;
+    uint32_t tcount = 0, incl = 0;
+    uint32_t *my_status = status + (size_t)tile * kRsRadix + (dig ? tid : 0);
+    if (dig) {
+        uint32_t run = 0;
 #pragma unroll
-    for (int w = 0; w < kRsWaves; w++) {
-        uint32_t c = s_wh[w][tid];
-        s_wh[w][tid] = run;
-        run += c;
-    }
-    const uint32_t tcount = run;
-    uint32_t *my_status = status + (size_t)tile * kRsRadix + tid;
-    if (tile == 0)
-        st_relaxed_agent(my_status, kFlagInc | tcount);
-    else
-        st_relaxed_agent(my_status, kFlagAgg | tcount);
-
-    // tile-local exclusive scan over the 256 digits
-    uint32_t incl = tcount;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += t;
-    }
-    if (lane == kWave - 1) s_scan[wave] = incl;
-    __syncthreads();
-    uint32_t wb = 0;
-    for (int w = 0; w < wave; w++) wb += s_scan[w];
-    const uint32_t toff = wb + incl - tcount;
-    s_toff[tid] = toff;
-
-    // decoupled look-back for digit tid
-    uint32_t prefix = 0;
-    if (tile > 0) {
-        int64_t p = (int64_t)tile - 1;
-        uint32_t spins = 0;
-        while (p >= 0) {
-            uint32_t sv = ld_relaxed_agent(status + (size_t)p * kRsRadix + tid);
-            uint32_t flag = sv & ~kValMask;
-            if (flag == 0) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            prefix += sv & kValMask;
-            if (flag == kFlagInc) break;
-            p--;
+        for (int w = 0; w < NW; w++) {
+            uint32_t c = s_wh[w][tid];
+            s_wh[w][tid] = run;
+            run += c;
         }
-        st_relaxed_agent(my_status, kFlagInc | ((prefix + tcount) & kValMask));
+        tcount = run;
+        if (tile == 0)
+            st_relaxed_agent(my_status, kFlagInc | tcount);
+        else
+            st_relaxed_agent(my_status, kFlagAgg | tcount);
+        // tile-local exclusive scan over the 256 digits
+        incl = tcount;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            uint32_t t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        if (lane == kWave - 1) s_scan[wave] = incl;
     }
-    s_gb[tid] = gbase[tid] + prefix - toff;
+    __syncthreads();
+    if (dig) {
+        uint32_t wb = 0;
+        for (int w = 0; w < wave; w++) wb += s_scan[w];
+        const uint32_t toff = wb + incl - tcount;
+        s_toff[tid] = toff;
+
+        // decoupled look-back for digit tid, kRsLookback predecessors per round
+        // trip: their status words are loaded together, then consumed in order
+        // (aggregates summed) up to the first inclusive one; a word not yet
+        // published ends the round and is loaded again.  Tiles below 0 read as an
+        // inclusive zero.
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            int64_t p = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                uint32_t sv[kRsLookback];
+#pragma unroll
+                for (int q = 0; q < kRsLookback; q++)
+                    sv[q] = p - q >= 0 ? ld_relaxed_agent(status + (size_t)(p - q) * kRsRadix + tid) : kFlagInc;
+                int used = 0;
+                bool inc = false, stalled = false;
+#pragma unroll
+                for (int q = 0; q < kRsLookback; q++) {
+                    if (inc || stalled) continue;
+                    const uint32_t flag = sv[q] & ~kValMask;
+                    if (flag == 0) {
+                        stalled = true;
+                        continue;
+                    }
+                    prefix += sv[q] & kValMask;
+                    used++;
+                    inc = flag == kFlagInc;
+                }
+                if (inc) break;
+                p -= used;
+                if (used == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            st_relaxed_agent(my_status, kFlagInc | ((prefix + tcount) & kValMask));
+        }
+        s_gb[tid] = gbase[tid] + prefix - toff;
+    }
     __syncthreads();
 
     // reorder the tile in LDS by digit
@@ -207,7 +237,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_onesweep_pass(const uint64_t *_
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRsItems; j++) {
-        uint32_t p = (uint32_t)j * kRsThreads + tid;
+        uint32_t p = (uint32_t)j * T + tid;
         if (p < tile_n) {
             uint64_t key = s_keys[p];
             uint32_t d = (uint32_t)(key >> shift) & dmask;
@@ -276,7 +306,7 @@ int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t 
     SortWork w;
     sort_layout(c, n, bits, hv, &w);
     const int np = npasses_for(bits);
-    const uint64_t ntiles = div_up(n, kRsTile);
+    const uint64_t ntiles = div_up(n, hv ? kRsTile : kRsTileK);  // <= the laid-out div_up(n, kRsTile)
     PassDesc pd;
     for (int p = 0; p < kRsMaxPasses; p++) {
         pd.shift[p] = lo + 8 * p;
@@ -294,11 +324,13 @@ int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t 
     for (int p = 0; p < np; p++) {
         uint32_t *st = w.status + (size_t)p * ntiles * kRsRadix;
         if (hv)
-            hipLaunchKernelGGL(rs_onesweep_pass<true>, dim3((unsigned)ntiles), dim3(kRsThreads), 0, s, ka, va, kb,
+            hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads>), dim3((unsigned)ntiles), dim3(kRsThreads), 0, s,
+                               ka, va, kb,
                                vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p,
                                w.counters + np);
         else
-            hipLaunchKernelGGL(rs_onesweep_pass<false>, dim3((unsigned)ntiles), dim3(kRsThreads), 0, s, ka,
+            hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK>), dim3((unsigned)ntiles), dim3(kRsThreadsK), 0,
+                               s, ka,
                                (const uint32_t *)nullptr, kb, (uint32_t *)nullptr, n, pd.shift[p], pd.bits[p],
                                w.ghist + p * kRsRadix, st, w.counters + p, w.counters + np);
         GCHI_LAUNCH_CHECK();
