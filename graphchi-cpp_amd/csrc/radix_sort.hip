@@ -35,7 +35,8 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 26;
-constexpr int kRsLookback = 16;  // predecessor tiles read per look-back round trip
+constexpr int kRsLookback = 4;
+constexpr int kRsHistKeys = 8;  // keys per thread in flight in the histogram pass  // predecessor tiles read per look-back round trip
 
 struct PassDesc {
     int shift[kRsMaxPasses];
@@ -49,12 +50,28 @@ __global__ __launch_bounds__(kRsThreads) void rs_histogram(const uint64_t *keys,
     for (int i = threadIdx.x; i < kRsWaves * kRsMaxPasses * kRsRadix; i += kRsThreads)
         (&s_h[0][0][0])[i] = 0;
     __syncthreads();
-    uint64_t stride = (uint64_t)gridDim.x * kRsThreads;
-    for (uint64_t i = (uint64_t)blockIdx.x * kRsThreads + threadIdx.x; i < n; i += stride) {
-        uint64_t k = keys[i];
-        for (int p = 0; p < npasses; p++) {
-            uint32_t d = (uint32_t)(k >> pd.shift[p]) & ((1u << pd.bits[p]) - 1u);
-            atomicAdd(&s_h[wave][p][d], 1u);
+    // kRsHistKeys keys per thread are loaded before any is counted (one load in
+    // flight per thread would leave the pass bound by memory latency).  Digit 0,
+    // the common digit of the high passes for graph ids, is counted by one
+    // ballot per wavefront instead of up to 64 LDS atomics on one word.
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr uint64_t kSpan = (uint64_t)kRsThreads * kRsHistKeys;
+    for (uint64_t base = (uint64_t)blockIdx.x * kSpan; base < n; base += (uint64_t)gridDim.x * kSpan) {
+        uint64_t kk[kRsHistKeys];
+#pragma unroll
+        for (int u = 0; u < kRsHistKeys; u++) {
+            const uint64_t i = base + (uint64_t)u * kRsThreads + threadIdx.x;
+            kk[u] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kRsHistKeys; u++) {
+            const bool valid = base + (uint64_t)u * kRsThreads + threadIdx.x < n;
+            for (int p = 0; p < npasses; p++) {
+                const uint32_t d = (uint32_t)(kk[u] >> pd.shift[p]) & ((1u << pd.bits[p]) - 1u);
+                const uint64_t zero = __ballot(valid && d == 0u);
+                if (valid && d != 0u) atomicAdd(&s_h[wave][p][d], 1u);
+                if (lane == 0 && zero) atomicAdd(&s_h[wave][p][0], (uint32_t)__popcll(zero));
+            }
         }
     }
     __syncthreads();
