@@ -43,8 +43,26 @@ struct PassDesc {
     int bits[kRsMaxPasses];
 };
 
+// (src, dst) pairs read as the key (dst << id_bits) | src by a first pass, and
+// written back as pairs by a last pass, so the pair sort needs no separate pack
+// and unpack over HBM.  IO bit 1: the pass (or histogram) reads pairs; bit 2:
+// the pass writes pairs.
+struct PairIO {
+    const uint32_t *src_in, *dst_in;
+    uint32_t *src_out, *dst_out;
+    int id_bits;
+};
+constexpr int kIoPackIn = 1, kIoUnpackOut = 2;
+
+template <int IO>
+GCHI_DEV uint64_t load_key(const uint64_t *kin, const PairIO &io, uint64_t i) {
+    if (IO & kIoPackIn) return ((uint64_t)io.dst_in[i] << io.id_bits) | (uint64_t)io.src_in[i];
+    return kin[i];
+}
+
+template <int IO>
 __global__ __launch_bounds__(kRsThreads) void rs_histogram(const uint64_t *keys, uint64_t n, int npasses,
-                                                           PassDesc pd, uint32_t *ghist) {
+                                                           PassDesc pd, uint32_t *ghist, PairIO io) {
     __shared__ uint32_t s_h[kRsWaves][kRsMaxPasses][kRsRadix];  // 32 KB
     const int wave = threadIdx.x / kWave;
     for (int i = threadIdx.x; i < kRsWaves * kRsMaxPasses * kRsRadix; i += kRsThreads)
@@ -61,7 +79,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_histogram(const uint64_t *keys,
 #pragma unroll
         for (int u = 0; u < kRsHistKeys; u++) {
             const uint64_t i = base + (uint64_t)u * kRsThreads + threadIdx.x;
-            kk[u] = i < n ? keys[i] : 0ull;
+            kk[u] = i < n ? load_key<IO>(keys, io, i) : 0ull;
         }
 #pragma unroll
         for (int u = 0; u < kRsHistKeys; u++) {
@@ -101,14 +119,14 @@ __global__ void rs_hist_scan(uint32_t *ghist) {
     h[threadIdx.x] = s[threadIdx.x];
 }
 
-template <bool kVals, int T>
+template <bool kVals, int T, int IO = 0>
 __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict__ kin,
                                                                const uint32_t *__restrict__ vin,
                                                                uint64_t *__restrict__ kout,
                                                                uint32_t *__restrict__ vout, uint64_t n, int shift,
                                                                int nbits, const uint32_t *__restrict__ gbase,
                                                                uint32_t *status, uint32_t *tile_counter,
-                                                               uint32_t *err) {
+                                                               uint32_t *err, PairIO io) {
     constexpr int NW = T / kWave;
     constexpr int TILE = T * kRsItems;
     static_assert(T >= kRsRadix && T % kRsRadix == 0, "a thread per digit");
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
 #pragma unroll
     for (int i = 0; i < kRsItems; i++) {
         uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-        k[i] = idx < n ? kin[idx] : 0ull;
+        k[i] = idx < n ? load_key<IO>(kin, io, idx) : 0ull;
         if (kVals) v[i] = idx < n ? vin[idx] : 0u;
     }
     const uint64_t lt = lanemask_lt();
@@ -259,7 +277,12 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
             uint64_t key = s_keys[p];
             uint32_t d = (uint32_t)(key >> shift) & dmask;
             uint32_t g = s_gb[d] + p;
-            kout[g] = key;
+            if (IO & kIoUnpackOut) {
+                io.src_out[g] = (uint32_t)(key & ((1ull << io.id_bits) - 1ull));
+                io.dst_out[g] = (uint32_t)(key >> io.id_bits);
+            } else {
+                kout[g] = key;
+            }
             if (kVals) vout[g] = s_vals[p];
         }
     }
@@ -308,8 +331,37 @@ size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
 // back into keys; otherwise *out receives the buffer that holds it (keys or the
 // second buffer) and nothing is copied.  Sorting on [0, b) and then on [b, hi)
 // gives the same order as sorting on [0, hi) at once (every pass is stable).
-int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
-                        size_t work_bytes, hipStream_t s, uint64_t **out) {
+// Key-only pass with the pair reads/writes chosen at run time.
+static void launch_key_pass(int io_mode, const uint64_t *kin, uint64_t *kout, uint64_t n, int shift, int bits,
+                            const uint32_t *gbase, uint32_t *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
+                            uint64_t ntiles, hipStream_t s) {
+    const dim3 grid((unsigned)ntiles), blk(kRsThreadsK);
+    const uint32_t *nv = nullptr;
+    uint32_t *nvo = nullptr;
+    switch (io_mode) {
+    case 0:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, 0>), grid, blk, 0, s, kin, nv, kout, nvo, n, shift,
+                           bits, gbase, st, ctr, err, io);
+        break;
+    case kIoPackIn:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn>), grid, blk, 0, s, kin, nv, kout, nvo, n,
+                           shift, bits, gbase, st, ctr, err, io);
+        break;
+    case kIoUnpackOut:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoUnpackOut>), grid, blk, 0, s, kin, nv, kout, nvo,
+                           n, shift, bits, gbase, st, ctr, err, io);
+        break;
+    default:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn | kIoUnpackOut>), grid, blk, 0, s, kin, nv,
+                           kout, nvo, n, shift, bits, gbase, st, ctr, err, io);
+    }
+}
+
+// The sort itself; with pio, the keys come from pio's (src, dst) input and the
+// result goes to its (src, dst) output (keys and alt are then the two
+// intermediate buffers, and nothing is copied back).
+static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
+                     size_t work_bytes, hipStream_t s, uint64_t **out, const PairIO *pio) {
     if (out) *out = keys;
     if (n <= 1 || hi <= lo) return GCHI_OK;
     GCHI_CHECK(keys, GCHI_ERR_INVALID, "sort: null keys");
@@ -333,7 +385,12 @@ int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t 
     launch_fill_u32(w.status, 0u, (size_t)np * ntiles * kRsRadix, s);
     launch_fill_u32(w.counters, 0u, np + 1, s);
     unsigned hblocks = (unsigned)std::min<uint64_t>(ntiles, 1024);
-    hipLaunchKernelGGL(rs_histogram, dim3(hblocks), dim3(kRsThreads), 0, s, keys, n, np, pd, w.ghist);
+    const PairIO io = pio ? *pio : PairIO{};
+    if (pio)
+        hipLaunchKernelGGL(rs_histogram<kIoPackIn>, dim3(hblocks), dim3(kRsThreads), 0, s, keys, n, np, pd, w.ghist,
+                           io);
+    else
+        hipLaunchKernelGGL(rs_histogram<0>, dim3(hblocks), dim3(kRsThreads), 0, s, keys, n, np, pd, w.ghist, io);
     hipLaunchKernelGGL(rs_hist_scan, dim3(np), dim3(kRsRadix), 0, s, w.ghist);
     GCHI_LAUNCH_CHECK();
     uint64_t *ka = keys, *kb = alt ? alt : w.ktmp;
@@ -344,16 +401,16 @@ int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t 
             hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads>), dim3((unsigned)ntiles), dim3(kRsThreads), 0, s,
                                ka, va, kb,
                                vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p,
-                               w.counters + np);
+                               w.counters + np, PairIO{});
         else
-            hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK>), dim3((unsigned)ntiles), dim3(kRsThreadsK), 0,
-                               s, ka,
-                               (const uint32_t *)nullptr, kb, (uint32_t *)nullptr, n, pd.shift[p], pd.bits[p],
-                               w.ghist + p * kRsRadix, st, w.counters + p, w.counters + np);
+            launch_key_pass(pio ? (p == 0 ? kIoPackIn : 0) | (p == np - 1 ? kIoUnpackOut : 0) : 0, ka, kb, n,
+                            pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p, w.counters + np, io,
+                            ntiles, s);
         GCHI_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);
     }
+    if (pio) return GCHI_OK;
     if (out) {
         *out = ka;
     } else if (ka != keys) {
@@ -365,6 +422,11 @@ int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t 
     return GCHI_OK;
 }
 
+int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
+                        size_t work_bytes, hipStream_t s, uint64_t **out) {
+    return sort_impl(keys, alt, vals, n, lo, hi, work, work_bytes, s, out, nullptr);
+}
+
 // Sorts keys (and vals) in place on their low key_bits bits.
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, void *work, size_t work_bytes,
                    hipStream_t s) {
@@ -373,21 +435,7 @@ int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, voi
     return radix_sort_u64_bits(keys, nullptr, vals, n, 0, key_bits, work, work_bytes, s, nullptr);
 }
 
-// ---- (src,dst) pair sort by (dst,src) ----
-__global__ void pack_dst_src(const uint32_t *src, const uint32_t *dst, uint64_t n, int id_bits, uint64_t *keys) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        keys[i] = ((uint64_t)dst[i] << id_bits) | (uint64_t)src[i];
-}
-__global__ void unpack_dst_src(const uint64_t *keys, uint64_t n, int id_bits, uint32_t *src, uint32_t *dst) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t m = (1ull << id_bits) - 1ull;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t k = keys[i];
-        src[i] = (uint32_t)(k & m);
-        dst[i] = (uint32_t)(k >> id_bits);
-    }
-}
+// ---- (src,dst) pair sort by (dst,src): sort_impl with PairIO ----
 
 }  // namespace gchi
 
@@ -404,11 +452,16 @@ extern "C" int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int
     return radix_sort_u64(d_keys, d_vals, n, key_bits, d_work, work_bytes, as_stream(stream));
 }
 
+// Key bits of the pair sort: at least 9, so that there are two passes and the
+// pairs are read by one kernel and written by another (the output may be the
+// input); the extra high bits are zero and leave the order as it is.
+static int pair_key_bits(int id_bits) { return std::max(2 * id_bits, 9); }
+
 extern "C" int gchi_sort_pairs_dst_src_workspace(uint64_t n, int id_bits, size_t *bytes) {
     GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
     Sizer z;
     z.take<uint64_t>(n);
-    z.take<char>(sort_workspace_bytes(n, 2 * id_bits, false));
+    z.take<char>(sort_workspace_bytes(n, pair_key_bits(id_bits), false));
     *bytes = z.off;
     return GCHI_OK;
 }
@@ -425,13 +478,13 @@ extern "C" int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_
     hipStream_t s = as_stream(stream);
     Carver c(d_work, work_bytes);
     uint64_t *keys = c.take<uint64_t>(n);
-    size_t sw_bytes = sort_workspace_bytes(n, 2 * id_bits, false);
+    size_t sw_bytes = sort_workspace_bytes(n, pair_key_bits(id_bits), false);
     void *sw = c.take<char>(sw_bytes);
-    unsigned b = (unsigned)std::min<uint64_t>(div_up(n, 256), 65536);
-    hipLaunchKernelGGL(pack_dst_src, dim3(b), dim3(256), 0, s, d_src, d_dst, n, id_bits, keys);
-    GCHI_LAUNCH_CHECK();
-    GCHI_TRY(radix_sort_u64(keys, nullptr, n, 2 * id_bits, sw, sw_bytes, s));
-    hipLaunchKernelGGL(unpack_dst_src, dim3(b), dim3(256), 0, s, keys, n, id_bits, d_src_out, d_dst_out);
-    GCHI_LAUNCH_CHECK();
-    return GCHI_OK;
+    const PairIO io{d_src, d_dst, d_src_out, d_dst_out, id_bits};
+    if (n == 1) {  // nothing to sort: the pair is copied
+        if (d_src_out != d_src) GCHI_HIP(hipMemcpyAsync(d_src_out, d_src, 4, hipMemcpyDeviceToDevice, s));
+        if (d_dst_out != d_dst) GCHI_HIP(hipMemcpyAsync(d_dst_out, d_dst, 4, hipMemcpyDeviceToDevice, s));
+        return GCHI_OK;
+    }
+    return sort_impl(keys, nullptr, nullptr, n, 0, pair_key_bits(id_bits), sw, sw_bytes, s, nullptr, &io);
 }

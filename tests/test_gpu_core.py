@@ -57,6 +57,32 @@ def test_sort_pairs_dst_src(dev):
     np.testing.assert_array_equal(dev.to_host_u32(do, cnt), hd[o])
 
 
+@pytest.mark.parametrize("n,id_bits", [(1, 1), (2, 1), (1000, 3), (5000, 4), (70001, 11), (300007, 32), (8192, 13)])
+def test_sort_pairs_dst_src_sizes(dev, n, id_bits):
+    """Pair sorts whose first pass reads the (src, dst) arrays and whose last
+    pass writes them: one- and two-digit keys (an extra all-zero pass), ragged
+    tile counts, 32-bit ids; then the same sort in place (outputs = inputs)."""
+    import ctypes
+    rng = np.random.default_rng(n + id_bits)
+    hi = (1 << id_bits) - 1
+    hs = rng.integers(0, hi + 1, n, dtype=np.uint64).astype(np.uint32)
+    hd = rng.integers(0, hi + 1, n, dtype=np.uint64).astype(np.uint32)
+    hd[: n // 3] = hd[0]  # long runs of one destination
+    o = np.lexsort((hs, hd))
+    s, d = dev.to_dev_u32(hs), dev.to_dev_u32(hd)
+    so, do = dev.sort_pairs_dst_src(s, d, id_bits)
+    np.testing.assert_array_equal(dev.to_host_u32(so, n), hs[o])
+    np.testing.assert_array_equal(dev.to_host_u32(do, n), hd[o])
+    L = dev.lib()
+    nb = dev.c_sz()
+    dev.check(L.gchi_sort_pairs_dst_src_workspace(n, id_bits, ctypes.byref(nb)))
+    ws = dev.workspace(nb.value)
+    dev.check(L.gchi_sort_pairs_dst_src(dev.dptr(s), dev.dptr(d), n, id_bits, dev.dptr(s), dev.dptr(d), dev.dptr(ws),
+                                        nb.value, dev.stream()), "sort_pairs in place")
+    np.testing.assert_array_equal(dev.to_host_u32(s, n), hs[o])
+    np.testing.assert_array_equal(dev.to_host_u32(d, n), hd[o])
+
+
 @pytest.mark.parametrize("case", [c for c in graphs.PR_CASES] + graphs.CC_CASES[:1],
                          ids=[c["name"] for c in graphs.PR_CASES] + [graphs.CC_CASES[0]["name"]])
 def test_csc_build_bit_exact(dev, case):
