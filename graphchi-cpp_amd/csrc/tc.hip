@@ -256,6 +256,54 @@ GCHI_DEV int tc_find(const uint32_t *keys, uint32_t mask, int sh, uint32_t w) {
 
 GCHI_DEV void tc_credit(uint32_t *cw, uint32_t slot, uint32_t c) { atomicAdd(&cw[slot >> 1], c << ((slot & 1) * 16)); }
 
+// Entry `lane` of N+(b_j) (b_j, its start and length in lane j of b/bb/lb), or
+// kTcEmpty past the list; the load is issued at an in-bounds address either way.
+GCHI_DEV uint32_t tc_first_w(const uint32_t *adj, uint32_t bb, uint32_t lb, uint32_t j, int lane) {
+    const uint32_t bbj = (uint32_t)__builtin_amdgcn_readlane((int)bb, (int)j);
+    const uint32_t lbj = (uint32_t)__builtin_amdgcn_readlane((int)lb, (int)j);
+    const uint32_t v = adj[lbj ? bbj + min((uint32_t)lane, lbj - 1) : 0u];
+    return (uint32_t)lane < lbj ? v : kTcEmpty;
+}
+
+// Triangles (a, b_j, w) for w in N+(b_j): w0 holds the list's first 64
+// entries, the rest are loaded here; each w found in the table is credited,
+// and b_j once with the wavefront's total.  Returns that total.
+GCHI_DEV uint32_t tc_count_b(const uint32_t *adj, const uint32_t *keys, uint32_t *cw, uint32_t mask, int sh,
+                             uint32_t b, uint32_t bb, uint32_t lb, uint32_t j, uint32_t w0, int lane) {
+    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)j);
+    const uint32_t bbj = (uint32_t)__builtin_amdgcn_readlane((int)bb, (int)j);
+    const uint32_t lbj = (uint32_t)__builtin_amdgcn_readlane((int)lb, (int)j);
+    const uint32_t *B = adj + bbj;
+    uint32_t hits = 0;
+    if (w0 != kTcEmpty) {
+        const int h = tc_find(keys, mask, sh, w0);
+        if (h >= 0) {
+            tc_credit(cw, (uint32_t)h, 1u);
+            hits++;
+        }
+    }
+    for (uint32_t t = kWave + lane; t < lbj; t += kTcInflight * kWave) {
+        uint32_t w[kTcInflight];
+#pragma unroll
+        for (int q = 0; q < kTcInflight; q++) {
+            const uint32_t tq = t + q * kWave;
+            w[q] = tq < lbj ? B[tq] : kTcEmpty;
+        }
+#pragma unroll
+        for (int q = 0; q < kTcInflight; q++) {
+            if (w[q] == kTcEmpty) continue;
+            const int h = tc_find(keys, mask, sh, w[q]);
+            if (h >= 0) {
+                tc_credit(cw, (uint32_t)h, 1u);
+                hits++;
+            }
+        }
+    }
+    hits = wave_sum(hits);
+    if (hits && lane == 0) tc_credit(cw, (uint32_t)tc_find(keys, mask, sh, bj), hits);
+    return hits;
+}
+
 __global__ __launch_bounds__(kTcHashWaves *kWave) void tc_count_hash_kernel(const uint32_t *__restrict__ off,
                                                                             const uint32_t *__restrict__ adj,
                                                                             uint32_t n, unsigned long long *cnt,
@@ -306,34 +354,15 @@ __global__ __launch_bounds__(kTcHashWaves *kWave) void tc_count_hash_kernel(cons
             const uint32_t bb = valid ? off[b] : 0u;
             const uint32_t lb = valid ? off[b + 1] - bb : 0u;
             const uint32_t nb = min((uint32_t)kWave, da - 1 - bc);
-            for (uint32_t j = 0; j < nb; j++) {
-                const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)j);
-                const uint32_t bbj = (uint32_t)__builtin_amdgcn_readlane((int)bb, (int)j);
-                const uint32_t lbj = (uint32_t)__builtin_amdgcn_readlane((int)lb, (int)j);
-                const uint32_t *B = adj + bbj;
-                uint32_t hits = 0;
-                for (uint32_t t = lane; t < lbj; t += kTcInflight * kWave) {
-                    uint32_t w[kTcInflight];
-#pragma unroll
-                    for (int q = 0; q < kTcInflight; q++) {
-                        const uint32_t tq = t + q * kWave;
-                        w[q] = tq < lbj ? B[tq] : kTcEmpty;
-                    }
-#pragma unroll
-                    for (int q = 0; q < kTcInflight; q++) {
-                        if (w[q] == kTcEmpty) continue;
-                        const int h = tc_find(keys, mask, sh, w[q]);
-                        if (h >= 0) {
-                            tc_credit(cw, (uint32_t)h, 1u);
-                            hits++;
-                        }
-                    }
-                }
-                hits = wave_sum(hits);
-                if (hits) {
-                    if (lane == 0) tc_credit(cw, (uint32_t)tc_find(keys, mask, sh, bj), hits);
-                    acc_a += hits;
-                }
+            // The first 64 entries of N+(b_j) are loaded one b ahead (slots w2[0],
+            // w2[1] in turn), so that b_j's probes overlap b_{j+1}'s loads.
+            uint32_t w2[2];
+            w2[0] = tc_first_w(adj, bb, lb, 0, lane);
+            for (uint32_t j0 = 0; j0 < nb; j0 += 2) {  // loads unconditional (clamped j), uses not
+                w2[1] = tc_first_w(adj, bb, lb, min(j0 + 1, nb - 1), lane);
+                acc_a += tc_count_b(adj, keys, cw, mask, sh, b, bb, lb, j0, w2[0], lane);
+                w2[0] = tc_first_w(adj, bb, lb, min(j0 + 2, nb - 1), lane);
+                if (j0 + 1 < nb) acc_a += tc_count_b(adj, keys, cw, mask, sh, b, bb, lb, j0 + 1, w2[1], lane);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
