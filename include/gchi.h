@@ -90,7 +90,8 @@ int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits,
 
 /* Sorts uint32 (src,dst) pairs by (dst,src) -- the order the sharder's shovel
  * sort (sharder.hpp:192, dst key) followed by a full tie order produces.
- * Pairs are read from d_src/d_dst and written sorted to d_src_out/d_dst_out.
+ * Pairs are read from d_src/d_dst and written sorted to d_src_out/d_dst_out,
+ * which may be d_src/d_dst themselves (an in-place sort).
  * id_bits = bits needed for the largest vertex id. */
 int gchi_sort_pairs_dst_src_workspace(uint64_t n, int id_bits, size_t *bytes);
 int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_dst, uint64_t n, int id_bits,
