@@ -23,6 +23,8 @@
 // Precision: phase 1 stores ranks in BF16 with FP32 accumulation; phase 2
 // converts to FP32 and refines with FP32 sweeps until every query meets the
 // tolerance.  The per-query L1 residuals are FP64 sums.
+#include <type_traits>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -128,7 +130,7 @@ GCHI_DEV float seed_mass(const PpArgs &a, uint32_t qb, uint32_t v, int lane) {
 
 // sum over in-edges [beg, end) of r_u * inv_u for this lane's query
 template <typename T>
-GCHI_DEV float gather_rows(const uint32_t *src_idx, uint32_t beg, uint32_t end, const float *inv, const T *R,
+GCHI_DEV float gather_rows_1(const uint32_t *src_idx, uint32_t beg, uint32_t end, const float *inv, const T *R,
                            int lane) {
     float acc = 0.0f;
     for (uint32_t e = beg; e < end; e += kWave) {
@@ -171,6 +173,80 @@ GCHI_DEV float gather_rows(const uint32_t *src_idx, uint32_t beg, uint32_t end, 
         }
     }
     return acc;
+}
+
+// BF16 rows: a 64-query row is 128 bytes, so one load instruction can carry
+// two edges.  Lanes 0-31 take the even edges of the row (counted from beg),
+// lanes 32-63 the odd ones; lane 32h + p loads queries 2p and 2p+1 of its
+// edge's source row as one 32-bit word.  Each query's sum is its even-edge
+// sum plus its odd-edge sum (both in edge order), combined at the end and
+// moved to lane q for query q.
+GCHI_DEV float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+GCHI_DEV float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
+
+GCHI_DEV float gather_rows_bf16x2(const uint32_t *src_idx, uint32_t beg, uint32_t end, const float *inv,
+                                  const uint16_t *R, int lane) {
+    const bool odd = lane >= 32;
+    const int p = lane & 31;
+    const uint32_t *R32 = reinterpret_cast<const uint32_t *>(R);  // row s: words s*32 .. s*32+31
+    float a0 = 0.0f, a1 = 0.0f;
+    for (uint32_t e = beg; e < end; e += kWave) {
+        const uint32_t n = min((uint32_t)kWave, end - e);
+        const uint32_t my = lane < (int)n ? __builtin_nontemporal_load(&src_idx[e + lane]) : 0u;
+        const float myi = lane < (int)n ? inv[my] : 0.0f;
+        uint32_t j = 0;
+        for (; j + 16 <= n; j += 16) {  // 8 edge pairs in flight
+            uint32_t v[8];
+            float w[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t s0 = __builtin_amdgcn_readlane(my, j + 2 * q);
+                const uint32_t s1 = __builtin_amdgcn_readlane(my, j + 2 * q + 1);
+                const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j + 2 * q));
+                const float w1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j + 2 * q + 1));
+                w[q] = odd ? w1 : w0;
+                v[q] = R32[(size_t)(odd ? s1 : s0) * (kQ / 2) + p];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                a0 += bf16_lo(v[q]) * w[q];
+                a1 += bf16_hi(v[q]) * w[q];
+            }
+        }
+        for (; j + 2 <= n; j += 2) {
+            const uint32_t s0 = __builtin_amdgcn_readlane(my, j), s1 = __builtin_amdgcn_readlane(my, j + 1);
+            const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j));
+            const float w1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j + 1));
+            const float w = odd ? w1 : w0;
+            const uint32_t v = R32[(size_t)(odd ? s1 : s0) * (kQ / 2) + p];
+            a0 += bf16_lo(v) * w;
+            a1 += bf16_hi(v) * w;
+        }
+        if (j < n) {  // the chunk's last edge is even: the odd half loads it and discards it
+            const uint32_t s0 = __builtin_amdgcn_readlane(my, j);
+            const float w0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myi), j));
+            const uint32_t v = R32[(size_t)s0 * (kQ / 2) + p];
+            if (!odd) {
+                a0 += bf16_lo(v) * w0;
+                a1 += bf16_hi(v) * w0;
+            }
+        }
+    }
+    // query 2p: even part (lane p) + odd part (lane 32 + p), in that order
+    const float o0 = __shfl_xor(a0, 32, kWave), o1 = __shfl_xor(a1, 32, kWave);
+    const float t0 = odd ? o0 + a0 : a0 + o0;
+    const float t1 = odd ? o1 + a1 : a1 + o1;
+    const float u0 = __shfl(t0, lane >> 1, kWave), u1 = __shfl(t1, lane >> 1, kWave);
+    return (lane & 1) ? u1 : u0;
+}
+
+template <typename T>
+GCHI_DEV float gather_rows(const uint32_t *src_idx, uint32_t beg, uint32_t end, const float *inv, const T *R,
+                           int lane) {
+    if constexpr (std::is_same<T, uint16_t>::value)
+        return gather_rows_bf16x2(src_idx, beg, end, inv, R, lane);
+    else
+        return gather_rows_1<T>(src_idx, beg, end, inv, R, lane);
 }
 
 template <typename T>
