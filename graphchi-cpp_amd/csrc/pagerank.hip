@@ -332,6 +332,27 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_persistent_kernel(PrArgs 
     }
 }
 
+// Copies layout positions [0, H) of xg into the LDS cache, kFillAhead loads per
+// thread in flight (one at a time would take a memory round trip per 1024
+// values).
+constexpr uint32_t kFillAhead = 8;
+
+GCHI_DEV void fill_hot(float *hot, const float *__restrict__ xg, uint32_t H, uint32_t tid, uint32_t nthreads) {
+    for (uint32_t i0 = tid; i0 < H; i0 += nthreads * kFillAhead) {
+        float v[kFillAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kFillAhead; u++) {
+            const uint32_t i = i0 + u * nthreads;
+            v[u] = xg[i < H ? i : 0u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kFillAhead; u++) {
+            const uint32_t i = i0 + u * nthreads;
+            if (i < H) hot[i] = v[u];
+        }
+    }
+}
+
 // Persistent workgroups with a static tile list (tile = blockIdx.x, then every
 // gridDim.x-th) and an LDS cache of layout positions [0, H), loaded once per
 // workgroup; several such workgroups share a CU, each with its own copy.
@@ -342,7 +363,7 @@ __global__ __launch_bounds__(kPrThreads) void pr_sweep_hotwg_kernel(PrArgs a, Pr
     __shared__ double s_red[2 * kPrWaves];
     float *hot = lds;
     float *s_val = lds + H;
-    for (uint32_t i = threadIdx.x; i < H; i += kPrThreads) hot[i] = a.xg[i];
+    fill_hot(hot, a.xg, H, threadIdx.x, kPrThreads);
     __syncthreads();
     for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         pr_tile<kPrThreads, true>(a, c, tile, s_val, s_long, &s_nlong, s_red, hot, H);
@@ -719,7 +740,7 @@ __global__ __launch_bounds__(T) void pr_sweep_wave_kernel(PrArgs a, PrConsts c, 
     float *hot = lds;
     float *st = lds + H + (size_t)wave * PER * kWave;
     if (HOT) {
-        for (uint32_t i = tid; i < H; i += T) hot[i] = a.xg[i];
+        fill_hot(hot, a.xg, H, tid, T);
     }
     __syncthreads();
     const uint32_t nwv = gridDim.x * NW;
