@@ -339,8 +339,17 @@ __global__ __launch_bounds__(kPpThreads) void ppr_hub_kernel(PpArgs a, const T *
     for (uint32_t it = blockIdx.x * kPpWaves + wave; it < items; it += nw) {
         const uint32_t qb = it / a.pv.nhub, h = it % a.pv.nhub;
         const uint32_t c0 = a.pv.hub_chunk_off[h], c1 = a.pv.hub_chunk_off[h + 1];
+        // the chunk partials added in chunk order, 16 loads in flight
+        const float *part = a.hub_part + (size_t)qb * a.pv.nchunk * kQ + lane;
         float acc = 0.0f;
-        for (uint32_t c = c0; c < c1; c++) acc += a.hub_part[((size_t)qb * a.pv.nchunk + c) * kQ + lane];
+        for (uint32_t c = c0; c < c1; c += 16) {
+            float pv[16];
+#pragma unroll
+            for (uint32_t q = 0; q < 16; q++) pv[q] = part[(size_t)min(c + q, c1 - 1) * kQ];
+#pragma unroll
+            for (uint32_t q = 0; q < 16; q++)
+                if (c + q < c1) acc += pv[q];
+        }
         const uint32_t v = a.pv.hub_v[h];
         const T *Rin = Rin_all + qb * stride_q;
         T *Rout = Rout_all + qb * stride_q;
