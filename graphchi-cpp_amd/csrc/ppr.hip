@@ -336,8 +336,17 @@ __global__ __launch_bounds__(kPpThreads) void ppr_hub_kernel(PpArgs a, const T *
     const uint32_t nw = gridDim.x * kPpWaves;
     const size_t stride_q = (size_t)a.pv.N * kQ;
     const uint32_t items = a.pv.nhub * a.nqb;
+    // residuals summed per wavefront while its items stay in one query block,
+    // then one atomic per query (not one per hub row and query)
+    double racc = 0.0;
+    uint32_t rqb = 0xffffffffu;
     for (uint32_t it = blockIdx.x * kPpWaves + wave; it < items; it += nw) {
         const uint32_t qb = it / a.pv.nhub, h = it % a.pv.nhub;
+        if (a.residual && qb != rqb) {  // wavefront-uniform
+            if (rqb != 0xffffffffu && racc != 0.0) atomicAdd(&a.res[rqb * kQ + lane], racc);
+            racc = 0.0;
+            rqb = qb;
+        }
         const uint32_t c0 = a.pv.hub_chunk_off[h], c1 = a.pv.hub_chunk_off[h + 1];
         // the chunk partials added in chunk order, 16 loads in flight
         const float *part = a.hub_part + (size_t)qb * a.pv.nchunk * kQ + lane;
@@ -354,12 +363,10 @@ __global__ __launch_bounds__(kPpThreads) void ppr_hub_kernel(PpArgs a, const T *
         const T *Rin = Rin_all + qb * stride_q;
         T *Rout = Rout_all + qb * stride_q;
         float rn = Store<T>::round(seed_mass(a, qb, v, lane) + a.damping * acc);
-        if (a.residual) {
-            double d = fabs((double)rn - (double)Store<T>::ld(Rin + (size_t)v * kQ + lane));
-            if (d != 0.0) atomicAdd(&a.res[qb * kQ + lane], d);
-        }
+        if (a.residual) racc += fabs((double)rn - (double)Store<T>::ld(Rin + (size_t)v * kQ + lane));
         Store<T>::st(Rout + (size_t)v * kQ + lane, rn);
     }
+    if (a.residual && rqb != 0xffffffffu && racc != 0.0) atomicAdd(&a.res[rqb * kQ + lane], racc);
     if (blockIdx.x == 0)  // work queue ready for the next sweep
         for (uint32_t i = threadIdx.x; i < kPpStripes; i += blockDim.x) a.counter[i * kPpStripeGap] = 0u;
 }
