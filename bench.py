@@ -140,6 +140,8 @@ def bench_load(gchi, torch, src, dst, E, N, tune):
 
 def sweep_kernels(info):
     """Kernel names of one sweep of the plan (what rocprofv3 lists)."""
+    if info["persistent"] == 4:  # rank space: no per-sweep layout refresh
+        return "pr_seg_sweep_kernel + pr_seg_finalize_kernel"
     main = {0: "pr_sweep_kernel", 1: "pr_sweep_persistent_kernel", 2: "pr_sweep_wave_kernel",
             3: "pr_sweep_hotwg_kernel"}[info["persistent"]]
     ks = ([] if not info["gather_sources"] else ["pr_gather_layout_kernel"]) + [main]
@@ -150,19 +152,23 @@ def sweep_kernels(info):
 # ------------------------------------------------------------------ PPR
 def bench_ppr(gchi, torch, args, g, plan, N):
     oracle = oracle_mod()
-    rng = np.random.default_rng(5)
-    queries = [rng.choice(N, size=int(rng.integers(1, 65)), replace=False).astype(np.uint32)
-               for _ in range(args.ppr_queries)]
+    queries = gchi.ppr_workload_queries(N, args.ppr_queries)
     info = plan.info()
-    # the batched kernel reads vertex ids from the plan's edges: same block shape, no gather layout
-    vplan = plan if not info["gather_sources"] else gchi.PageRankPlan(
-        g, edges_per_block=info["edges_per_block"], short_row_max=info["short_row_max"], persistent=1)
+    # the batched kernel reads vertex ids from the plan's edges: a vertex-layout
+    # plan (the tuned block shape where it has one)
+    if info["persistent"] == 4:
+        vplan = gchi.PageRankPlan(g, edges_per_block=2048, persistent=1)
+    elif info["gather_sources"]:
+        vplan = gchi.PageRankPlan(g, edges_per_block=info["edges_per_block"], short_row_max=info["short_row_max"],
+                                  persistent=1)
+    else:
+        vplan = plan
     b = gchi.PprBatch(vplan, len(queries))
     b.run(queries[:1] * len(queries), max_sweeps=8)  # warm-up (seed tables, caches)
     (info, resid), t_batch = timed(lambda: b.run(queries), torch)
     log(f"ppr batched: {len(queries)} queries in {t_batch:.3f}s, sweeps bf16={info['sweeps_bf16']} "
         f"fp32={info['sweeps_fp32']} max residual {info['max_residual']:.2e}")
-    plain = gchi.PlainPpr(plan)
+    plain = gchi.PlainPpr(vplan)
     plain.run(queries[0], max_sweeps=8)
     nplain = min(args.ppr_plain_queries, len(queries))
     sweeps_plain = []
@@ -181,7 +187,7 @@ def bench_ppr(gchi, torch, args, g, plan, N):
     errs_b, errs_p = [], []
     for qi in range(min(args.ppr_check, len(queries))):
         q = len(queries) - 1 - qi if qi % 2 else qi
-        r64, _ = oracle.ppr_f64(N, in_off, hs, outdeg, queries[q], 0.85, tol=1e-13)
+        r64, _ = oracle.ppr_f64(N, in_off, hs, outdeg, queries[q], 0.85, tol=1e-11)
         rb = b.ranks(q).cpu().numpy().astype(np.float64)
         rp, _ = plain.run(queries[q])
         rp = rp.cpu().numpy().astype(np.float64)
@@ -193,7 +199,8 @@ def bench_ppr(gchi, torch, args, g, plan, N):
                         "scale-22 graph; tol L1<=1e-6 (residual bound)",
             "batched_queries_per_s": round(qps_b, 2), "plain_fp32_queries_per_s": round(qps_p, 2),
             "speedup": round(qps_b / qps_p, 2), "batched_sweeps_bf16": info["sweeps_bf16"],
-            "batched_sweeps_fp32": info["sweeps_fp32"], "plain_sweeps_mean": float(np.mean(sweeps_plain)),
+            "batched_sweeps_fp32": info["sweeps_fp32"], "batched_block_sweeps_fp32": info["block_sweeps_fp32"],
+            "query_blocks": (len(queries) + 63) // 64, "plain_sweeps_mean": float(np.mean(sweeps_plain)),
             "plain_queries_timed": nplain,
             "l1_vs_fp64_batched_max": max(e[0] for e in errs_b) if errs_b else None,
             "maxabs_vs_fp64_batched_max": max(e[1] for e in errs_b) if errs_b else None,
@@ -241,15 +248,27 @@ def bench_cc(gchi, torch, scale):
     assert torch.equal(lab_all[:n], lab[:n])
     _, t_lp = timed(lambda: gchi.cc_labelprop(s2, d2, n), torch)
     _, sweeps = gchi.cc_labelprop(s2, d2, n)
-    # selective scheduling over the symmetric adjacency (CSR built once, outside the timing)
+    # forms over the symmetric adjacency (CSR built once, outside the timing):
+    # selective scheduling, and union-find with subgraph sampling
     g = gchi.DeviceGraph(s2, d2, n)
     del s2, d2
     gchi.cc_scheduled(g.in_off, g.in_src, n)
     (lab_s, rounds, evals), t_s = timed(lambda: gchi.cc_scheduled(g.in_off, g.in_src, n), torch)
     assert torch.equal(lab_s[:n], lab[:n])
+    gchi.cc_afforest(g.in_off, g.in_src, n)
+    lab_a, t_a = timed(lambda: gchi.cc_afforest(g.in_off, g.in_src, n), torch)
+    assert torch.equal(lab_a[:n], lab[:n])
+    (lab_p, psweeps), t_p = timed(lambda: gchi.cc_pull(g.in_off, g.in_src, n), torch)
+    assert torch.equal(lab_p[:n], lab[:n])
     del g
+    algo = 8 * e + 12 * n + 4  # one synchronous sweep's bytes: the workload's algorithmic unit
     return {"workload": f"undirected R-MAT scale {scale}, {e} directed edges (both directions), seed 3",
             "time_to_fixed_point_ms": round(t * 1e3, 3), "edges_per_s": round(e / t, 1), "components": ncomp,
+            "algorithmic_bytes_per_sweep": algo,
+            "unionfind_hbm_frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4),
+            "afforest_ms": round(t_a * 1e3, 3), "afforest_hbm_frac": round(algo / t_a / 1e9 / HBM_PEAK_GBS, 4),
+            "pull_sweeps": psweeps, "pull_ms_per_sweep": round(t_p * 1e3 / psweeps, 3),
+            "pull_sweep_hbm_frac": round(algo * psweeps / t_p / 1e9 / HBM_PEAK_GBS, 4),
             "symmetric_half_list": True, "time_all_entries_ms": round(t_all * 1e3, 3),
             "labelprop_sweeps": sweeps, "labelprop_time_ms": round(t_lp * 1e3, 2),
             "labelprop_edges_sweeps_per_s": round(e * sweeps / t_lp, 1),
@@ -287,12 +306,9 @@ def bench_multigraph(gchi, torch, ngraphs, sweeps, seed=9):
     own (local ids), the batch plan copies them side by side (64-bit offsets
     across graphs, so the total may exceed 2^32 edges), and every sweep of all
     graphs is one launch of the persistent sweep kernel plus the finalize pair."""
-    rng = np.random.default_rng(seed)
-    scales = rng.integers(10, 21, ngraphs)
-    efs = rng.integers(8, 33, ngraphs)
     graphs, ns = [], []
-    for i in range(ngraphs):
-        s, d, cnt, mx = gchi.rmat_device(int(scales[i]), int(efs[i]) << int(scales[i]), 10000 + i)
+    for scale, ne, gseed in gchi.multigraph_workload(ngraphs, seed):
+        s, d, cnt, mx = gchi.rmat_device(scale, ne, gseed)
         graphs.append(gchi.DeviceGraph(s, d, mx + 1))
         ns.append(mx + 1)
         del s, d
@@ -315,15 +331,45 @@ def bench_multigraph(gchi, torch, ngraphs, sweeps, seed=9):
             "graphs_per_s": round(ngraphs * sweeps / t, 1)}
 
 
+def host_cpus():
+    """Host CPUs this process may use: the affinity mask, capped by a cgroup
+    CPU quota when one is set; with the machine's count and CPU model."""
+    avail = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = avail if quota is None else max(1, min(avail, int(quota)))
+    return {"usable": usable, "affinity": avail, "nproc": os.cpu_count(), "cgroup_quota_cpus": quota,
+            "model": model}
+
+
 def cpu_baseline(hs, hd, n, sweeps, nshards):
     oracle = oracle_mod()
-    threads = min(16, os.cpu_count() or 1)
+    cpus = host_cpus()
+    threads = cpus["usable"]
     t = time.time()
     _, secs = oracle.pr_psw_omp(n, hs, hd, nshards, 0.15, 0.85, 1.0, sweeps, threads)
-    log(f"cpu baseline: {sweeps} sweeps in {secs:.2f}s (+{time.time() - t - secs:.1f}s preprocessing)")
+    log(f"cpu baseline: {sweeps} sweeps in {secs:.2f}s on {threads} threads "
+        f"(+{time.time() - t - secs:.1f}s preprocessing); host {cpus}")
     return {"value": len(hs) * sweeps / secs, "unit": "edges/s/iter", "cores": threads, "kind": "port",
             "sample": f"same scale-22 graph, {sweeps} sweeps of the oracle's OpenMP restatement of the "
-                      f"reference PSW sweep (shards preloaded, {nshards} dst-interval shards)"}
+                      f"reference PSW sweep (shards preloaded, {nshards} dst-interval shards), one thread per "
+                      f"usable host CPU",
+            "host": cpus}
 
 
 def main():
@@ -342,7 +388,7 @@ def main():
     ap.add_argument("--only", default="", help="comma list of sections: pagerank,load,ppr,sort,cc,tc,multigraph")
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--ppr-plain-queries", type=int, default=64)
-    ap.add_argument("--ppr-check", type=int, default=1)
+    ap.add_argument("--ppr-check", type=int, default=8)
     ap.add_argument("--cc-scale", type=int, default=24)
     ap.add_argument("--tc-scale", type=int, default=21)
     ap.add_argument("--graphs", type=int, default=2048)

@@ -4,7 +4,8 @@
 // the one-lane-per-row threshold, the launch form (one workgroup per tile,
 // persistent workgroups, or persistent wavefronts), and (N < 2^30) the
 // out-degree gather layout with and without an LDS cache of the most
-// referenced source values.  Each candidate gets its own plan on the caller's graph and is
+// referenced source values, and the rank-space tiled sweep (form 4).  Each
+// candidate gets its own plan on the caller's graph and is
 // timed with HIP events over a few sweeps after a warm-up sweep; the fastest
 // wins.  Results are cached per (N, E) in the process (a plan depends on the
 // graph, a tune only on its shape and degree distribution).
@@ -21,6 +22,9 @@ int pr_init(const gchi_pr_plan *p, float init, const uint32_t *seed_bits, float 
             hipStream_t s);
 int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
              float *r_out, double *d_stats, hipStream_t s);
+int pr_init_rank_space(const gchi_pr_plan *p, float init, hipStream_t s);
+int pr_sweep_rank_space(const gchi_pr_plan *p, const gchi_pr_op *op, int src, int dst, double *d_stats,
+                        hipStream_t s);
 
 static std::mutex g_mu;
 static std::map<std::pair<uint32_t, uint64_t>, std::pair<gchi_pr_tune, double>> g_cache;
@@ -60,6 +64,8 @@ extern "C" int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, 
     if (layout_ok)
         for (int W : {1024, 2048})
             for (int sm : {32, 64}) cand.push_back(gchi_pr_tune{W, sm, 3, 1 << 20});
+    if (g->nvertices < (1u << 31))
+        for (int hot : {0, 16384, 1 << 20}) cand.push_back(gchi_pr_tune{512, 0, 4, hot});
     const size_t N = g->nvertices;
     float *buf = nullptr;
     double *st = nullptr;
@@ -80,13 +86,20 @@ extern "C" int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, 
         gchi_pr_plan *p = nullptr;
         int rc = gchi_pr_plan_create(g, &c, &p, stream);
         if (rc != GCHI_OK) continue;  // e.g. too little memory for this shape: skip it
-        rc = pr_init(p, 1.0f, nullptr, 0.0f, x[0], r[0], s);
-        if (rc == GCHI_OK) rc = pr_sweep(p, &op, x[0], r[0], x[1], r[1], st, s);  // warm-up
+        // form 4 is timed as a HIP-graph exec runs it: sweeps in rank space
+        gchi_pr_plan_info info;
+        gchi_pr_plan_info_get(p, &info);
+        const bool rank_space = info.persistent == 4;
+        auto sweep = [&](int k) {
+            return rank_space ? pr_sweep_rank_space(p, &op, k & 1, (k + 1) & 1, st, s)
+                              : pr_sweep(p, &op, x[k & 1], r[k & 1], x[(k + 1) & 1], r[(k + 1) & 1], st, s);
+        };
+        rc = rank_space ? pr_init_rank_space(p, 1.0f, s) : pr_init(p, 1.0f, nullptr, 0.0f, x[0], r[0], s);
+        if (rc == GCHI_OK) rc = sweep(0);  // warm-up
         float ms = 0.0f;
         if (rc == GCHI_OK) {
             (void)hipEventRecord(e0, s);
-            for (int k = 0; k < reps && rc == GCHI_OK; k++)
-                rc = pr_sweep(p, &op, x[(k + 1) & 1], r[(k + 1) & 1], x[k & 1], r[k & 1], st, s);
+            for (int k = 1; k <= reps && rc == GCHI_OK; k++) rc = sweep(k);
             (void)hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) rc = fail(GCHI_ERR_HIP, "autotune_pr: event sync failed");
             (void)hipEventElapsedTime(&ms, e0, e1);

@@ -22,6 +22,9 @@
 //       each, atomicMin on the row's label), labels updated in place -- so a
 //       drop is seen by rows evaluated later in the same round, as add_task's
 //       also_this_iteration does.  Same fixed point as the other forms.
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -75,6 +78,107 @@ __global__ void cc_hook(const uint32_t *__restrict__ src, const uint32_t *__rest
     }
 }
 
+// The union of cc_hook for one pair: roots hooked under smaller roots by CAS.
+GCHI_DEV void uf_link(uint32_t *parent, uint32_t u, uint32_t v) {
+    u = uf_find(parent, u);
+    v = uf_find(parent, v);
+    while (u != v) {
+        if (u < v) {
+            uint32_t t = u;
+            u = v;
+            v = t;
+        }
+        uint32_t old = atomicCAS(&parent[u], u, v);
+        if (old == u) break;
+        u = uf_find(parent, old);
+        v = uf_find(parent, v);
+    }
+}
+
+// ---- subgraph sampling (Afforest: Sutton, Ben-Nun, Barak, "Optimizing
+// Parallel Graph Connectivity Computation via Subgraph Sampling", IPDPS 2018)
+// over a symmetric CSR: link every vertex to its neighbour r for r < kAffRounds,
+// compress, find the component that most sampled vertices belong to (nearly
+// always the giant one), then link the remaining neighbours of the vertices
+// outside it only.  An edge between that component and another vertex is
+// linked from the other vertex's row (the CSR holds it both ways), so the sets
+// are those of the full union-find: every root is its set's smallest id.
+constexpr uint32_t kAffRounds = 2;
+constexpr uint32_t kAffSamples = 1024;
+constexpr uint32_t kAffLong = 256;  // remaining neighbours above this: the whole wavefront links them
+
+__global__ void cca_sample(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n,
+                           uint32_t r, uint32_t *parent, unsigned long long *bad) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t b = off[v], e = off[v + 1];
+        if (e - b <= r) continue;
+        const uint32_t u = adj[b + r];
+        if (u >= n) {
+            atomicAdd(bad, 1ull);
+            continue;
+        }
+        uf_link(parent, v, u);
+    }
+}
+
+// parent[v] = its root, comp[v] = the same (a snapshot for the skip test)
+__global__ void cca_flatten(uint32_t *parent, uint32_t *comp, uint32_t n) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        uint32_t r = v, p = parent[v];
+        while (p != r) {
+            r = p;
+            p = parent[r];
+        }
+        parent[v] = r;
+        comp[v] = r;
+    }
+}
+
+// roots of kAffSamples vertices spread over [0, n) (multiplicative hash)
+__global__ void cca_samples(const uint32_t *comp, uint32_t n, uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kAffSamples) out[i] = comp[(uint32_t)(((uint64_t)i * 2654435761ull + 12345u) % n)];
+}
+
+// The neighbours from kAffRounds on of every vertex outside component c; rows
+// with more than kAffLong of them are linked by the whole wavefront.
+__global__ void cca_rest(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n,
+                         const uint32_t *__restrict__ comp, uint32_t c, uint32_t *parent, unsigned long long *bad) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1)); base < n;
+         base += stride) {
+        const uint64_t v = base + lane;
+        bool is_long = false;
+        if (v < n && comp[v] != c) {
+            const uint32_t b = off[v] + kAffRounds, e = off[v + 1];
+            if (e > b && e - b > kAffLong) {
+                is_long = true;
+            } else {
+                for (uint32_t k = b; k < e; k++) {
+                    const uint32_t u = adj[k];
+                    if (u < n) uf_link(parent, (uint32_t)v, u);
+                    else atomicAdd(bad, 1ull);
+                }
+            }
+        }
+        uint64_t m = __ballot(is_long);
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t w = (uint32_t)(base + j);
+            const uint32_t b = off[w] + kAffRounds, e = off[w + 1];
+            for (uint32_t k = b + lane; k < e; k += kWave) {
+                const uint32_t u = adj[k];
+                if (u < n) uf_link(parent, w, u);
+                else atomicAdd(bad, 1ull);
+            }
+        }
+    }
+}
+
 __global__ void cc_compress(uint32_t *parent, uint32_t *labels, uint32_t n) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
@@ -111,6 +215,61 @@ __global__ void cc_count_changed(const uint32_t *lin, const uint32_t *lout, uint
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) c += lin[v] != lout[v];
     c = wave_sum(c);
     if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd((unsigned long long *)changed, (unsigned long long)c);
+}
+
+// One synchronous sweep pulled over a symmetric CSR: l_out[v] = min(l_in[v],
+// l_in[u] for u in adj(v)), one write per vertex.  Rows up to kPullShort
+// entries: one lane each, reading its row in order; longer rows: the whole
+// wavefront, lane-strided, then a min across lanes.  *changed += labels that
+// dropped (one atomic per wavefront).
+constexpr uint32_t kPullShort = 32;
+
+__global__ void cc_pull_sweep(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n,
+                              const uint32_t *__restrict__ lin, uint32_t *__restrict__ lout,
+                              unsigned long long *changed) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t dropped = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1)); base < n;
+         base += stride) {
+        const uint64_t v = base + lane;
+        bool is_long = false;
+        if (v < n) {
+            const uint32_t b = off[v], e = off[v + 1];
+            uint32_t m = lin[v];
+            if (e - b > kPullShort) {
+                is_long = true;
+            } else {
+                for (uint32_t k = b; k < e; k++) {
+                    const uint32_t u = adj[k];
+                    m = min(m, lin[u < n ? u : (uint32_t)v]);
+                }
+                lout[v] = m;
+                dropped += m != lin[v];
+            }
+        }
+        uint64_t mask = __ballot(is_long);
+        while (mask) {
+            const int j = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t w = (uint32_t)(base + j);
+            const uint32_t b = off[w], e = off[w + 1];
+            const uint32_t own = lin[w];
+            uint32_t m = own;
+            for (uint32_t k = b + lane; k < e; k += kWave) {
+                const uint32_t u = adj[k];
+                m = min(m, lin[u < n ? u : w]);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
+            if (lane == j) {
+                lout[w] = m;
+                dropped += m != own;
+            }
+        }
+    }
+    dropped = wave_sum(dropped);
+    if (lane == 0 && dropped) atomicAdd(changed, (unsigned long long)dropped);
 }
 
 __global__ void cc_iota(uint32_t *l, uint32_t n) {
@@ -333,6 +492,62 @@ extern "C" int gchi_cc_minlabel_ex(const uint32_t *d_src, const uint32_t *d_dst,
     return GCHI_OK;
 }
 
+extern "C" int gchi_cc_afforest_workspace(uint32_t nvertices, size_t *bytes) {
+    GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
+    Sizer z;
+    z.take<uint32_t>(nvertices ? nvertices : 1);  // parent
+    z.take<uint32_t>(kAffSamples);
+    z.take<unsigned long long>(1);
+    *bytes = z.off;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_cc_afforest(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_labels,
+                                void *d_work, size_t work_bytes, void *stream) {
+    GCHI_CHECK(d_off && d_labels && nvertices > 0, GCHI_ERR_INVALID, "cc_afforest: null argument");
+    size_t need = 0;
+    gchi_cc_afforest_workspace(nvertices, &need);
+    GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "cc_afforest: workspace too small");
+    hipStream_t s = as_stream(stream);
+    const uint32_t n = nvertices;
+    Carver c(d_work, work_bytes);
+    uint32_t *parent = c.take<uint32_t>(n);
+    uint32_t *samp = c.take<uint32_t>(kAffSamples);
+    unsigned long long *bad = c.take<unsigned long long>(1);
+    uint32_t *comp = d_labels;  // the snapshot lives in the output until the last pass
+    const unsigned gv = grid_for(n, 256, 65536);
+    launch_fill_u64(reinterpret_cast<uint64_t *>(bad), 0ull, 1, s);
+    hipLaunchKernelGGL(cc_init, dim3(gv), dim3(256), 0, s, parent, n);
+    for (uint32_t r = 0; r < kAffRounds; r++)
+        hipLaunchKernelGGL(cca_sample, dim3(gv), dim3(256), 0, s, d_off, d_adj, n, r, parent, bad);
+    hipLaunchKernelGGL(cca_flatten, dim3(gv), dim3(256), 0, s, parent, comp, n);
+    hipLaunchKernelGGL(cca_samples, dim3(kAffSamples / 256), dim3(256), 0, s, comp, n, samp);
+    GCHI_LAUNCH_CHECK();
+    std::vector<uint32_t> h(kAffSamples);
+    GCHI_HIP(hipMemcpyAsync(h.data(), samp, 4 * (size_t)kAffSamples, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    std::sort(h.begin(), h.end());
+    uint32_t best = h[0], best_n = 0;
+    for (size_t i = 0; i < h.size();) {
+        size_t j = i;
+        while (j < h.size() && h[j] == h[i]) j++;
+        if (j - i > best_n) {
+            best_n = (uint32_t)(j - i);
+            best = h[i];
+        }
+        i = j;
+    }
+    hipLaunchKernelGGL(cca_rest, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, d_off, d_adj, n, comp, best, parent,
+                       bad);
+    hipLaunchKernelGGL(cc_compress, dim3(gv), dim3(256), 0, s, parent, d_labels, n);
+    GCHI_LAUNCH_CHECK();
+    unsigned long long h_bad = 0;
+    GCHI_HIP(hipMemcpyAsync(&h_bad, bad, 8, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(h_bad == 0, GCHI_ERR_INVALID, "cc_afforest: neighbour id >= nvertices");
+    return GCHI_OK;
+}
+
 extern "C" int gchi_cc_minlabel(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                                 uint32_t *d_labels, void *d_work, size_t work_bytes, void *stream) {
     return gchi_cc_minlabel_ex(d_src, d_dst, nedges, nvertices, 0, d_labels, d_work, work_bytes, stream);
@@ -365,6 +580,19 @@ extern "C" int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_
     return GCHI_OK;
 }
 
+extern "C" int gchi_cc_pull_sweep(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices,
+                                  const uint32_t *d_labels_in, uint32_t *d_labels_out, uint64_t *d_changed,
+                                  void *stream) {
+    GCHI_CHECK(d_off && d_labels_in && d_labels_out && d_changed && nvertices > 0 && d_labels_in != d_labels_out,
+               GCHI_ERR_INVALID, "cc_pull_sweep: bad argument");
+    hipStream_t s = as_stream(stream);
+    launch_fill_u64(d_changed, 0ull, 1, s);
+    hipLaunchKernelGGL(cc_pull_sweep, dim3(grid_for(nvertices, 256, 16384)), dim3(256), 0, s, d_off, d_adj, nvertices,
+                       d_labels_in, d_labels_out, reinterpret_cast<unsigned long long *>(d_changed));
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
 extern "C" int gchi_cc_scheduled_workspace(uint32_t nvertices, uint64_t nedges, size_t *bytes) {
     GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
     SizerAdapter a;
@@ -373,9 +601,10 @@ extern "C" int gchi_cc_scheduled_workspace(uint32_t nvertices, uint64_t nedges, 
     return GCHI_OK;
 }
 
-extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
-                                 uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, void *d_work,
-                                 size_t work_bytes, void *stream) {
+extern "C" int gchi_cc_scheduled_ex(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices,
+                                    uint32_t max_rounds, uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated,
+                                    gchi_cc_round_fn on_round, void *user, void *d_work, size_t work_bytes,
+                                    void *stream) {
     GCHI_CHECK(d_off && d_labels && h_rounds && nvertices > 0, GCHI_ERR_INVALID, "cc_scheduled: null argument");
     hipStream_t s = as_stream(stream);
     uint32_t e32 = 0;
@@ -406,7 +635,7 @@ extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, u
     const uint64_t limit = max_rounds ? max_rounds : (uint64_t)n + 1;
     uint32_t rounds = 0;
     uint64_t evaluated = 0;
-    bool fixed = false;
+    bool fixed = false, stopped = false;
     int cur = 0;
     while (rounds < limit) {
         uint8_t *fc = w.flags[cur], *fn = w.flags[cur ^ 1];
@@ -429,14 +658,23 @@ extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, u
         GCHI_CHECK(st[2] == 0, GCHI_ERR_INVALID, "cc_scheduled: neighbour id >= nvertices");
         rounds++;
         evaluated += st[1];
-        if (st[0] == 0) {
-            fixed = true;
+        if (st[0] == 0) fixed = true;
+        if (on_round && on_round(user, rounds - 1, st[0], st[1]) != 0) {
+            stopped = true;
             break;
         }
+        if (fixed) break;
         cur ^= 1;
     }
     *h_rounds = rounds;
     if (h_evaluated) *h_evaluated = evaluated;
-    GCHI_CHECK(fixed, GCHI_ERR_INVALID, "cc_scheduled: no fixed point within max_rounds");
+    GCHI_CHECK(fixed || stopped, GCHI_ERR_INVALID, "cc_scheduled: no fixed point within max_rounds");
     return GCHI_OK;
+}
+
+extern "C" int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
+                                 uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, void *d_work,
+                                 size_t work_bytes, void *stream) {
+    return gchi_cc_scheduled_ex(d_off, d_adj, nvertices, max_rounds, d_labels, h_rounds, h_evaluated, nullptr, nullptr,
+                                d_work, work_bytes, stream);
 }

@@ -90,10 +90,10 @@ template <typename C>
 static void csc_layout(C &c, uint64_t E, uint32_t N, uint64_t **keys, uint32_t **indeg, uint32_t **run_start,
                        uint32_t **err, void **sw, size_t *sw_bytes) {
     const CscBits cb = csc_bits(N);
+    *err = c.template take<uint32_t>(1);  // status word (GCHI_WORK_* bits) first: gchi_work_status reads it
     *keys = c.template take<uint64_t>(E ? E : 1);
     *indeg = c.template take<uint32_t>(N ? N : 1);
     *run_start = c.template take<uint32_t>(N ? N : 1);
-    *err = c.template take<uint32_t>(1);
     size_t ss = std::max(sort_workspace_bytes(E, std::max(cb.sb, cb.vb), false), scan_workspace((size_t)N + 1));
     *sw = c.template take<char>(ss);
     *sw_bytes = ss;
@@ -122,12 +122,12 @@ int csc_build(const uint32_t *src, const uint32_t *dst, uint64_t E, uint32_t N, 
         GCHI_LAUNCH_CHECK();
         // src passes: keys ordered by src; out-degrees from its runs
         uint64_t *by_src = nullptr, *by_dst = nullptr;
-        GCHI_TRY(radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, cb.sb, sw, swb, s, &by_src));
+        GCHI_TRY(radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, cb.sb, sw, swb, s, &by_src, err));
         if (outdeg) GCHI_TRY(count_field_runs(by_src, E, 0, vmask, N, run_start, outdeg, nullptr, 0u, s));
         // dst passes (stable, so src order within a dst run is kept); the
         // buffer not holding by_src takes the other half of the ping-pong
         uint64_t *other = by_src == keys ? nullptr : keys;
-        GCHI_TRY(radix_sort_u64_bits(by_src, other, nullptr, E, cb.sb, cb.sb + cb.vb, sw, swb, s, &by_dst));
+        GCHI_TRY(radix_sort_u64_bits(by_src, other, nullptr, E, cb.sb, cb.sb + cb.vb, sw, swb, s, &by_dst, err));
         GCHI_TRY(count_field_runs(by_dst, E, cb.sb, vmask, N, run_start, ind, in_src, vmask, s));
     }
     GCHI_TRY(exclusive_scan_u32(ind, in_off, N, sw, swb, s));

@@ -90,6 +90,7 @@ struct gchi_pr_plan {
     uint32_t *gorder = nullptr;  // [A] vertex at each layout position
     float *xg = nullptr;         // [A] x in layout order, refreshed by every sweep
     void *mem = nullptr;
+    gchi_seg_plan *seg = nullptr;  // launch form 4: the rank-space tiled plan (pr_seg.hip) instead of the above
 };
 
 namespace gchi {
@@ -1016,6 +1017,7 @@ struct CscView {
     const uint32_t *vbase;    // [G+1]
     uint32_t G, N;
     int local_ids;            // ids are per graph (checked against its vertex count)
+    uint32_t src_lim;         // bound of stored ids for a union CSC (N, or more for a row slice)
 };
 
 GCHI_DEV uint32_t view_deg(const CscView &cv, uint32_t v, uint32_t g) {
@@ -1110,7 +1112,7 @@ __global__ void plan_copy(CscView cv, const uint32_t *hub_flag, const uint32_t *
         const uint32_t b = cv.off_ext[v + g], e = cv.off_ext[v + g + 1];
         const uint32_t *src = reinterpret_cast<const uint32_t *>((uintptr_t)cv.g_src[g]);
         const uint32_t idoff = cv.g_idoff[g];
-        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.N;
+        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.src_lim;
         uint32_t *dst = reg_src + ebase[g] + reg_rel[v + g];
         for (uint32_t k = b + lane; k < e; k += kWave) dst[k - b] = plan_map_src(src[k], idoff, lim, grank, bad);
     }
@@ -1134,7 +1136,7 @@ __global__ void plan_copy_hubs(CscView cv, const uint32_t *hub_v, const uint32_t
         const uint32_t b = cv.off_ext[v + g] + j * kHubChunk;
         const uint32_t *src = reinterpret_cast<const uint32_t *>((uintptr_t)cv.g_src[g]) + b;
         const uint32_t idoff = cv.g_idoff[g];
-        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.N;
+        const uint32_t lim = cv.local_ids ? cv.vbase[g + 1] - cv.vbase[g] : cv.src_lim;
         uint32_t *dst = hub_src + chunk_beg[c];
         const uint32_t len = chunk_len[c];
         for (uint32_t k = lane; k < len; k += kWave) dst[k] = plan_map_src(src[k], idoff, lim, grank, bad);
@@ -1276,6 +1278,7 @@ struct PlanInput {
     const uint32_t *h_vbase = nullptr;  // [G+1]
     const gchi_dgraph *unionc = nullptr;
     const gchi_dgraph *graphs = nullptr;  // [G]
+    uint32_t src_limit = 0;  // single-graph plans over a slice of rows: sources may be any id < src_limit
 };
 
 static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s) {
@@ -1326,7 +1329,7 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
         GCHI_HIP(hipMemcpyAsync(g_idoff, in.h_vbase, 4 * (size_t)G, hipMemcpyHostToDevice, s));
         GCHI_HIP(hipStreamSynchronize(s));  // h_src is a host temporary
     }
-    const CscView cv{off_ext, g_src, g_idoff, vbase, G, N, in.unionc ? 0 : 1};
+    const CscView cv{off_ext, g_src, g_idoff, vbase, G, N, in.unionc ? 0 : 1, in.src_limit ? in.src_limit : N};
     launch_fill_u32(bits, 0u, nwords, s);
     hipLaunchKernelGGL(plan_rows, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, cv, W, reg_deg, hub_flag, bits);
     GCHI_LAUNCH_CHECK();
@@ -1392,6 +1395,7 @@ static int plan_build(const PlanInput &in, int W, gchi_pr_plan *p, hipStream_t s
         hipLaunchKernelGGL(plan_order_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, p->outdeg, N, keys);
         GCHI_LAUNCH_CHECK();
         GCHI_TRY(gchi_sort_u64(keys, nullptr, N, 64, sws, sb, s));
+        GCHI_TRY(read_work_status(static_cast<const uint32_t *>(sws), s));
         hipLaunchKernelGGL(plan_gather_rank, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, keys, N, gorder_tmp,
                            grank, dA);
         GCHI_LAUNCH_CHECK();
@@ -1510,9 +1514,24 @@ static int plan_create(const PlanInput &in, const gchi_pr_tune *tune, gchi_pr_pl
     p->short_max = short_max;
     p->persistent = tune ? tune->persistent : 0;
     p->hot = tune ? tune->hot_vertices : 0;
-    if (p->persistent < 0 || p->persistent > 3) {
+    if (p->persistent < 0 || p->persistent > 4) {
         delete p;
-        return fail(GCHI_ERR_INVALID, "pr_plan_create: persistent must be 0, 1, 2 or 3");
+        return fail(GCHI_ERR_INVALID, "pr_plan_create: persistent must be 0, 1, 2, 3 or 4");
+    }
+    if (p->persistent == 4) {
+        // the rank-space tiled sweep: one graph given as one CSC
+        if (G != 1 || !in.unionc || in.src_limit) {
+            delete p;
+            return fail(GCHI_ERR_UNSUPPORTED, "pr_plan_create: launch form 4 takes one graph (gchi_pr_plan_create)");
+        }
+        p->outdeg = in.unionc->outdeg;
+        const int st4 = seg_plan_create(in.unionc, p->hot, &p->seg, s);
+        if (st4 != GCHI_OK) {
+            delete p;
+            return st4;
+        }
+        *out = p;
+        return GCHI_OK;
     }
     // the LDS cache lives in the persistent kernels 2 and 3 (default 2)
     if (p->hot > 0 && p->persistent != 3) p->persistent = 2;
@@ -1596,6 +1615,7 @@ int pr_init(const gchi_pr_plan *p, float init, const uint32_t *seed_bits, float 
 
 int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, const float *r_in, float *x_out,
              float *r_out, double *d_stats, hipStream_t s) {
+    if (p->seg) return seg_sweep(p->seg, op, x_in, r_in, x_out, r_out, d_stats, s);
     PrConsts c{op->teleport, op->damping, op->d_graph_teleport, op->d_seed_bits, op->seed_teleport};
     const float *xg = x_in;
     if (p->hot) {
@@ -1639,6 +1659,18 @@ int pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const float *x_in, con
     return GCHI_OK;
 }
 
+// Form-4 plans timed in the state a HIP-graph exec keeps them in (rank space).
+int pr_init_rank_space(const gchi_pr_plan *p, float init, hipStream_t s) {
+    GCHI_CHECK(p->seg, GCHI_ERR_INVALID, "pr_init_rank_space: not a form-4 plan");
+    return seg_init(p->seg, init, nullptr, 0.0f, 0, s);
+}
+
+int pr_sweep_rank_space(const gchi_pr_plan *p, const gchi_pr_op *op, int src, int dst, double *d_stats,
+                        hipStream_t s) {
+    GCHI_CHECK(p->seg, GCHI_ERR_INVALID, "pr_sweep_rank_space: not a form-4 plan");
+    return seg_sweep_rank(p->seg, op, src, dst, d_stats, s);
+}
+
 struct PlanView {
     uint32_t N, nblk, nhub, nchunk, npersist;
     const uint32_t *reg_off, *reg_src, *blk_row, *hub_bits, *hub_v, *hub_chunk_off;
@@ -1651,6 +1683,8 @@ struct PlanView {
 int pr_plan_view(const gchi_pr_plan *p, PlanView *v) {
     GCHI_CHECK(p && v, GCHI_ERR_INVALID, "plan view: null argument");
     GCHI_CHECK(p->G == 1, GCHI_ERR_UNSUPPORTED, "plan view: needs a single-graph plan");
+    GCHI_CHECK(!p->seg, GCHI_ERR_UNSUPPORTED, "plan view: launch form 4 keeps the graph in rank order; build the "
+                                              "plan with persistent 0-3");
     GCHI_CHECK(!p->hot, GCHI_ERR_UNSUPPORTED, "plan view: the plan's edges hold gather-layout positions "
                                               "(hot_vertices != 0); build it with hot_vertices = 0");
     *v = PlanView{p->N, p->nblk, p->nhub, p->nchunk, p->npersist, p->reg_off, p->reg_src, p->blk_row, p->hub_bits,
@@ -1718,6 +1752,7 @@ extern "C" int gchi_pr_plan_create_batch(const gchi_dgraph *graphs, uint32_t ngr
 
 extern "C" int gchi_pr_plan_destroy(gchi_pr_plan *p) {
     if (!p) return GCHI_OK;
+    seg_plan_destroy(p->seg);
     if (p->mem) (void)hipFree(p->mem);
     if (p->own_outdeg) (void)hipFree(p->own_outdeg);
     delete p;
@@ -1744,6 +1779,22 @@ extern "C" int gchi_pr_plan_info_get(const gchi_pr_plan *p, gchi_pr_plan_info *i
     info->hot_vertices = p->H;
     info->gather_sources = p->A;
     info->hot_threads = p->persistent == 2 ? (uint32_t)kWaveThreads : p->persistent == 3 ? (uint32_t)kPrThreads : 0u;
+    if (p->seg) {  // tiles as blocks, split rows as hub rows, every slot as a regular edge
+        uint32_t tiles = 0, nsplit = 0, hot = 0, nwg = 0;
+        uint64_t slots = 0;
+        seg_plan_info(p->seg, &tiles, &nsplit, &slots, &hot, &nwg);
+        info->edges_per_block = 512;
+        info->short_row_max = 0;
+        info->regular_blocks = tiles;
+        info->hub_rows = nsplit;
+        info->hub_chunks = 0;
+        info->regular_edges = slots;
+        info->hub_edges = 0;
+        info->persistent_workgroups = nwg;
+        info->hot_vertices = hot;
+        info->gather_sources = p->N;
+        info->hot_threads = 1024;
+    }
     return GCHI_OK;
 }
 
@@ -1765,6 +1816,194 @@ extern "C" int gchi_pr_sweep(const gchi_pr_plan *p, const gchi_pr_op *op, const 
     GCHI_CHECK(d_x_in != d_x_out && d_r_in != d_r_out, GCHI_ERR_INVALID, "pr_sweep: in and out buffers must differ");
     GCHI_CHECK(p->G == 1 || op->d_graph_teleport || op->teleport >= 0, GCHI_ERR_INVALID, "pr_sweep: bad operator");
     return pr_sweep(p, op, d_x_in, d_r_in, d_x_out, d_r_out, d_stats, as_stream(stream));
+}
+
+// ------------------------------------------------------ semi-synchronous sweeps
+// The functional engine's semisync mode (functional_semisync.hpp:76-110 under
+// the interval loop of graphchi_engine.hpp:825-921): the vertex blocks (the
+// engine's sub-intervals) run in order within a sweep, and block j gathers the
+// x written this sweep by earlier blocks and the previous sweep's x of the
+// rest.  Each block has its own plan over its rows (sources keep their global
+// ids), and x lives in one buffer: block j's sweep reads it whole, writes its
+// new x to a scratch copy, and a copy kernel then moves that slice into x, so
+// rows of block j never see each other's new values.
+namespace gchi {
+
+__global__ void blk_slice_offsets(const uint32_t *__restrict__ in_off, uint32_t a, uint32_t n, uint32_t *out) {
+    const uint32_t base = in_off[a];
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += stride) out[k] = in_off[a + k] - base;
+}
+
+__global__ void blk_copy_f32(const float *__restrict__ src, float *__restrict__ dst, uint32_t n) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+}
+
+// out bit k = bit a+k of bits, k < n (one word per thread)
+__global__ void blk_shift_bits(const uint32_t *__restrict__ bits, uint32_t a, uint32_t n, uint32_t *out) {
+    const uint32_t words = (n + 31) / 32;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        const uint64_t first = (uint64_t)a + 32ull * w;  // global index of bit 0 of this word
+        const uint32_t q = (uint32_t)(first >> 5), r = (uint32_t)(first & 31);
+        const uint32_t left = n - 32 * w, take = left < 32 ? left : 32;
+        uint32_t word = bits[q] >> r;
+        // the next global word holds bits of this one only when they are inside
+        // [a, a + n), so it exists
+        if (r && r + take > 32) word |= bits[q + 1] << (32 - r);
+        if (take < 32) word &= (1u << take) - 1u;
+        out[w] = word;
+    }
+}
+
+// d_stats[0..1] = sums of the blocks' statistics, in block order
+__global__ void blk_stats_sum(const double *__restrict__ bst, uint32_t nb, double *out) {
+    if (blockIdx.x || threadIdx.x) return;
+    double r = 0.0, t = 0.0;
+    for (uint32_t j = 0; j < nb; j++) {
+        r += bst[2 * j];
+        t += bst[2 * j + 1];
+    }
+    out[0] = r;
+    out[1] = t;
+}
+
+}  // namespace gchi
+
+struct gchi_pr_blocked {
+    uint32_t N = 0, nb = 0;
+    std::vector<uint32_t> bstart;        // [nb+1]
+    std::vector<gchi_pr_plan *> plans;   // [nb]
+    std::vector<uint32_t *> seed_local;  // [nb] block-local seed bitmaps
+    const uint32_t *outdeg = nullptr;
+    float *xnew = nullptr;
+    double *bstat = nullptr;
+    void *mem = nullptr;
+};
+
+namespace gchi {
+
+static void blocked_free(gchi_pr_blocked *b) {
+    for (gchi_pr_plan *p : b->plans) gchi_pr_plan_destroy(p);
+    if (b->mem) (void)hipFree(b->mem);
+    delete b;
+}
+
+}  // namespace gchi
+
+extern "C" int gchi_pr_blocked_create(const gchi_dgraph *g, const uint32_t *h_bstart, uint32_t nblocks,
+                                      const gchi_pr_tune *tune, gchi_pr_blocked **out, void *stream) {
+    GCHI_TRY(check_csc(g));
+    GCHI_CHECK(h_bstart && out && nblocks >= 1, GCHI_ERR_INVALID, "pr_blocked_create: null argument or no blocks");
+    const uint32_t N = g->nvertices;
+    GCHI_CHECK(h_bstart[0] == 0 && h_bstart[nblocks] == N, GCHI_ERR_INVALID,
+               "pr_blocked_create: blocks must start at 0 and end at N");
+    for (uint32_t j = 0; j < nblocks; j++)
+        GCHI_CHECK(h_bstart[j + 1] > h_bstart[j], GCHI_ERR_INVALID, "pr_blocked_create: empty or unordered block");
+    hipStream_t s = as_stream(stream);
+    gchi_pr_tune t = tune ? *tune : gchi_pr_tune{0, 0, 0, 0};
+    t.hot_vertices = 0;  // block plans gather by vertex id (a gather layout would rank each slice's rows only)
+    if (t.persistent == 3) t.persistent = 2;
+    if (t.persistent == 2 && t.edges_per_block == 0) t.edges_per_block = 256;
+    gchi_pr_blocked *b = new gchi_pr_blocked();
+    b->N = N;
+    b->nb = nblocks;
+    b->bstart.assign(h_bstart, h_bstart + nblocks + 1);
+    b->outdeg = g->outdeg;
+    std::vector<uint32_t> eoff(nblocks + 1);
+    Sizer z;
+    z.take<float>(N);
+    z.take<double>(2 * (size_t)nblocks);
+    for (uint32_t j = 0; j < nblocks; j++) {
+        z.take<uint32_t>((size_t)(h_bstart[j + 1] - h_bstart[j]) + 1);
+        z.take<uint32_t>((size_t)(h_bstart[j + 1] - h_bstart[j] + 31) / 32);
+    }
+    if (hipMalloc(&b->mem, z.off) != hipSuccess) {
+        b->mem = nullptr;
+        blocked_free(b);
+        return fail(GCHI_ERR_HIP, "pr_blocked_create: hipMalloc failed");
+    }
+    Carver c(b->mem, z.off);
+    b->xnew = c.take<float>(N);
+    b->bstat = c.take<double>(2 * (size_t)nblocks);
+    std::vector<uint32_t *> offs(nblocks);
+    for (uint32_t j = 0; j < nblocks; j++) {
+        offs[j] = c.take<uint32_t>((size_t)(h_bstart[j + 1] - h_bstart[j]) + 1);
+        b->seed_local.push_back(c.take<uint32_t>((size_t)(h_bstart[j + 1] - h_bstart[j] + 31) / 32));
+    }
+    int st = GCHI_OK;
+    for (uint32_t j = 0; j <= nblocks && st == GCHI_OK; j++)
+        st = read_u32(g->in_off + h_bstart[j], &eoff[j], s);
+    for (uint32_t j = 0; j < nblocks && st == GCHI_OK; j++) {
+        const uint32_t a = h_bstart[j], nj = h_bstart[j + 1] - a;
+        hipLaunchKernelGGL(blk_slice_offsets, dim3(grid_for((uint64_t)nj + 1, 256, 65536)), dim3(256), 0, s,
+                           g->in_off, a, nj, offs[j]);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            st = hip_fail(e, "pr_blocked_create", __FILE__, __LINE__);
+            break;
+        }
+        const gchi_dgraph gj{nj, 0, (uint64_t)(eoff[j + 1] - eoff[j]), offs[j], g->in_src ? g->in_src + eoff[j] : nullptr,
+                             g->outdeg + a, g->indeg ? g->indeg + a : nullptr};
+        uint32_t vb[2] = {0, nj};
+        PlanInput in;
+        in.N = nj;
+        in.E = gj.nedges;
+        in.h_vbase = vb;
+        in.unionc = &gj;
+        in.src_limit = N;
+        gchi_pr_plan *pj = nullptr;
+        st = plan_create(in, &t, &pj, s);
+        if (st == GCHI_OK) b->plans.push_back(pj);
+    }
+    if (st == GCHI_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "pr_blocked_create", __FILE__, __LINE__);
+    }
+    if (st != GCHI_OK) {
+        blocked_free(b);
+        return st;
+    }
+    *out = b;
+    return GCHI_OK;
+}
+
+extern "C" int gchi_pr_blocked_destroy(gchi_pr_blocked *b) {
+    if (b) blocked_free(b);
+    return GCHI_OK;
+}
+
+extern "C" int gchi_pr_blocked_init(const gchi_pr_blocked *b, float init, const uint32_t *d_seed_bits,
+                                    float seed_value, float *d_x, float *d_r, void *stream) {
+    GCHI_CHECK(b && d_x && d_r, GCHI_ERR_INVALID, "pr_blocked_init: null argument");
+    hipLaunchKernelGGL(pr_init_kernel, dim3(grid_for(b->N, 256, 65536)), dim3(256), 0, as_stream(stream), b->outdeg,
+                       b->N, init, d_seed_bits, seed_value, d_x, d_r);
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
+extern "C" int gchi_pr_blocked_sweep(const gchi_pr_blocked *b, const gchi_pr_op *op, float *d_x, const float *d_r_in,
+                                     float *d_r_out, double *d_stats, void *stream) {
+    GCHI_CHECK(b && op && d_x && d_r_in && d_r_out && d_stats, GCHI_ERR_INVALID, "pr_blocked_sweep: null argument");
+    GCHI_CHECK(d_r_in != d_r_out, GCHI_ERR_INVALID, "pr_blocked_sweep: rank buffers must differ");
+    GCHI_CHECK(!op->d_graph_teleport, GCHI_ERR_UNSUPPORTED, "pr_blocked_sweep: one graph only (no graph_teleport)");
+    hipStream_t s = as_stream(stream);
+    for (uint32_t j = 0; j < b->nb; j++) {
+        const uint32_t a = b->bstart[j], nj = b->bstart[j + 1] - a;
+        gchi_pr_op oj = *op;
+        if (op->d_seed_bits) {
+            hipLaunchKernelGGL(blk_shift_bits, dim3(grid_for((nj + 31) / 32, 256, 4096)), dim3(256), 0, s,
+                               op->d_seed_bits, a, nj, b->seed_local[j]);
+            oj.d_seed_bits = b->seed_local[j];
+        }
+        GCHI_TRY(pr_sweep(b->plans[j], &oj, d_x, d_r_in + a, b->xnew + a, d_r_out + a, b->bstat + 2 * (size_t)j, s));
+        hipLaunchKernelGGL(blk_copy_f32, dim3(grid_for(nj, 256, 65536)), dim3(256), 0, s, b->xnew + a, d_x + a, nj);
+        GCHI_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(blk_stats_sum, dim3(1), dim3(64), 0, s, b->bstat, b->nb, d_stats);
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
 }
 
 // ---------------------------------------------------------------- graph exec
@@ -1791,6 +2030,19 @@ extern "C" int gchi_pr_exec_create_ex(const gchi_pr_plan *p, const gchi_pr_op *o
         float *xs[2] = {d_x0, d_x1};
         float *rs[2] = {d_r0, d_r1};
         int s1 = GCHI_OK;
+        if (p->seg) {
+            // every sweep in rank space; the caller's buffers only on entry and exit
+            if (do_init)
+                s1 = op->d_seed_bits ? seg_init(p->seg, 0.0f, op->d_seed_bits, (float)op->seed_teleport, 0, cs)
+                                     : seg_init(p->seg, init, nullptr, 0.0f, 0, cs);
+            else
+                s1 = seg_enter(p->seg, xs[0], rs[0], 0, cs);
+            for (int k = 0; k < nsweeps && s1 == GCHI_OK; k++)
+                s1 = seg_sweep_rank(p->seg, op, k & 1, (k + 1) & 1, d_stats_all + 2 * (size_t)k, cs);
+            if (s1 == GCHI_OK) s1 = seg_leave(p->seg, nsweeps & 1, xs[nsweeps & 1], rs[nsweeps & 1], cs);
+            do_init = 0;
+            nsweeps = 0;  // captured above
+        }
         if (do_init) {
             if (op->d_seed_bits)
                 s1 = pr_init(p, 0.0f, op->d_seed_bits, (float)op->seed_teleport, xs[0], rs[0], cs);

@@ -57,11 +57,18 @@ struct gchi_ppr_batch {
     uint32_t *seed_n = nullptr;    // [nqb]
     float *tele = nullptr;         // [nqb*64] restart mass per seed
     uint32_t *counter = nullptr;
+    uint32_t *active = nullptr;    // [nqb] query blocks still swept, then their count at [nqb]
     size_t words = 0;
     int cur = 0;                   // buffer holding the current ranks
     int ncu = 256;
     bool fp32 = false;
     void *mem = nullptr;
+    // chunks of sweeps captured once per phase and replayed (the residual is
+    // read back only after each chunk)
+    hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};  // [0] BF16, [1] FP32
+    hipGraph_t chunk_graph[2] = {nullptr, nullptr};
+    int chunk_len[2] = {0, 0};
+    float chunk_damping[2] = {0.0f, 0.0f};
 };
 
 namespace gchi {
@@ -104,6 +111,7 @@ struct PpArgs {
     float *hub_part;
     double *res;
     uint32_t *counter;
+    const uint32_t *active;  // query blocks still swept; active[nqb] = their count
     size_t words;
     uint32_t nqb;
     float damping;
@@ -257,7 +265,8 @@ __global__ __launch_bounds__(kPpThreads) void ppr_sweep_kernel(PpArgs a, const T
     __shared__ uint32_t s_tile;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint32_t T_per_qb = a.pv.nchunk + a.pv.nblk;
-    const uint32_t ntiles = T_per_qb * a.nqb;
+    const uint32_t nact = a.active[a.nqb];
+    const uint32_t ntiles = T_per_qb * nact;
     const size_t stride_q = (size_t)a.pv.N * kQ;
     uint32_t cur_qb = 0xffffffffu;
     double res = 0.0;  // this lane's query, accumulated by this wave for cur_qb
@@ -277,7 +286,8 @@ __global__ __launch_bounds__(kPpThreads) void ppr_sweep_kernel(PpArgs a, const T
             spent++;
             stripe = (stripe + 1) % kPpStripes;
         }
-        const uint32_t qb = t < ntiles ? t / T_per_qb : 0xffffffffu;
+        const uint32_t qi = t < ntiles ? t / T_per_qb : 0xffffffffu;  // index into the active list
+        const uint32_t qb = t < ntiles ? a.active[qi] : 0xffffffffu;
         if (qb != cur_qb) {  // block-uniform: every thread read the same tile
             if (a.residual && cur_qb != 0xffffffffu) {
                 // add the waves' residuals in wave order, then one atomic per query
@@ -296,7 +306,7 @@ __global__ __launch_bounds__(kPpThreads) void ppr_sweep_kernel(PpArgs a, const T
             cur_qb = qb;
         }
         if (t >= ntiles) break;
-        const uint32_t lt = t - qb * T_per_qb;
+        const uint32_t lt = t - qi * T_per_qb;
         const T *Rin = Rin_all + qb * stride_q;
         T *Rout = Rout_all + qb * stride_q;
         if (lt < a.pv.nchunk) {
@@ -335,13 +345,13 @@ __global__ __launch_bounds__(kPpThreads) void ppr_hub_kernel(PpArgs a, const T *
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint32_t nw = gridDim.x * kPpWaves;
     const size_t stride_q = (size_t)a.pv.N * kQ;
-    const uint32_t items = a.pv.nhub * a.nqb;
+    const uint32_t items = a.pv.nhub * a.active[a.nqb];
     // residuals summed per wavefront while its items stay in one query block,
     // then one atomic per query (not one per hub row and query)
     double racc = 0.0;
     uint32_t rqb = 0xffffffffu;
     for (uint32_t it = blockIdx.x * kPpWaves + wave; it < items; it += nw) {
-        const uint32_t qb = it / a.pv.nhub, h = it % a.pv.nhub;
+        const uint32_t qb = a.active[it / a.pv.nhub], h = it % a.pv.nhub;
         if (a.residual && qb != rqb) {  // wavefront-uniform
             if (rqb != 0xffffffffu && racc != 0.0) atomicAdd(&a.res[rqb * kQ + lane], racc);
             racc = 0.0;
@@ -424,6 +434,7 @@ static PpArgs make_args(gchi_ppr_batch *b, float damping, int residual) {
     a.hub_part = b->hub_part;
     a.res = b->res;
     a.counter = b->counter;
+    a.active = b->active;
     a.words = b->words;
     a.nqb = b->nqb;
     a.damping = damping;
@@ -444,14 +455,61 @@ static int ppr_sweep(gchi_ppr_batch *b, const T *rin, T *rout, float damping, in
     return GCHI_OK;
 }
 
+// Captures `len` sweeps of one phase (0: BF16, 1: FP32) from buffer 0, the
+// last one with the residual, into a graph kept by the batch; recaptured only
+// when the chunk length or the damping changes.  The kernels read the active
+// list from device memory, so retiring blocks needs no recapture.
+static int ppr_capture_chunk(gchi_ppr_batch *b, int phase, int len, float d) {
+    if (b->chunk_exec[phase] && b->chunk_len[phase] == len && b->chunk_damping[phase] == d) return GCHI_OK;
+    if (b->chunk_exec[phase]) (void)hipGraphExecDestroy(b->chunk_exec[phase]);
+    if (b->chunk_graph[phase]) (void)hipGraphDestroy(b->chunk_graph[phase]);
+    b->chunk_exec[phase] = nullptr;
+    b->chunk_graph[phase] = nullptr;
+    hipStream_t cs;
+    GCHI_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int st = GCHI_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) st = hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+    if (st == GCHI_OK) {
+        int s1 = GCHI_OK;
+        for (int k = 0; k < len && s1 == GCHI_OK; k++) {
+            const int res = k == len - 1;
+            if (phase == 0)
+                s1 = ppr_sweep<uint16_t>(b, (const uint16_t *)b->r16[k & 1], (uint16_t *)b->r16[(k + 1) & 1], d, res,
+                                         cs);
+            else
+                s1 = ppr_sweep<float>(b, b->r32[k & 1], b->r32[(k + 1) & 1], d, res, cs);
+        }
+        e = hipStreamEndCapture(cs, &b->chunk_graph[phase]);
+        if (s1 != GCHI_OK) st = s1;
+        else if (e != hipSuccess) st = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    }
+    if (st == GCHI_OK) {
+        e = hipGraphInstantiate(&b->chunk_exec[phase], b->chunk_graph[phase], nullptr, nullptr, 0);
+        if (e != hipSuccess) st = hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    (void)hipStreamDestroy(cs);
+    if (st != GCHI_OK) {
+        if (b->chunk_graph[phase]) (void)hipGraphDestroy(b->chunk_graph[phase]);
+        b->chunk_graph[phase] = nullptr;
+        b->chunk_exec[phase] = nullptr;
+        return st;
+    }
+    b->chunk_len[phase] = len;
+    b->chunk_damping[phase] = d;
+    return GCHI_OK;
+}
+
 }  // namespace gchi
 
 using namespace gchi;
 
 extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries, gchi_ppr_batch **out) {
     GCHI_CHECK(plan && out && nqueries > 0, GCHI_ERR_INVALID, "ppr_batch_create: bad argument");
+    PlanView pv;
+    GCHI_TRY(pr_plan_view(plan, &pv));  // before the batch exists: a refused plan leaks nothing
     gchi_ppr_batch *b = new gchi_ppr_batch();
-    GCHI_TRY(pr_plan_view(plan, &b->pv));
+    b->pv = pv;
     b->nq = nqueries;
     {
         int dev = 0;
@@ -476,6 +534,7 @@ extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries
     z.take<uint32_t>(b->nqb);
     z.take<float>((size_t)b->nqb * kQ);
     z.take<uint32_t>(kPpStripes * kPpStripeGap);
+    z.take<uint32_t>((size_t)b->nqb + 1);
     if (hipMalloc(&b->mem, z.off) != hipSuccess) {
         delete b;
         return fail(GCHI_ERR_HIP, "ppr_batch_create: hipMalloc failed (state " + std::to_string(z.off) + " bytes)");
@@ -494,6 +553,7 @@ extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries
     b->seed_n = c.take<uint32_t>(b->nqb);
     b->tele = c.take<float>((size_t)b->nqb * kQ);
     b->counter = c.take<uint32_t>(kPpStripes * kPpStripeGap);
+    b->active = c.take<uint32_t>((size_t)b->nqb + 1);
     hipLaunchKernelGGL(inv_outdeg_kernel, dim3(grid_for(b->pv.N, 256, 65536)), dim3(256), 0, 0, b->pv.outdeg, b->pv.N,
                        b->inv_outdeg);
     launch_fill_u32(b->counter, 0u, kPpStripes * kPpStripeGap, 0);
@@ -505,6 +565,10 @@ extern "C" int gchi_ppr_batch_create(const gchi_pr_plan *plan, uint32_t nqueries
 
 extern "C" int gchi_ppr_batch_destroy(gchi_ppr_batch *b) {
     if (!b) return GCHI_OK;
+    for (int k = 0; k < 2; k++) {
+        if (b->chunk_exec[k]) (void)hipGraphExecDestroy(b->chunk_exec[k]);
+        if (b->chunk_graph[k]) (void)hipGraphDestroy(b->chunk_graph[k]);
+    }
     if (b->mem) (void)hipFree(b->mem);
     delete b;
     return GCHI_OK;
@@ -557,68 +621,90 @@ extern "C" int gchi_ppr_batch_run(gchi_ppr_batch *b, const uint32_t *h_seed_off,
 
     const float d = (float)cfg->damping;
     const double stop = cfg->tol * (1.0 - cfg->damping) / cfg->damping;  // residual bound
-    const int check = cfg->check_every > 0 ? cfg->check_every : 4;
+    // sweeps run in chunks of `check` (even, so a chunk ends in the buffer it
+    // started from), each chunk one HIP-graph launch whose last sweep computes
+    // the residuals; converged query blocks leave the active list
+    const int check = std::max(2, (cfg->check_every > 0 ? cfg->check_every : 4) & ~1);
     const int max_sweeps = cfg->max_sweeps > 0 ? cfg->max_sweeps : 1000;
-    std::vector<double> res((size_t)nqb * kQ, 0.0);
+    std::vector<double> res((size_t)nqb * kQ, 0.0), last((size_t)nqb * kQ, 0.0);
+    std::vector<uint32_t> active(nqb + 1);
+    for (uint32_t qb = 0; qb < nqb; qb++) active[qb] = qb;
+    active[nqb] = nqb;
+    GCHI_HIP(hipMemcpyAsync(b->active, active.data(), 4 * active.size(), hipMemcpyHostToDevice, s));
+    uint32_t nact = nqb;
+    auto read_res = [&]() -> int {
+        GCHI_HIP(hipMemcpyAsync(res.data(), b->res, res.size() * 8, hipMemcpyDeviceToHost, s));
+        GCHI_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < nact; i++)
+            for (int l = 0; l < kQ; l++) last[(size_t)active[i] * kQ + l] = res[(size_t)active[i] * kQ + l];
+        return GCHI_OK;
+    };
     auto max_res = [&]() {
         double m = 0.0;
-        for (uint32_t q = 0; q < b->nq; q++) m = std::max(m, res[q]);
+        for (uint32_t q = 0; q < b->nq; q++) m = std::max(m, last[q]);
         return m;
     };
-    result->sweeps_bf16 = result->sweeps_fp32 = 0;
+    result->sweeps_bf16 = result->sweeps_fp32 = result->block_sweeps_fp32 = 0;
     PpArgs a0 = make_args(b, d, 0);
     int sweeps = 0;
     b->cur = 0;
     b->fp32 = cfg->bf16_sweeps <= 0;
     if (!b->fp32) {
         hipLaunchKernelGGL(ppr_init_kernel<uint16_t>, dim3(65536), dim3(256), 0, s, a0, (uint16_t *)b->r16[0]);
+        GCHI_LAUNCH_CHECK();
+        GCHI_TRY(ppr_capture_chunk(b, 0, check, d));
         double prev = 1e300;
-        while (sweeps < max_sweeps && result->sweeps_bf16 < cfg->bf16_sweeps) {
-            const bool chk = ((result->sweeps_bf16 + 1) % check) == 0;
-            GCHI_TRY(ppr_sweep<uint16_t>(b, (const uint16_t *)b->r16[b->cur], (uint16_t *)b->r16[b->cur ^ 1], d, chk,
-                                         s));
-            b->cur ^= 1;
-            sweeps++;
-            result->sweeps_bf16++;
-            if (chk) {
-                GCHI_HIP(hipMemcpyAsync(res.data(), b->res, res.size() * 8, hipMemcpyDeviceToHost, s));
-                GCHI_HIP(hipStreamSynchronize(s));
-                double m = max_res();
-                // leave BF16 once the residual is small or no longer falls (rounding floor)
-                if (m <= cfg->bf16_switch_residual || m > 0.7 * prev) break;
-                prev = m;
-            }
+        while (sweeps + check <= max_sweeps && result->sweeps_bf16 + check <= cfg->bf16_sweeps) {
+            GCHI_HIP(hipGraphLaunch(b->chunk_exec[0], s));
+            sweeps += check;
+            result->sweeps_bf16 += check;
+            GCHI_TRY(read_res());
+            const double m = max_res();
+            // leave BF16 once the residual is small or no longer falls (rounding floor)
+            if (m <= cfg->bf16_switch_residual || m > 0.7 * prev) break;
+            prev = m;
         }
         const size_t state = (size_t)nqb * N * kQ;
-        hipLaunchKernelGGL(ppr_widen_kernel, dim3(65536), dim3(256), 0, s, (const uint16_t *)b->r16[b->cur],
-                           b->r32[b->cur], state);
+        hipLaunchKernelGGL(ppr_widen_kernel, dim3(65536), dim3(256), 0, s, (const uint16_t *)b->r16[0], b->r32[0],
+                           state);
         GCHI_LAUNCH_CHECK();
         b->fp32 = true;
     } else {
         hipLaunchKernelGGL(ppr_init_kernel<float>, dim3(65536), dim3(256), 0, s, a0, b->r32[0]);
         GCHI_LAUNCH_CHECK();
     }
+    GCHI_TRY(ppr_capture_chunk(b, 1, check, d));
     bool converged = false;
-    while (sweeps < max_sweeps) {
-        const bool chk = ((result->sweeps_fp32 + 1) % check) == 0;
-        GCHI_TRY(ppr_sweep<float>(b, b->r32[b->cur], b->r32[b->cur ^ 1], d, chk, s));
-        b->cur ^= 1;
-        sweeps++;
-        result->sweeps_fp32++;
-        if (chk) {
-            GCHI_HIP(hipMemcpyAsync(res.data(), b->res, res.size() * 8, hipMemcpyDeviceToHost, s));
-            GCHI_HIP(hipStreamSynchronize(s));
-            if (max_res() <= stop) {
-                converged = true;
-                break;
-            }
+    while (sweeps + check <= max_sweeps) {
+        GCHI_HIP(hipGraphLaunch(b->chunk_exec[1], s));
+        sweeps += check;
+        result->sweeps_fp32 += check;
+        GCHI_TRY(read_res());
+        // retire the query blocks whose every query meets the bound
+        uint32_t keep = 0;
+        for (uint32_t i = 0; i < nact; i++) {
+            const uint32_t qb = active[i];
+            double m = 0.0;
+            for (int l = 0; l < kQ; l++) m = std::max(m, last[(size_t)qb * kQ + l]);
+            if (m > stop) active[keep++] = qb;
+        }
+        result->block_sweeps_fp32 += (int)nact * check;
+        if (keep == 0) {
+            converged = true;
+            break;
+        }
+        if (keep != nact) {
+            nact = keep;
+            active[nqb] = nact;
+            GCHI_HIP(hipMemcpyAsync(b->active, active.data(), 4 * nact, hipMemcpyHostToDevice, s));
+            GCHI_HIP(hipMemcpyAsync(b->active + nqb, &active[nqb], 4, hipMemcpyHostToDevice, s));
         }
     }
     GCHI_HIP(hipStreamSynchronize(s));
     result->converged = converged ? 1 : 0;
     result->max_residual = max_res();
     if (h_residuals)
-        for (uint32_t q = 0; q < b->nq; q++) h_residuals[q] = res[q];
+        for (uint32_t q = 0; q < b->nq; q++) h_residuals[q] = last[q];
     return GCHI_OK;
 }
 

@@ -34,7 +34,7 @@ constexpr int kRsMaxPasses = 8;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
-constexpr uint32_t kSpinLimit = 1u << 26;
+constexpr uint32_t kSpinLimit = 1u << 26;  // default bound of one look-back wait
 constexpr int kRsLookback = 4;
 constexpr int kRsHistKeys = 8;  // keys per thread in flight in the histogram pass  // predecessor tiles read per look-back round trip
 
@@ -126,7 +126,8 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
                                                                uint32_t *__restrict__ vout, uint64_t n, int shift,
                                                                int nbits, const uint32_t *__restrict__ gbase,
                                                                uint32_t *status, uint32_t *tile_counter,
-                                                               uint32_t *err, PairIO io) {
+                                                               uint32_t *err, PairIO io, uint32_t spin_limit,
+                                                               int force_stall) {
     constexpr int NW = T / kWave;
     constexpr int TILE = T * kRsItems;
     static_assert(T >= kRsRadix && T % kRsRadix == 0, "a thread per digit");
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
 #pragma unroll
                 for (int q = 0; q < kRsLookback; q++) {
                     if (inc || stalled) continue;
-                    const uint32_t flag = sv[q] & ~kValMask;
+                    const uint32_t flag = force_stall ? 0u : sv[q] & ~kValMask;
                     if (flag == 0) {
                         stalled = true;
                         continue;
@@ -245,8 +246,11 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
                 if (inc) break;
                 p -= used;
                 if (used == 0) {
-                    if (++spins > kSpinLimit) {
-                        atomicOr(err, 1u);
+                    if (++spins > spin_limit) {
+                        // the prefix so far is at most the true one, so every
+                        // write below stays inside [0, n); the output is wrong
+                        // and the caller's status word says so
+                        atomicOr(err, GCHI_WORK_TIMEOUT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -300,24 +304,31 @@ __global__ void copy_u32(const uint32_t *a, uint32_t *b, uint64_t n) {
 static int npasses_for(int key_bits) { return (key_bits + 7) / 8; }
 
 struct SortWork {
+    uint32_t *err;  // status word (GCHI_WORK_* bits); first in the workspace
     uint64_t *ktmp;
     uint32_t *vtmp;
     uint32_t *ghist;
     uint32_t *status;
-    uint32_t *counters;  // [passes] tile counters + [1] error word
+    uint32_t *counters;  // [passes] tile counters
 };
 
 template <typename C>
 static void sort_layout(C &c, uint64_t n, int key_bits, bool vals, SortWork *w) {
     int np = npasses_for(key_bits);
     uint64_t ntiles = div_up(n, kRsTile);
+    auto *er = c.template take<uint32_t>(1);
     auto *kt = c.template take<uint64_t>(n);
     auto *vt = c.template take<uint32_t>(vals ? n : 1);
     auto *gh = c.template take<uint32_t>((size_t)np * kRsRadix);
     auto *st = c.template take<uint32_t>((size_t)np * ntiles * kRsRadix);
-    auto *ct = c.template take<uint32_t>(np + 1);
-    if (w) *w = SortWork{kt, vt, gh, st, ct};
+    auto *ct = c.template take<uint32_t>(np);
+    if (w) *w = SortWork{er, kt, vt, gh, st, ct};
 }
+
+// Bound of every look-back wait and the stall switch of the test hook
+// gchi_debug_sort_lookback (process-wide; read at each launch).
+static uint32_t g_spin_limit = kSpinLimit;
+static int g_force_stall = 0;
 
 size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
     SizerAdapter a;
@@ -335,45 +346,56 @@ size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
 static void launch_key_pass(int io_mode, const uint64_t *kin, uint64_t *kout, uint64_t n, int shift, int bits,
                             const uint32_t *gbase, uint32_t *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
                             uint64_t ntiles, hipStream_t s) {
+    const uint32_t sl = g_spin_limit;
+    const int fs = g_force_stall;
     const dim3 grid((unsigned)ntiles), blk(kRsThreadsK);
     const uint32_t *nv = nullptr;
     uint32_t *nvo = nullptr;
     switch (io_mode) {
     case 0:
         hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, 0>), grid, blk, 0, s, kin, nv, kout, nvo, n, shift,
-                           bits, gbase, st, ctr, err, io);
+                           bits, gbase, st, ctr, err, io, sl, fs);
         break;
     case kIoPackIn:
         hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn>), grid, blk, 0, s, kin, nv, kout, nvo, n,
-                           shift, bits, gbase, st, ctr, err, io);
+                           shift, bits, gbase, st, ctr, err, io, sl, fs);
         break;
     case kIoUnpackOut:
         hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoUnpackOut>), grid, blk, 0, s, kin, nv, kout, nvo,
-                           n, shift, bits, gbase, st, ctr, err, io);
+                           n, shift, bits, gbase, st, ctr, err, io, sl, fs);
         break;
     default:
         hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn | kIoUnpackOut>), grid, blk, 0, s, kin, nv,
-                           kout, nvo, n, shift, bits, gbase, st, ctr, err, io);
+                           kout, nvo, n, shift, bits, gbase, st, ctr, err, io, sl, fs);
     }
 }
 
 // The sort itself; with pio, the keys come from pio's (src, dst) input and the
 // result goes to its (src, dst) output (keys and alt are then the two
-// intermediate buffers, and nothing is copied back).
+// intermediate buffers, and nothing is copied back).  A look-back wait that
+// exceeds its bound sets GCHI_WORK_TIMEOUT in *err (the workspace's own status
+// word when err is null, cleared here); a caller that passes its own word
+// clears it and reads it back once the stream has run.
 static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
-                     size_t work_bytes, hipStream_t s, uint64_t **out, const PairIO *pio) {
+                     size_t work_bytes, hipStream_t s, uint64_t **out, const PairIO *pio, uint32_t *err) {
     if (out) *out = keys;
-    if (n <= 1 || hi <= lo) return GCHI_OK;
-    GCHI_CHECK(keys, GCHI_ERR_INVALID, "sort: null keys");
+    if ((n <= 1 || hi <= lo) && !work) return GCHI_OK;  // nothing to sort and no status word to clear
     GCHI_CHECK(lo >= 0 && hi <= 64, GCHI_ERR_INVALID, "sort: key bits outside [0, 64)");
     GCHI_CHECK(n < (1ull << 30), GCHI_ERR_INVALID, "sort: n must be < 2^30");
     GCHI_CHECK(!(out && vals), GCHI_ERR_INVALID, "sort: values are always copied back");
+    GCHI_CHECK(n <= 1 || hi <= lo || keys, GCHI_ERR_INVALID, "sort: null keys");
     const bool hv = vals != nullptr;
-    const int bits = hi - lo;
-    GCHI_CHECK(work_bytes >= sort_workspace_bytes(n, bits, hv), GCHI_ERR_WORKSPACE, "sort: workspace too small");
+    const int bits = hi > lo ? hi - lo : 0;
+    GCHI_CHECK(work && work_bytes >= sort_workspace_bytes(n, bits, hv), GCHI_ERR_WORKSPACE,
+               "sort: workspace too small");
     Carver c(work, work_bytes);
     SortWork w;
     sort_layout(c, n, bits, hv, &w);
+    if (!err) {
+        err = w.err;
+        launch_fill_u32(err, 0u, 1, s);
+    }
+    if (n <= 1 || hi <= lo) return GCHI_OK;
     const int np = npasses_for(bits);
     const uint64_t ntiles = div_up(n, hv ? kRsTile : kRsTileK);  // <= the laid-out div_up(n, kRsTile)
     PassDesc pd;
@@ -383,7 +405,7 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
     }
     launch_fill_u32(w.ghist, 0u, (size_t)np * kRsRadix, s);
     launch_fill_u32(w.status, 0u, (size_t)np * ntiles * kRsRadix, s);
-    launch_fill_u32(w.counters, 0u, np + 1, s);
+    launch_fill_u32(w.counters, 0u, np, s);
     unsigned hblocks = (unsigned)std::min<uint64_t>(ntiles, 1024);
     const PairIO io = pio ? *pio : PairIO{};
     if (pio)
@@ -399,13 +421,11 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
         uint32_t *st = w.status + (size_t)p * ntiles * kRsRadix;
         if (hv)
             hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads>), dim3((unsigned)ntiles), dim3(kRsThreads), 0, s,
-                               ka, va, kb,
-                               vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p,
-                               w.counters + np, PairIO{});
+                               ka, va, kb, vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st,
+                               w.counters + p, err, PairIO{}, g_spin_limit, g_force_stall);
         else
             launch_key_pass(pio ? (p == 0 ? kIoPackIn : 0) | (p == np - 1 ? kIoUnpackOut : 0) : 0, ka, kb, n,
-                            pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p, w.counters + np, io,
-                            ntiles, s);
+                            pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p, err, io, ntiles, s);
         GCHI_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -423,16 +443,25 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
 }
 
 int radix_sort_u64_bits(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, int lo, int hi, void *work,
-                        size_t work_bytes, hipStream_t s, uint64_t **out) {
-    return sort_impl(keys, alt, vals, n, lo, hi, work, work_bytes, s, out, nullptr);
+                        size_t work_bytes, hipStream_t s, uint64_t **out, uint32_t *err) {
+    return sort_impl(keys, alt, vals, n, lo, hi, work, work_bytes, s, out, nullptr, err);
 }
 
 // Sorts keys (and vals) in place on their low key_bits bits.
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int key_bits, void *work, size_t work_bytes,
-                   hipStream_t s) {
-    if (n <= 1 || key_bits <= 0) return GCHI_OK;
-    GCHI_CHECK(key_bits <= 64, GCHI_ERR_INVALID, "sort: key_bits > 64");
-    return radix_sort_u64_bits(keys, nullptr, vals, n, 0, key_bits, work, work_bytes, s, nullptr);
+                   hipStream_t s, uint32_t *err) {
+    GCHI_CHECK(key_bits >= 0 && key_bits <= 64, GCHI_ERR_INVALID, "sort: key_bits outside [0, 64]");
+    return radix_sort_u64_bits(keys, nullptr, vals, n, 0, key_bits, work, work_bytes, s, nullptr, err);
+}
+
+int read_work_status(const uint32_t *err, hipStream_t s) {
+    uint32_t h = 0;
+    GCHI_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, s));
+    GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(!(h & GCHI_WORK_TIMEOUT), GCHI_ERR_TIMEOUT,
+               "a bounded device wait (sort look-back) gave up: the output is invalid");
+    GCHI_CHECK(!(h & GCHI_WORK_BAD_ID), GCHI_ERR_INVALID, "vertex id out of range");
+    return GCHI_OK;
 }
 
 // ---- (src,dst) pair sort by (dst,src): sort_impl with PairIO ----
@@ -449,7 +478,7 @@ extern "C" int gchi_sort_workspace(uint64_t n, int key_bits, int with_values, si
 
 extern "C" int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits, void *d_work,
                              size_t work_bytes, void *stream) {
-    return radix_sort_u64(d_keys, d_vals, n, key_bits, d_work, work_bytes, as_stream(stream));
+    return radix_sort_u64(d_keys, d_vals, n, key_bits, d_work, work_bytes, as_stream(stream), nullptr);
 }
 
 // Key bits of the pair sort: at least 9, so that there are two passes and the
@@ -460,8 +489,8 @@ static int pair_key_bits(int id_bits) { return std::max(2 * id_bits, 9); }
 extern "C" int gchi_sort_pairs_dst_src_workspace(uint64_t n, int id_bits, size_t *bytes) {
     GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
     Sizer z;
+    z.take<char>(sort_workspace_bytes(n, pair_key_bits(id_bits), false));  // status word first
     z.take<uint64_t>(n);
-    z.take<char>(sort_workspace_bytes(n, pair_key_bits(id_bits), false));
     *bytes = z.off;
     return GCHI_OK;
 }
@@ -470,21 +499,33 @@ extern "C" int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_
                                        uint32_t *d_src_out, uint32_t *d_dst_out, void *d_work, size_t work_bytes,
                                        void *stream) {
     GCHI_CHECK(id_bits >= 1 && id_bits <= 32, GCHI_ERR_INVALID, "sort_pairs: id_bits must be in [1,32]");
-    if (n == 0) return GCHI_OK;
-    GCHI_CHECK(d_src && d_dst && d_src_out && d_dst_out, GCHI_ERR_INVALID, "sort_pairs: null pointer");
     size_t need = 0;
     gchi_sort_pairs_dst_src_workspace(n, id_bits, &need);
-    GCHI_CHECK(work_bytes >= need, GCHI_ERR_WORKSPACE, "sort_pairs: workspace too small");
+    GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "sort_pairs: workspace too small");
     hipStream_t s = as_stream(stream);
     Carver c(d_work, work_bytes);
-    uint64_t *keys = c.take<uint64_t>(n);
     size_t sw_bytes = sort_workspace_bytes(n, pair_key_bits(id_bits), false);
     void *sw = c.take<char>(sw_bytes);
+    uint64_t *keys = c.take<uint64_t>(n);
+    if (n == 0) return radix_sort_u64(nullptr, nullptr, 0, 0, sw, sw_bytes, s, nullptr);  // clears the status
+    GCHI_CHECK(d_src && d_dst && d_src_out && d_dst_out, GCHI_ERR_INVALID, "sort_pairs: null pointer");
     const PairIO io{d_src, d_dst, d_src_out, d_dst_out, id_bits};
     if (n == 1) {  // nothing to sort: the pair is copied
+        GCHI_TRY(radix_sort_u64(nullptr, nullptr, 0, 0, sw, sw_bytes, s, nullptr));
         if (d_src_out != d_src) GCHI_HIP(hipMemcpyAsync(d_src_out, d_src, 4, hipMemcpyDeviceToDevice, s));
         if (d_dst_out != d_dst) GCHI_HIP(hipMemcpyAsync(d_dst_out, d_dst, 4, hipMemcpyDeviceToDevice, s));
         return GCHI_OK;
     }
-    return sort_impl(keys, nullptr, nullptr, n, 0, pair_key_bits(id_bits), sw, sw_bytes, s, nullptr, &io);
+    return sort_impl(keys, nullptr, nullptr, n, 0, pair_key_bits(id_bits), sw, sw_bytes, s, nullptr, &io, nullptr);
+}
+
+extern "C" int gchi_work_status(const void *d_work, void *stream) {
+    GCHI_CHECK(d_work, GCHI_ERR_INVALID, "work_status: null workspace");
+    return read_work_status(static_cast<const uint32_t *>(d_work), as_stream(stream));
+}
+
+extern "C" int gchi_debug_sort_lookback(uint32_t spin_limit, int force_stall) {
+    g_spin_limit = spin_limit ? spin_limit : kSpinLimit;
+    g_force_stall = force_stall ? 1 : 0;
+    return GCHI_OK;
 }

@@ -235,6 +235,18 @@ extern "C" int gchi_free(void *d_ptr) {
     return GCHI_OK;
 }
 
+extern "C" int gchi_host_alloc(void **h_ptr, size_t bytes) {
+    GCHI_CHECK(h_ptr, GCHI_ERR_INVALID, "host_alloc: null pointer");
+    *h_ptr = nullptr;
+    GCHI_HIP(hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return GCHI_OK;
+}
+
+extern "C" int gchi_host_free(void *h_ptr) {
+    if (h_ptr) GCHI_HIP(hipHostFree(h_ptr));
+    return GCHI_OK;
+}
+
 extern "C" int gchi_synchronize(void *stream) {
     GCHI_HIP(hipStreamSynchronize(as_stream(stream)));
     return GCHI_OK;

@@ -20,6 +20,8 @@
 //      4 bytes per dst) is scanned into offsets and written in parallel.
 //      .adjidx entries (every >= 1M edges at a run boundary, :765-771) are
 //      found greedily by binary search over run starts.
+#include <zlib.h>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -358,11 +360,11 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
     if (E) {
         hipLaunchKernelGGL(sh_pack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N, db, keys, err);
         uint64_t *by_dst = nullptr;
-        st = radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, db, sw, swb, s, &by_dst);
+        st = radix_sort_u64_bits(keys, nullptr, nullptr, E, 0, db, sw, swb, s, &by_dst, err);
         if (st == GCHI_OK) st = count_field_runs(by_dst, E, 0, idmask, N, run_start, indeg, nullptr, 0u, s);
         if (st == GCHI_OK)
             st = radix_sort_u64_bits(by_dst, by_dst == keys ? nullptr : keys, nullptr, E, db, db + idb, sw, swb, s,
-                                     &by_src_dst);
+                                     &by_src_dst, err);
         if (st == GCHI_OK) st = count_field_runs(by_src_dst, E, db, idmask, N, run_start, outdeg, nullptr, 0u, s);
     }
     if (st == GCHI_OK) {
@@ -377,6 +379,8 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
         uint32_t h_err = 0;
         st = sh_read(h_ns, d_ns, sizeof(h_ns), s);
         if (st == GCHI_OK) st = sh_read(&h_err, err, 4, s);
+        if (st == GCHI_OK && (h_err & GCHI_WORK_TIMEOUT))
+            st = fail(GCHI_ERR_TIMEOUT, "shard_build: a sort look-back wait gave up");
         if (st == GCHI_OK && h_err) st = fail(GCHI_ERR_INVALID, "shard_build: vertex id > max_id");
         if (st == GCHI_OK && h_ns[1]) st = fail(GCHI_ERR_FORMAT, "shard_build: the streaming cut needs more shards "
                                                                  "than requested (the reference asserts here)");
@@ -394,10 +398,11 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
                            sh->nshards);
         uint64_t *sorted = nullptr;
         st = radix_sort_u64_bits(by_src_dst, by_src_dst == keys ? nullptr : keys, nullptr, E, 2 * idb,
-                                 2 * idb + pb, sw, swb, s, &sorted);
+                                 2 * idb + pb, sw, swb, s, &sorted, err);
         if (st == GCHI_OK)
             hipLaunchKernelGGL(sh_unpack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, sorted, E, idb, sh->src,
                                sh->dst);
+        if (st == GCHI_OK) st = read_work_status(err, s);  // waits for the stream
         hipError_t e = hipStreamSynchronize(s);
         if (st == GCHI_OK && e != hipSuccess) st = hip_fail(e, "shard_build", __FILE__, __LINE__);
     }
@@ -540,9 +545,35 @@ int write_file(const std::string &path, const void *data, size_t n) {
 // of edge_value_bytes) and <edata>.size -- the layout of
 // src/api/chifilenames.hpp:65-153 that find_shards (:200-275) accepts.
 // Compressed (zlib) edge-data blocks are not written.
+// zlib deflate of one edge-data block at level 1 (Z_BEST_SPEED), one
+// Z_FINISH stream: write_compressed of src/util/ioutil.hpp:144-199.
+static int deflate_block(const uint8_t *in, size_t n, std::vector<uint8_t> &out) {
+    z_stream z;
+    memset(&z, 0, sizeof(z));
+    if (deflateInit(&z, Z_BEST_SPEED) != Z_OK) return fail(GCHI_ERR_IO, "shard_write_files: deflateInit failed");
+    out.resize(deflateBound(&z, (uLong)n) + 64);
+    z.next_in = const_cast<Bytef *>(in);
+    z.avail_in = (uInt)n;
+    z.next_out = out.data();
+    z.avail_out = (uInt)out.size();
+    const int r = deflate(&z, Z_FINISH);
+    const size_t have = out.size() - z.avail_out;
+    deflateEnd(&z);
+    if (r != Z_STREAM_END) return fail(GCHI_ERR_IO, "shard_write_files: deflate did not finish");
+    out.resize(have);
+    return GCHI_OK;
+}
+
 extern "C" int gchi_shard_write_files(const gchi_shards *sh, const char *basefilename, uint32_t edge_value_bytes,
                                       void *stream) {
+    return gchi_shard_write_files_ex(sh, basefilename, edge_value_bytes, 0, stream);
+}
+
+extern "C" int gchi_shard_write_files_ex(const gchi_shards *sh, const char *basefilename, uint32_t edge_value_bytes,
+                                         int flags, void *stream) {
     GCHI_CHECK(sh && basefilename, GCHI_ERR_INVALID, "shard_write_files: null argument");
+    GCHI_CHECK((flags & ~GCHI_SHARD_ZLIB) == 0, GCHI_ERR_INVALID, "shard_write_files: unknown flags");
+    const bool zlib = (flags & GCHI_SHARD_ZLIB) != 0;
     const std::string base(basefilename);
     const int P = sh->nshards_req;
     hipStream_t s = as_stream(stream);
@@ -572,17 +603,25 @@ extern "C" int gchi_shard_write_files(const gchi_shards *sh, const char *basefil
         GCHI_TRY(write_file(adjname, adj.data(), bytes));
         GCHI_TRY(write_file(adjname + "idx", idx.data(), 24 * (size_t)nidx));
         if (edge_value_bytes) {
-            std::string ed = base + ".edata.e" + std::to_string(edge_value_bytes) + "B" + part(p, P);
+            // filename_shard_edata (src/api/chifilenames.hpp:91-105): ".edata." then
+            // ".Z." when compressed, hence the double dot
+            std::string ed = base + ".edata." + (zlib ? ".Z." : "") + "e" + std::to_string(edge_value_bytes) + "B" +
+                             part(p, P);
             uint64_t total = sh->counts[p] * edge_value_bytes;
             std::string sz = std::to_string(total);
             GCHI_TRY(write_file(ed + ".size", sz.data(), sz.size()));
             std::string dir = ed + "_blockdir_" + std::to_string(block);
             mkdir(dir.c_str(), 0777);
-            std::vector<uint8_t> zeros(block, 0);
+            std::vector<uint8_t> zeros(block, 0), packed;
             uint64_t nblocks = total ? (total + block - 1) / block : 1;
             for (uint64_t b = 0; b < nblocks; b++) {
                 uint64_t len = std::min<uint64_t>(block, total - std::min<uint64_t>(total, b * block));
-                GCHI_TRY(write_file(dir + "/" + std::to_string(b), zeros.data(), len));
+                if (zlib) {
+                    GCHI_TRY(deflate_block(zeros.data(), len, packed));
+                    GCHI_TRY(write_file(dir + "/" + std::to_string(b), packed.data(), packed.size()));
+                } else {
+                    GCHI_TRY(write_file(dir + "/" + std::to_string(b), zeros.data(), len));
+                }
             }
         }
     }

@@ -19,6 +19,7 @@
 //       binary-searches its w in N+(a) in global memory.  Counts: a, b and w
 //       each get +1 per triangle, gathered per wavefront and flushed with
 //       uint64 atomics, converted to uint32 with an overflow flag.
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -420,10 +421,10 @@ static int tc_degree_counts(const uint32_t *src, const uint32_t *dst, uint64_t E
     hipLaunchKernelGGL(tc_degree_pack, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, src, dst, E, N, sb, ekeys, err);
     GCHI_LAUNCH_CHECK();
     uint64_t *by_src = nullptr, *by_dst = nullptr;
-    GCHI_TRY(radix_sort_u64_bits(ekeys, nullptr, nullptr, E, 0, sb, sw, swb, s, &by_src));
+    GCHI_TRY(radix_sort_u64_bits(ekeys, nullptr, nullptr, E, 0, sb, sw, swb, s, &by_src, err));
     GCHI_TRY(count_field_runs(by_src, E, 0, fmask, N, run_start, deg, nullptr, 0u, s, true));
     GCHI_TRY(radix_sort_u64_bits(by_src, by_src == ekeys ? nullptr : ekeys, nullptr, E, sb, sb + fb, sw, swb, s,
-                                 &by_dst));
+                                 &by_dst, err));
     return count_field_runs(by_dst, E, sb, fmask, N, run_start, deg, nullptr, 0u, s, true);
 }
 
@@ -469,7 +470,7 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
     GCHI_TRY(tc_degree_counts(d_src, d_dst, E, N, deg, err, vkeys, ekeys, sw, swb, s));
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();
-    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
+    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s, err));
     hipLaunchKernelGGL(tc_vertexmap, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, vkeys, N, idb, d_vertexmap);
     const uint64_t sentinel = (2 * idb >= 64) ? ~0ull : ((1ull << (2 * idb)) - 1ull);
     uint32_t *lo_count = deg;  // reuse: degrees are no longer needed
@@ -478,7 +479,7 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
         hipLaunchKernelGGL(tc_orient_keys, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, d_src, d_dst, E, N,
                            d_vertexmap, idb, sentinel, ekeys);
         GCHI_LAUNCH_CHECK();
-        GCHI_TRY(radix_sort_u64(ekeys, nullptr, E, 2 * idb, sw, swb, s));
+        GCHI_TRY(radix_sort_u64(ekeys, nullptr, E, 2 * idb, sw, swb, s, err));
         hipLaunchKernelGGL(tc_unique_flags, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, E, sentinel,
                            flags);
         GCHI_LAUNCH_CHECK();
@@ -495,18 +496,59 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
     GCHI_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
     GCHI_HIP(hipMemcpyAsync(&h_m, d_off + N, 4, hipMemcpyDeviceToHost, s));
     GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(!(h_err & GCHI_WORK_TIMEOUT), GCHI_ERR_TIMEOUT, "tc_prepare: a sort look-back wait gave up");
     GCHI_CHECK(h_err == 0, GCHI_ERR_INVALID, "tc_prepare: vertex id >= nvertices");
     *h_noriented = h_m;
     return GCHI_OK;
 }
 
+namespace gchi {
+
+// The counting launches, over counters cnt[n+1] / tot[1] / ticket stripes wk
+// (all cleared here first).
+static int tc_count_impl(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_per_vertex,
+                         uint64_t *d_total, uint32_t *d_overflow, unsigned long long *cnt, unsigned long long *tot,
+                         uint32_t *wk, hipStream_t s) {
+    launch_fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, nvertices, s);
+    launch_fill_u64(reinterpret_cast<uint64_t *>(tot), 0ull, 1, s);
+    launch_fill_u32(wk, 0u, kTcStripes * kTcStripeGap, s);
+    launch_fill_u32(d_overflow, 0u, 1, s);
+    int ncu = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+    }
+    hipLaunchKernelGGL(tc_count_hash_kernel, dim3(ncu * 6), dim3(kTcHashWaves * kWave), 0, s, d_off, d_adj,
+                       nvertices, cnt, tot, wk);
+    hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices,
+                       kTcHashMax, cnt, tot);
+    hipLaunchKernelGGL(tc_to_u32, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, cnt, nvertices,
+                       d_per_vertex, d_overflow);
+    GCHI_HIP(hipMemcpyAsync(d_total, tot, 8, hipMemcpyDeviceToDevice, s));
+    GCHI_LAUNCH_CHECK();
+    return GCHI_OK;
+}
+
+template <typename C>
+static void tc_count_layout(C &c, uint32_t nvertices, unsigned long long **cnt, unsigned long long **tot,
+                            uint32_t **wk) {
+    *cnt = c.template take<unsigned long long>((size_t)nvertices + 1);
+    *tot = c.template take<unsigned long long>(1);
+    *wk = c.template take<uint32_t>(kTcStripes * kTcStripeGap);
+}
+
+}  // namespace gchi
+
+using namespace gchi;
+
 extern "C" int gchi_tc_count_workspace(uint32_t nvertices, size_t *bytes) {
     GCHI_CHECK(bytes, GCHI_ERR_INVALID, "null bytes");
-    Sizer z;
-    z.take<unsigned long long>(nvertices + 1);
-    z.take<unsigned long long>(1);
-    z.take<uint32_t>(kTcStripes * kTcStripeGap);
-    *bytes = z.off;
+    SizerAdapter a;
+    unsigned long long *cnt, *tot;
+    uint32_t *wk;
+    tc_count_layout(a, nvertices, &cnt, &tot, &wk);
+    *bytes = a.z.off;
     return GCHI_OK;
 }
 
@@ -518,30 +560,12 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
     size_t need = 0;
     gchi_tc_count_workspace(nvertices, &need);
     GCHI_CHECK(d_work && work_bytes >= need, GCHI_ERR_WORKSPACE, "tc_count: workspace too small");
-    hipStream_t s = as_stream(stream);
     Carver c(d_work, work_bytes);
-    auto *cnt = c.take<unsigned long long>(nvertices + 1);
-    auto *tot = c.take<unsigned long long>(1);
-    auto *wk = c.take<uint32_t>(kTcStripes * kTcStripeGap);
-    launch_fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, nvertices, s);
-    launch_fill_u64(reinterpret_cast<uint64_t *>(tot), 0ull, 1, s);
-    launch_fill_u32(wk, 0u, kTcStripes * kTcStripeGap, s);
-    launch_fill_u32(d_overflow, 0u, 1, s);
-    int ncu = 256;
-    int dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
-    }
-    hipLaunchKernelGGL(tc_count_hash_kernel, dim3(ncu * 6), dim3(kTcHashWaves * kWave), 0, s, d_off, d_adj,
-                       nvertices, cnt, tot, wk);
-    hipLaunchKernelGGL(tc_count_kernel, dim3(ncu * 8), dim3(kTcWaves * kWave), 0, s, d_off, d_adj, nvertices,
-                       kTcHashMax, cnt, tot);
-    hipLaunchKernelGGL(tc_to_u32, dim3(grid_for(nvertices, 256, 65536)), dim3(256), 0, s, cnt, nvertices,
-                       d_per_vertex, d_overflow);
-    GCHI_HIP(hipMemcpyAsync(d_total, tot, 8, hipMemcpyDeviceToDevice, s));
-    GCHI_LAUNCH_CHECK();
-    return GCHI_OK;
+    unsigned long long *cnt, *tot;
+    uint32_t *wk;
+    tc_count_layout(c, nvertices, &cnt, &tot, &wk);
+    return tc_count_impl(d_off, d_adj, nvertices, d_per_vertex, d_total, d_overflow, cnt, tot, wk,
+                         as_stream(stream));
 }
 
 __global__ void relabel_kernel(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, const uint32_t *map,
@@ -586,11 +610,12 @@ extern "C" int gchi_degree_order(const uint32_t *d_src, const uint32_t *d_dst, u
     GCHI_TRY(tc_degree_counts(d_src, d_dst, nedges, N, deg, err, vkeys, ekeys, sw, swb, s));
     hipLaunchKernelGGL(tc_degree_keys, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, deg, N, idb, vkeys);
     GCHI_LAUNCH_CHECK();
-    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s));
+    GCHI_TRY(radix_sort_u64(vkeys, nullptr, N, idb + degb, sw, swb, s, err));
     hipLaunchKernelGGL(tc_vertexmap, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, vkeys, N, idb, d_vertexmap);
     uint32_t h_err = 0;
     GCHI_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
     GCHI_HIP(hipStreamSynchronize(s));
+    GCHI_CHECK(!(h_err & GCHI_WORK_TIMEOUT), GCHI_ERR_TIMEOUT, "degree_order: a sort look-back wait gave up");
     GCHI_CHECK(h_err == 0, GCHI_ERR_INVALID, "degree_order: vertex id >= nvertices");
     return GCHI_OK;
 }

@@ -56,7 +56,7 @@ class PprConfig(ctypes.Structure):
 
 
 class PprResult(ctypes.Structure):
-    _fields_ = [("sweeps_bf16", c_i32), ("sweeps_fp32", c_i32), ("converged", c_i32), ("reserved", c_i32),
+    _fields_ = [("sweeps_bf16", c_i32), ("sweeps_fp32", c_i32), ("converged", c_i32), ("block_sweeps_fp32", c_i32),
                 ("max_residual", ctypes.c_double)]
 
 
@@ -72,10 +72,15 @@ _SIGS = {
     "gchi_rmat_generate_host": (c_i32, [c_vp, c_vp, c_u64, c_u64, c_i32, c_u64, ctypes.POINTER(ctypes.c_double), c_i32]),
     "gchi_drop_self_loops_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
     "gchi_drop_self_loops": (c_i32, [c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_parse_edgelist_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_parse_edgelist": (c_i32, [c_vp, c_u64, c_vp, c_vp, c_u64, ctypes.POINTER(c_u64), ctypes.POINTER(c_u32),
+                                    c_vp, c_sz, c_vp]),
     "gchi_sort_workspace": (c_i32, [c_u64, c_i32, c_i32, ctypes.POINTER(c_sz)]),
     "gchi_sort_u64": (c_i32, [c_vp, c_vp, c_u64, c_i32, c_vp, c_sz, c_vp]),
     "gchi_sort_pairs_dst_src_workspace": (c_i32, [c_u64, c_i32, ctypes.POINTER(c_sz)]),
     "gchi_sort_pairs_dst_src": (c_i32, [c_vp, c_vp, c_u64, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_work_status": (c_i32, [c_vp, c_vp]),
+    "gchi_debug_sort_lookback": (c_i32, [c_u32, c_i32]),
     "gchi_csc_build_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
     "gchi_csc_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "gchi_pr_plan_create": (c_i32, [ctypes.POINTER(DGraph), ctypes.POINTER(PrTune), ctypes.POINTER(c_vp), c_vp]),
@@ -91,6 +96,13 @@ _SIGS = {
                                           ctypes.POINTER(c_vp), c_vp]),
     "gchi_pr_init_seeds": (c_i32, [c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp]),
     "gchi_pr_exec_launch": (c_i32, [c_vp, c_vp]),
+    "gchi_pr_blocked_create": (c_i32, [ctypes.POINTER(DGraph), c_vp, c_u32, ctypes.POINTER(PrTune),
+                                       ctypes.POINTER(c_vp), c_vp]),
+    "gchi_pr_blocked_destroy": (c_i32, [c_vp]),
+    "gchi_pr_blocked_init": (c_i32, [c_vp, ctypes.c_float, c_vp, ctypes.c_float, c_vp, c_vp, c_vp]),
+    "gchi_pr_blocked_sweep": (c_i32, [c_vp, ctypes.POINTER(PrOp), c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gchi_host_alloc": (c_i32, [ctypes.POINTER(c_vp), c_sz]),
+    "gchi_host_free": (c_i32, [c_vp]),
     "gchi_pr_exec_destroy": (c_i32, [c_vp]),
     "gchi_autotune_pr": (c_i32, [ctypes.POINTER(DGraph), c_i32, ctypes.POINTER(PrTune), ctypes.POINTER(ctypes.c_double),
                                  c_vp, c_vp, c_i32, ctypes.POINTER(c_i32), c_vp]),
@@ -104,6 +116,9 @@ _SIGS = {
     "gchi_cc_labelprop_init": (c_i32, [c_vp, c_u32, c_vp]),
     "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_scheduled_workspace": (c_i32, [c_u32, c_u64, ctypes.POINTER(c_sz)]),
+    "gchi_cc_afforest_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_cc_pull_sweep": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp]),
+    "gchi_cc_afforest": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_sz, c_vp]),
     "gchi_shard_decode_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
     "gchi_shard_decode_many_workspace": (c_i32, [c_vp, c_u32, ctypes.POINTER(c_sz)]),
     "gchi_shard_decode_many": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_sz, c_vp]),
@@ -125,6 +140,7 @@ _SIGS = {
     "gchi_shard_encode": (c_i32, [c_vp, c_i32, c_vp, ctypes.POINTER(c_u64), c_vp, c_u32, ctypes.POINTER(c_u32),
                                   c_vp]),
     "gchi_shard_write_files": (c_i32, [c_vp, ctypes.c_char_p, c_u32, c_vp]),
+    "gchi_shard_write_files_ex": (c_i32, [c_vp, ctypes.c_char_p, c_u32, c_i32, c_vp]),
 }
 
 _lib = None
@@ -241,26 +257,67 @@ def rmat_device(scale, nedges, seed, abcd=RMAT_ABCD, drop_self_loops=True):
     return s1[:n], d1[:n], n, maxid
 
 
+# ---------------------------------------------------------------- text input
+def parse_edgelist(text):
+    """Edge-list bytes (host bytes or a device uint8 tensor) parsed on the
+    device: returns (src, dst, count, max_id) with self-loops dropped."""
+    torch = _torch()
+    L = lib()
+    if isinstance(text, (bytes, bytearray)):
+        t = torch.from_numpy(np.frombuffer(bytes(text), np.uint8).copy()).to("cuda") if text else None
+        n = len(text)
+    else:
+        t, n = text, int(text.numel())
+    nb = c_sz()
+    check(L.gchi_parse_edgelist_workspace(n, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    ne, mx = c_u64(), c_u32()
+    check(L.gchi_parse_edgelist(dptr(t), n, None, None, 0, ctypes.byref(ne), ctypes.byref(mx), dptr(ws), nb.value,
+                                stream()), "parse_edgelist (count)")
+    src, dst = empty_u32(ne.value), empty_u32(ne.value)
+    check(L.gchi_parse_edgelist(dptr(t), n, dptr(src), dptr(dst), max(ne.value, 1), ctypes.byref(ne),
+                                ctypes.byref(mx), dptr(ws), nb.value, stream()), "parse_edgelist")
+    return src[:ne.value], dst[:ne.value], ne.value, mx.value
+
+
 # ---------------------------------------------------------------- sort
-def sort_u64(keys, vals=None, key_bits=64):
+def work_status(ws, what="work_status"):
+    """Waits for the stream and raises if the call that used workspace ws
+    reported a device-side failure (status word in its first 4 bytes)."""
+    check(lib().gchi_work_status(dptr(ws), stream()), what)
+
+
+def sort_u64(keys, vals=None, key_bits=64, check_status=True):
     L = lib()
     n = keys.numel()
     nb = c_sz()
     check(L.gchi_sort_workspace(n, key_bits, 1 if vals is not None else 0, ctypes.byref(nb)))
     ws = workspace(nb.value)
     check(L.gchi_sort_u64(dptr(keys), dptr(vals), n, key_bits, dptr(ws), nb.value, stream()), "sort_u64")
+    if check_status:
+        work_status(ws, "sort_u64")
+    return ws
 
 
-def sort_pairs_dst_src(src, dst, id_bits):
+def sort_pairs_dst_src(src, dst, id_bits, check_status=True, out=None):
+    """(src, dst) sorted by (dst, src); out = (src_out, dst_out) to write into
+    given tensors (they may be src, dst themselves)."""
     L = lib()
     n = src.numel()
-    so, do = empty_u32(n), empty_u32(n)
+    so, do = out if out is not None else (empty_u32(n), empty_u32(n))
     nb = c_sz()
     check(L.gchi_sort_pairs_dst_src_workspace(n, id_bits, ctypes.byref(nb)))
     ws = workspace(nb.value)
     check(L.gchi_sort_pairs_dst_src(dptr(src), dptr(dst), n, id_bits, dptr(so), dptr(do), dptr(ws), nb.value,
                                     stream()), "sort_pairs_dst_src")
+    if check_status:
+        work_status(ws, "sort_pairs_dst_src")
     return so, do
+
+
+def debug_sort_lookback(spin_limit=0, force_stall=False):
+    """Test hook: bound of the sort's look-back waits (0 = default) and forced stalls."""
+    check(lib().gchi_debug_sort_lookback(spin_limit, 1 if force_stall else 0), "debug_sort_lookback")
 
 
 # ---------------------------------------------------------------- graph
@@ -281,6 +338,7 @@ class DeviceGraph:
         ws = workspace(nb.value)
         check(L.gchi_csc_build(dptr(src), dptr(dst), E, N, dptr(self.in_off), dptr(self.in_src), dptr(self.outdeg),
                                dptr(self.indeg), dptr(ws), nb.value, stream()), "csc_build")
+        work_status(ws, "csc_build")
         self.dg = DGraph(N, 0, E, self.in_off.data_ptr(), self.in_src.data_ptr(), self.outdeg.data_ptr(),
                          self.indeg.data_ptr())
 
@@ -406,6 +464,46 @@ class PageRankExec:
             pass
 
 
+class PageRankBlocked:
+    """Semi-synchronous sweeps over vertex blocks [bstart[j], bstart[j+1])
+    (gchi_pr_blocked_*): x is updated in place block by block."""
+
+    def __init__(self, graph, bstart, edges_per_block=0, short_row_max=0, persistent=0):
+        self.graph = graph
+        self.bstart = np.ascontiguousarray(bstart, dtype=np.uint32)
+        self.handle = c_vp()
+        tune = PrTune(edges_per_block, short_row_max, persistent, 0)
+        check(lib().gchi_pr_blocked_create(ctypes.byref(graph.dg), c_vp(self.bstart.ctypes.data),
+                                           len(self.bstart) - 1, ctypes.byref(tune), ctypes.byref(self.handle),
+                                           stream()), "pr_blocked_create")
+
+    def run(self, op, init, nsweeps, seed_bits=None, seed_value=0.0):
+        """init then nsweeps sweeps; returns (ranks, stats[nsweeps, 2])."""
+        torch = _torch()
+        n = self.graph.N
+        x = torch.empty(n, dtype=torch.float32, device="cuda")
+        r = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(2)]
+        stats = torch.zeros(max(2 * nsweeps, 2), dtype=torch.float64, device="cuda")
+        check(lib().gchi_pr_blocked_init(self.handle, init, dptr(seed_bits), seed_value, dptr(x), dptr(r[0]),
+                                         stream()), "pr_blocked_init")
+        for k in range(nsweeps):
+            check(lib().gchi_pr_blocked_sweep(self.handle, ctypes.byref(op), dptr(x), dptr(r[k & 1]),
+                                              dptr(r[(k + 1) & 1]), c_vp(stats.data_ptr() + 16 * k), stream()),
+                  "pr_blocked_sweep")
+        return r[nsweeps & 1], stats[:2 * nsweeps].reshape(-1, 2)
+
+    def close(self):
+        if self.handle:
+            lib().gchi_pr_blocked_destroy(self.handle)
+            self.handle = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- connected components
 CC_SYMMETRIC = 1
 
@@ -439,6 +537,36 @@ def cc_labelprop(src, dst, n, max_sweeps=100000):
         if int(changed.item()) == 0:
             return lab[(k + 1) & 1], k + 1
     raise GchiError("cc_labelprop: no fixed point within max_sweeps")
+
+
+def cc_pull(off, adj, n, max_sweeps=100000):
+    """Synchronous min-label sweeps pulled over a symmetric CSR until no label
+    drops; returns (labels, sweeps)."""
+    torch = _torch()
+    L = lib()
+    a = empty_u32(n)
+    b = empty_u32(n)
+    check(L.gchi_cc_labelprop_init(dptr(a), n, stream()), "cc_labelprop_init")
+    ch = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sweeps = 0
+    while sweeps < max_sweeps:
+        check(L.gchi_cc_pull_sweep(dptr(off), dptr(adj), n, dptr(a), dptr(b), dptr(ch), stream()), "cc_pull_sweep")
+        sweeps += 1
+        a, b = b, a
+        if int(ch.item()) == 0:
+            break
+    return a, sweeps
+
+
+def cc_afforest(off, adj, n):
+    """Min-id component labels from a symmetric CSR by sampled union-find."""
+    L = lib()
+    labels = empty_u32(n)
+    nb = c_sz()
+    check(L.gchi_cc_afforest_workspace(n, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    check(L.gchi_cc_afforest(dptr(off), dptr(adj), n, dptr(labels), dptr(ws), nb.value, stream()), "cc_afforest")
+    return labels
 
 
 def cc_scheduled(off, adj, n, max_rounds=0):
@@ -569,8 +697,9 @@ class Shards:
                                   ctypes.byref(ni), stream()), "shard_encode")
         return adj[:nbytes.value].tobytes(), idx[:3 * ni.value].reshape(-1, 3)
 
-    def write_files(self, base, edge_value_bytes):
-        check(lib().gchi_shard_write_files(self.handle, base.encode(), edge_value_bytes, stream()), "shard_write_files")
+    def write_files(self, base, edge_value_bytes, zlib=False):
+        check(lib().gchi_shard_write_files_ex(self.handle, base.encode(), edge_value_bytes, 1 if zlib else 0,
+                                              stream()), "shard_write_files")
 
     def close(self):
         if self.handle:
@@ -585,6 +714,25 @@ class Shards:
 
 
 # ---------------------------------------------------------------- personalized PageRank
+def ppr_workload_queries(n, nqueries=1024, seed=5, max_seeds=64):
+    """The batched-PPR workload: nqueries restart sets, each of 1..max_seeds
+    distinct vertices drawn uniformly from [0, n)."""
+    rng = np.random.default_rng(seed)
+    return [rng.choice(n, size=int(rng.integers(1, max_seeds + 1)), replace=False).astype(np.uint32)
+            for _ in range(nqueries)]
+
+
+def multigraph_workload(ngraphs=2048, seed=9):
+    """The many-graphs workload: graph i is an R-MAT graph of scale
+    scales[i] (uniform 10..20, so vertices log-uniform 2^10..2^20), edge
+    factor efs[i] (uniform 8..32), seed 10000 + i, self-loops dropped.
+    Returns [(scale, nedges_generated, seed)]."""
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 21, ngraphs)
+    efs = rng.integers(8, 33, ngraphs)
+    return [(int(scales[i]), int(efs[i]) << int(scales[i]), 10000 + i) for i in range(ngraphs)]
+
+
 def seed_csr(queries):
     off = np.zeros(len(queries) + 1, np.uint32)
     for i, q in enumerate(queries):

@@ -13,12 +13,14 @@
 // forms -- run as the device's synchronous pull sweep; any other kernel stops
 // with a fatal log (there is no CPU path).
 //
-// Iteration semantics follow the synchronous engine
-// (functional_bulksync.hpp:84-113): iteration 0 sets initial values and
-// scatters, iterations 1..niters-1 gather, apply and scatter, so niters
-// iterations are niters-1 sweeps.  The semi-synchronous runner uses the same
-// synchronous sweeps.
+// Iteration semantics: iteration 0 sets initial values and scatters,
+// iterations 1..niters-1 gather, apply and scatter, so niters iterations are
+// niters-1 sweeps.  The synchronous runner (functional_bulksync.hpp:84-113)
+// replays its sweeps from one HIP graph; the semi-synchronous runner
+// (functional_semisync.hpp:76-110) sweeps the engine's sub-intervals in order,
+// each seeing the values earlier ones wrote in the same sweep.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -186,8 +188,31 @@ inline void write_seed_bits(const std::vector<vid_t> &seeds, size_t n, device::b
     bits.upload(w.data(), w.size());
 }
 
+// The engine's sub-intervals for a functional kernel: each interval is cut into
+// windows of at most maxwindow + 1 vertices, maxwindow = membudget_mb * 2^20 /
+// 300 (src/engine/graphchi_engine.hpp:729-732, 862-869; determine_next_window
+// returns the window's upper bound for computational edges, :311-313).
+// Returns the block starts followed by n.
+inline std::vector<uint32_t> semisync_blocks(size_t n, const std::vector<vid_t> &interval_ends, int membudget_mb) {
+    const uint64_t maxwindow = (uint64_t)membudget_mb * 1024 * 1024 / 3 / 100;
+    std::vector<uint32_t> b;
+    uint64_t st = 0;
+    for (vid_t en : interval_ends) {
+        for (uint64_t sub = st; sub <= en;) {
+            b.push_back((uint32_t)sub);
+            sub = std::min<uint64_t>(sub + maxwindow, en) + 1;
+        }
+        st = (uint64_t)en + 1;
+    }
+    b.push_back((uint32_t)n);
+    return b;
+}
+
+// Loads the graph, identifies the kernel and runs niters iterations on the
+// device: synchronous sweeps replayed from one HIP graph, or semi-synchronous
+// block sweeps.
 template <class KERNEL>
-void run_functional_device(std::string filename, int niters, metrics &m) {
+void run_functional_device(std::string filename, int niters, metrics &m, bool semisync) {
     metrics_entry total = m.start_time();
     int nshards = convert_if_notexists<PairContainer<typename KERNEL::EdgeDataType>>(
         filename, get_option_string("nshards", "auto"));
@@ -204,14 +229,14 @@ void run_functional_device(std::string filename, int niters, metrics &m) {
     ctx.num_iterations = niters;
     affine_form f = identify<KERNEL>(ctx, indeg, outdeg);
     logstream(LOG_INFO) << "device operator: r = " << f.teleport << (f.seeds.empty() ? "" : " (+seed term)") << " + "
-                        << f.damping << " * sum(r/outdeg), init " << f.init << std::endl;
+                        << f.damping << " * sum(r/outdeg), init " << f.init << (semisync ? ", semi-synchronous" : "")
+                        << std::endl;
+    // launch shape: the library's default, or the autotuner's pick (autotune 1)
     gchi_pr_tune tune{0, 0, 0, 0};
-    if (get_option_int("autotune", 1)) {
+    if (get_option_int("autotune", 0)) {
         double ms = 0;
         device::check(gchi_autotune_pr(&g.dg, 3, &tune, &ms, nullptr, nullptr, 0, nullptr, nullptr), "autotune");
     }
-    gchi_pr_plan *plan = nullptr;
-    device::check(gchi_pr_plan_create(&g.dg, &tune, &plan, nullptr), "gchi_pr_plan_create");
     device::buffer<uint32_t> seed_bits;
     gchi_pr_op op{f.teleport, f.damping, nullptr, nullptr, 0.0};
     if (!f.seeds.empty()) {
@@ -219,23 +244,46 @@ void run_functional_device(std::string filename, int niters, metrics &m) {
         op.d_seed_bits = seed_bits.get();
         op.seed_teleport = f.seed_teleport;
     }
+    // the personalized form starts from the restart vector only when asked to
+    const bool seed_start = !f.seeds.empty() && f.init == 0.0f;
     const int sweeps = niters > 1 ? niters - 1 : 0;
     device::buffer<float> x0(n), x1(n), r0(n), r1(n);
     device::buffer<double> stats(2 * (size_t)(sweeps ? sweeps : 1));
     metrics_entry mu = m.start_time();
-    if (!f.seeds.empty() && f.init == 0.0f) {
-        // the personalized form starts from the restart vector only when asked to
-        device::check(gchi_pr_init_seeds(plan, seed_bits.get(), (float)f.seed_teleport, x0.get(), r0.get(), nullptr),
-                      "gchi_pr_init_seeds");
+    if (semisync) {
+        const std::vector<uint32_t> blocks =
+            semisync_blocks(n, load_interval_ends(filename, nshards), get_option_int("membudget_mb", 1024));
+        gchi_pr_blocked *bl = nullptr;
+        device::check(gchi_pr_blocked_create(&g.dg, blocks.data(), (uint32_t)blocks.size() - 1, &tune, &bl, nullptr),
+                      "gchi_pr_blocked_create");
+        m.set("semisync_blocks", (double)(blocks.size() - 1));
+        device::check(gchi_pr_blocked_init(bl, f.init, seed_start ? seed_bits.get() : nullptr, (float)f.seed_teleport,
+                                           x0.get(), r0.get(), nullptr),
+                      "gchi_pr_blocked_init");
+        float *rs[2] = {r0.get(), r1.get()};
+        for (int k = 0; k < sweeps; k++)
+            device::check(gchi_pr_blocked_sweep(bl, &op, x0.get(), rs[k & 1], rs[(k + 1) & 1], stats.get() + 2 * k,
+                                                nullptr),
+                          "gchi_pr_blocked_sweep");
+        device::check(gchi_synchronize(nullptr), "sync");
+        gchi_pr_blocked_destroy(bl);
     } else {
-        device::check(gchi_pr_init(plan, f.init, x0.get(), r0.get(), nullptr), "gchi_pr_init");
+        gchi_pr_plan *plan = nullptr;
+        device::check(gchi_pr_plan_create(&g.dg, &tune, &plan, nullptr), "gchi_pr_plan_create");
+        if (seed_start)
+            device::check(gchi_pr_init_seeds(plan, seed_bits.get(), (float)f.seed_teleport, x0.get(), r0.get(),
+                                             nullptr),
+                          "gchi_pr_init_seeds");
+        // iterations 1..niters-1 captured once into a HIP graph and replayed
+        gchi_pr_exec *ex = nullptr;
+        device::check(gchi_pr_exec_create_ex(plan, &op, f.init, seed_start ? 0 : 1, sweeps, x0.get(), x1.get(),
+                                             r0.get(), r1.get(), stats.get(), &ex),
+                      "gchi_pr_exec_create");
+        device::check(gchi_pr_exec_launch(ex, nullptr), "gchi_pr_exec_launch");
+        device::check(gchi_synchronize(nullptr), "sync");
+        gchi_pr_exec_destroy(ex);
+        gchi_pr_plan_destroy(plan);
     }
-    float *xs[2] = {x0.get(), x1.get()}, *rs[2] = {r0.get(), r1.get()};
-    for (int k = 0; k < sweeps; k++)
-        device::check(gchi_pr_sweep(plan, &op, xs[k & 1], rs[k & 1], xs[(k + 1) & 1], rs[(k + 1) & 1],
-                                    stats.get() + 2 * k, nullptr),
-                      "gchi_pr_sweep");
-    device::check(gchi_synchronize(nullptr), "sync");
     m.stop_time(mu, "execute-updates");
     std::vector<float> r = (sweeps & 1 ? r1 : r0).to_host(n);
     if (sweeps) {
@@ -243,7 +291,6 @@ void run_functional_device(std::string filename, int niters, metrics &m) {
         m.set("last_residual", st[2 * (sweeps - 1)]);
         m.set("rank_sum", st[2 * (sweeps - 1) + 1]);
     }
-    gchi_pr_plan_destroy(plan);
     write_vertex_values<float>(filename, r);
     m.set("iterations", (double)niters);
     m.stop_time(total, "runtime");
@@ -253,13 +300,12 @@ void run_functional_device(std::string filename, int niters, metrics &m) {
 
 template <class KERNEL>
 void run_functional_unweighted_synchronous(std::string filename, int niters, metrics &m) {
-    detail::run_functional_device<KERNEL>(filename, niters, m);
+    detail::run_functional_device<KERNEL>(filename, niters, m, false);
 }
 
 template <class KERNEL>
 void run_functional_unweighted_semisynchronous(std::string filename, int niters, metrics &m) {
-    logstream(LOG_WARNING) << "semi-synchronous mode runs synchronous device sweeps (same fixed point)" << std::endl;
-    detail::run_functional_device<KERNEL>(filename, niters, m);
+    detail::run_functional_device<KERNEL>(filename, niters, m, true);
 }
 
 }  // namespace graphchi

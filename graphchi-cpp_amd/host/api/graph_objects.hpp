@@ -9,6 +9,25 @@
 
 namespace graphchi {
 
+// Deleted edges carry a marker value in their edge data (the convention of
+// src/api/graph_objects.hpp:96-140 under SUPPORT_DELETIONS): true for bool,
+// all ones for 32-bit integers, zero for float.
+inline bool is_deleted_edge_value(bool v) { return v; }
+inline bool is_deleted_edge_value(int v) { return (unsigned int)v == 0xffffffffu; }
+inline bool is_deleted_edge_value(unsigned int v) { return v == 0xffffffffu; }
+inline bool is_deleted_edge_value(float v) { return !(v < 0 || v > 0); }
+
+template <typename T>
+inline T deleted_edge_value();
+template <>
+inline bool deleted_edge_value<bool>() { return true; }
+template <>
+inline int deleted_edge_value<int>() { return -1; }
+template <>
+inline unsigned int deleted_edge_value<unsigned int>() { return 0xffffffffu; }
+template <>
+inline float deleted_edge_value<float>() { return 0.0f; }
+
 template <typename EdgeDataType>
 class graphchi_edge {
   public:
@@ -59,7 +78,8 @@ class graphchi_vertex {
         std::sort(inc.begin(), inc.end(), by_id);
         std::sort(outc.begin(), outc.end(), by_id);
     }
-    void remove_edge(int) {}
+    // Marks edge i deleted (its data takes the deletion marker).
+    void remove_edge(int i) { edge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
     bool computational_edges() { return false; }
     static bool read_outedges() { return true; }
 };

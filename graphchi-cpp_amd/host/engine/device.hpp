@@ -80,6 +80,35 @@ class buffer {
     size_t n_ = 0;
 };
 
+// Pinned host memory (gchi_host_alloc): what the one-time uploads read from.
+template <typename T>
+class pinned {
+  public:
+    pinned() {}
+    explicit pinned(size_t n) { resize(n); }
+    ~pinned() { release(); }
+    pinned(const pinned &) = delete;
+    pinned &operator=(const pinned &) = delete;
+    void resize(size_t n) {
+        release();
+        void *p = nullptr;
+        check(gchi_host_alloc(&p, (n ? n : 1) * sizeof(T)), "gchi_host_alloc");
+        p_ = static_cast<T *>(p);
+        n_ = n;
+    }
+    void release() {
+        if (p_) gchi_host_free(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *get() const { return p_; }
+    size_t size() const { return n_; }
+
+  private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
 class workspace : public buffer<uint8_t> {
   public:
     explicit workspace(size_t n) : buffer<uint8_t>(n) {}
@@ -123,7 +152,7 @@ struct graph {
         check(gchi_csc_build(src.get(), dst.get(), e, n, in_off.get(), in_src.get(), outdeg.get(), indeg.get(),
                              w.get(), ws, nullptr),
               "gchi_csc_build");
-        check(gchi_synchronize(nullptr), "sync");
+        check(gchi_work_status(w.get(), nullptr), "gchi_csc_build status");  // waits for the stream
         dg = gchi_dgraph{n, 0, e, in_off.get(), in_src.get(), outdeg.get(), indeg.get()};
         if (!keep_edges) {
             src.release();

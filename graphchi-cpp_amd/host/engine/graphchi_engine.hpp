@@ -18,6 +18,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
+#include <cstdio>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -119,26 +121,120 @@ class graphchi_engine {
         run_impl(program, typename device_program<ProgramT>::impl());
     }
 
+    // The program's update() is replaced by the registered device algorithm;
+    // the iteration hooks still run between rounds.  A program whose update()
+    // sets a public `converged` flag false when it writes a label (as the
+    // reference's ConnectedComponentsProgram does, connectedcomponents.cpp:106-
+    // 116) gets that side effect from the round's result.
+    template <class P>
+    static auto set_converged(P &p, bool v, int) -> decltype(p.converged = v, void()) {
+        p.converged = v;
+    }
+    template <class P>
+    static void set_converged(P &, bool, long) {}
+
+    template <class ProgramT>
+    struct cc_rounds {
+        graphchi_engine *eng;
+        ProgramT *program;
+        int niters;
+        uint64_t updates = 0, work = 0;
+        static int hook(void *user, uint32_t round, uint64_t dropped, uint64_t evaluated) {
+            cc_rounds *c = static_cast<cc_rounds *>(user);
+            graphchi_context &ctx = c->eng->ctx_;
+            ctx.iteration = (int)round;
+            if (round > 0 && dropped) set_converged(*c->program, false, 0);
+            c->updates += evaluated;
+            c->work += c->eng->nedges_;  // every interval's edges are loaded each iteration
+            c->program->after_iteration((int)round, ctx);
+            c->eng->write_delta_log((int)round, c->updates, c->work);
+            const bool last = (ctx.last_iteration >= 0 && (int)round >= ctx.last_iteration) ||
+                              (int)round + 1 >= c->niters || dropped == 0;
+            if (last) return 1;
+            ctx.iteration = (int)round + 1;
+            c->program->before_iteration((int)round + 1, ctx);
+            return 0;
+        }
+    };
+
+    // connectedcomponents.cpp:79-121 on a three-vertex probe: iteration 0 takes
+    // the minimum of the vertex's and its neighbours' ids and writes it to the
+    // out-edges; later iterations take the minimum over the edge labels and
+    // write it to the edges holding larger ones.  A program registered as
+    // min-label components whose update() does not do that is refused.
+    template <class ProgramT>
+    void probe_min_label(ProgramT &program) {
+        ischeduler probe_sched;
+        graphchi_context pc;
+        pc.scheduler = &probe_sched;
+        pc.nvertices = 10;
+        uint32_t in_val = 7, out_val = 8, label = 0;
+        graphchi_vertex<VertexDataType, EdgeDataType> v;
+        v.vertexid = 5;
+        v.dataptr = reinterpret_cast<VertexDataType *>(&label);
+        v.add_inedge(3, reinterpret_cast<EdgeDataType *>(&in_val), false);
+        v.add_outedge(9, reinterpret_cast<EdgeDataType *>(&out_val), false);
+        ProgramT p = program;
+        pc.iteration = 0;
+        p.update(v, pc);
+        const bool ok0 = label == 3 && out_val == 3;
+        in_val = 1;
+        out_val = 4;
+        pc.iteration = 1;
+        p.update(v, pc);
+        const bool ok1 = label == 1 && out_val == 1 && in_val == 1;
+        if (!ok0 || !ok1)
+            logstream(LOG_FATAL) << "the program registered as min-label connected components does not compute "
+                                 << "min-label propagation in its update(); refusing to run the device algorithm"
+                                 << std::endl;
+    }
+
     template <class ProgramT>
     void run_impl(ProgramT &program, device::min_label_components) {
         static_assert(sizeof(VertexDataType) == 4, "min-label components writes 4-byte labels");
+        probe_min_label(program);
         device::graph g = load_graph();
+        const uint32_t n = (uint32_t)nvertices_;
+        // the undirected adjacency (every edge in both rows) as a CSR: the
+        // update() reads in- and out-edges alike
+        const uint64_t e2 = 2 * g.e;
+        device::buffer<uint32_t> s2(e2 ? e2 : 1), d2(e2 ? e2 : 1);
+        if (g.e) {
+            device::check(gchi_memcpy(s2.get(), g.src.get(), 4 * g.e, nullptr), "copy");
+            device::check(gchi_memcpy(s2.get() + g.e, g.dst.get(), 4 * g.e, nullptr), "copy");
+            device::check(gchi_memcpy(d2.get(), g.dst.get(), 4 * g.e, nullptr), "copy");
+            device::check(gchi_memcpy(d2.get() + g.e, g.src.get(), 4 * g.e, nullptr), "copy");
+        }
+        device::graph sym(std::move(s2), std::move(d2), e2, n, false);
         ctx_.iteration = 0;
+        ctx_.last_iteration = -1;
         program.before_iteration(0, ctx_);
         metrics_entry me = m_.start_time();
         device::buffer<uint32_t> labels(nvertices_);
         size_t ws = 0;
-        device::check(gchi_cc_workspace((uint32_t)nvertices_, &ws), "cc workspace");
+        device::check(gchi_cc_scheduled_workspace(n, e2, &ws), "cc workspace");
         device::workspace w(ws);
-        device::check(gchi_cc_minlabel(g.src.get(), g.dst.get(), g.e, (uint32_t)nvertices_, labels.get(), w.get(), ws,
-                                       nullptr),
-                      "gchi_cc_minlabel");
-        device::check(gchi_synchronize(nullptr), "sync");
+        cc_rounds<ProgramT> hooks{this, &program, ctx_.num_iterations > 0 ? ctx_.num_iterations : 1};
+        uint32_t rounds = 0;
+        uint64_t evaluated = 0;
+        device::check(gchi_cc_scheduled_ex(sym.in_off.get(), sym.in_src.get(), n, 0, labels.get(), &rounds, &evaluated,
+                                           &cc_rounds<ProgramT>::hook, &hooks, w.get(), ws, nullptr),
+                      "gchi_cc_scheduled");
         m_.stop_time(me, "execute-updates");
-        program.after_iteration(0, ctx_);
+        m_.set("iterations", (double)rounds);
+        m_.set("updates", (double)evaluated);
         std::vector<uint32_t> h = labels.to_host(nvertices_);
         std::vector<VertexDataType> out(h.begin(), h.end());
         write_vertex_values<VertexDataType>(base_, out);
+    }
+
+    // <base>.deltalog, one line per iteration: iteration, updates, work, delta
+    // (src/engine/graphchi_engine.hpp:593-599).
+    void write_delta_log(int iteration, uint64_t updates, uint64_t work) {
+        FILE *df = fopen((base_ + ".deltalog").c_str(), iteration == 0 ? "w" : "a");
+        if (!df) return;
+        fprintf(df, "%d,%lu,%lu,%lf\n", iteration, (unsigned long)updates, (unsigned long)work, ctx_.get_delta());
+        fclose(df);
     }
 
     template <class ProgramT>

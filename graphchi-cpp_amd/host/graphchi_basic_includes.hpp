@@ -17,3 +17,4 @@
 #include "metrics/metrics.hpp"
 #include "preprocessing/conversions.hpp"
 #include "util/cmdopts.hpp"
+#include "util/pthread_tools.hpp"

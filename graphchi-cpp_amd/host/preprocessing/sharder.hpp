@@ -57,8 +57,15 @@ class sharder {
         }
         gchi_shards *sh = nullptr;
         device::check(gchi_shard_build(s.get(), d.get(), e, max_id_, nshards, &sh, nullptr), "gchi_shard_build");
-        int st = gchi_shard_write_files(sh, base_.c_str(), no_values_ ? 0 : (uint32_t)sizeof(FinalEdgeDataType),
-                                        nullptr);
+        // edge-data blocks zlib-compressed unless the application opted out, as
+        // the reference's sharder (src/util/ioutil.hpp:144-199)
+#ifndef GRAPHCHI_DISABLE_COMPRESSION
+        const int wflags = GCHI_SHARD_ZLIB;
+#else
+        const int wflags = 0;
+#endif
+        int st = gchi_shard_write_files_ex(sh, base_.c_str(), no_values_ ? 0 : (uint32_t)sizeof(FinalEdgeDataType),
+                                           wflags, nullptr);
         int created = 0;
         gchi_shard_info(sh, &created, nullptr, nullptr);
         gchi_shard_destroy(sh);

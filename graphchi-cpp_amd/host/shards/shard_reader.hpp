@@ -7,6 +7,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -74,9 +75,29 @@ inline void load_shard_edges(const std::string &base, int nshards, std::vector<v
     }
 }
 
-// The device load path: every shard's .adj bytes are uploaded once and decoded
-// in HBM, all shards in one call (gchi_shard_decode_many), and the edges stay
-// there for the device build.
+inline uint64_t file_size_or_die(const std::string &path) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) logstream(LOG_FATAL) << "could not open " << path << std::endl;
+    fseek(f, 0, SEEK_END);
+    long n = ftell(f);
+    fclose(f);
+    return n > 0 ? (uint64_t)n : 0;
+}
+
+// Reads the first n bytes of a file into pinned host memory (the upload source).
+inline void read_file_pinned(const std::string &path, uint64_t n, device::pinned<uint8_t> &buf) {
+    if (buf.size() < n) buf.resize(n);
+    if (!n) return;
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) logstream(LOG_FATAL) << "could not open " << path << std::endl;
+    const size_t got = fread(buf.get(), 1, n, f);
+    fclose(f);
+    if (got != n) logstream(LOG_FATAL) << "short read from " << path << std::endl;
+}
+
+// The device load path: every shard's .adj bytes are read into pinned host
+// memory, uploaded once and decoded in HBM, all shards in one call
+// (gchi_shard_decode_many), and the edges stay there for the device build.
 // Same edge sequence as load_shard_edges.  Returns the edge count.
 inline uint64_t load_shard_edges_device(const std::string &base, int nshards, device::buffer<uint32_t> &src,
                                         device::buffer<uint32_t> &dst) {
@@ -84,11 +105,16 @@ inline uint64_t load_shard_edges_device(const std::string &base, int nshards, de
     std::vector<vid_t> ends = load_interval_ends(base, nshards);
     std::vector<device::buffer<uint8_t>> bytes(ends.size());
     std::vector<uint64_t> nbytes(ends.size()), count(ends.size());
+    uint64_t largest = 0;
     for (size_t p = 0; p < ends.size(); p++) {
-        std::vector<uint8_t> adj = read_file_bytes(filename_shard_adj(base, (int)p, nshards));
-        nbytes[p] = adj.size();
-        bytes[p].resize(adj.size());
-        if (!adj.empty()) bytes[p].upload(adj.data(), adj.size());
+        nbytes[p] = file_size_or_die(filename_shard_adj(base, (int)p, nshards));
+        largest = std::max(largest, nbytes[p]);
+    }
+    device::pinned<uint8_t> staging(largest);
+    for (size_t p = 0; p < ends.size(); p++) {
+        read_file_pinned(filename_shard_adj(base, (int)p, nshards), nbytes[p], staging);
+        bytes[p].resize(nbytes[p]);
+        if (nbytes[p]) bytes[p].upload(staging.get(), nbytes[p]);
     }
     std::vector<const uint8_t *> ptrs(ends.size());
     for (size_t p = 0; p < ends.size(); p++) ptrs[p] = bytes[p].get();

@@ -1,7 +1,8 @@
 // Command-line and configuration options (src/util/cmdopts.hpp:112-249,
 // src/util/configfile.hpp:83-119): "key value" pairs or --key=value on the
 // command line override conf/graphchi.local.cnf or conf/graphchi.cnf under
-// $GRAPHCHI_ROOT (or the working directory).  A missing file is not an error.
+// $GRAPHCHI_ROOT (or the working directory).  A missing file is not an error:
+// the reference's shipped settings are the defaults.
 #pragma once
 #include <cstdlib>
 #include <fstream>
@@ -51,6 +52,15 @@ inline void graphchi_init(int argc, const char **argv) {
     std::string dir = root ? std::string(root) : std::string(".");
     load_config_file(dir + "/conf/graphchi.local.cnf");
     load_config_file(dir + "/conf/graphchi.cnf");
+    // Without a configuration file the values the reference ships in
+    // conf/graphchi.cnf apply (it refuses to start without that file,
+    // src/util/configfile.hpp:83-92): membudget_mb sets the automatic shard
+    // count (sharder.hpp:565-575), so it must be 800 here too.
+    static const char *const defaults[][2] = {{"loadthreads", "4"},      {"niothreads", "2"},
+                                              {"membudget_mb", "800"},   {"cachesize_mb", "0"},
+                                              {"io.blocksize", "1048576"}, {"mmap", "0"}};
+    for (const auto &kv : defaults)
+        if (!t.count(kv[0])) t[kv[0]] = kv[1];
 }
 
 inline void set_conf(std::string key, std::string value) { option_table()[key] = value; }

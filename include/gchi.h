@@ -57,6 +57,11 @@ int gchi_device_info(int *num_cus, char *arch, size_t arch_len);
 int gchi_malloc(void **d_ptr, size_t bytes);
 int gchi_free(void *d_ptr);
 int gchi_synchronize(void *stream);
+/* Pinned (page-locked) host memory for the one-time shard upload: the engines
+ * read shard files straight into it, so the upload runs at DMA speed.  Replaces
+ * the pread buffers of src/io/stripedio.hpp:480-640. */
+int gchi_host_alloc(void **h_ptr, size_t bytes);
+int gchi_host_free(void *h_ptr);
 /* Copies bytes between any host/device pointers on the stream and waits for it
  * (the pinned one-time upload/readback of the engine). */
 int gchi_memcpy(void *dst, const void *src, size_t bytes, void *stream);
@@ -79,6 +84,20 @@ int gchi_drop_self_loops(const uint32_t *d_src, const uint32_t *d_dst, uint64_t 
                          uint32_t *d_src_out, uint32_t *d_dst_out, uint64_t *d_count,
                          uint32_t *d_maxid, void *d_work, size_t work_bytes, void *stream);
 
+/* Edge-list text parsed on the device: convert_edgelist
+ * (src/preprocessing/conversions.hpp:210-288) feeding preprocessing_add_edge
+ * (src/preprocessing/sharder.hpp:427-446).  d_text holds the file's bytes.
+ * Lines whose first byte is '#' or '%' are comments; other lines split on tab,
+ * comma and space, from = atoi(field 1), to = atoi(field 2) (further fields are
+ * edge values and ignored here); self-loops are dropped.  The kept pairs are
+ * written in file order to d_src/d_dst (up to max_edges; 0 = count only),
+ * *h_nedges = their number, *h_maxid = the largest id among them.  A line with
+ * fewer than two fields, or of 1023 bytes or more, is GCHI_ERR_FORMAT (the
+ * reference asserts on the first and splits the second).  Host-synchronous. */
+int gchi_parse_edgelist_workspace(uint64_t nbytes, size_t *bytes);
+int gchi_parse_edgelist(const uint8_t *d_text, uint64_t nbytes, uint32_t *d_src, uint32_t *d_dst, uint64_t max_edges,
+                        uint64_t *h_nedges, uint32_t *h_maxid, void *d_work, size_t work_bytes, void *stream);
+
 /* ---------------------------------------------------------------- radix sort */
 /* Stable LSD onesweep radix sort of 64-bit keys on their low key_bits bits
  * (8-bit digits; optional 32-bit payload).  Replaces iSort/radixStep/eBits of
@@ -87,6 +106,23 @@ int gchi_drop_self_loops(const uint32_t *d_src, const uint32_t *d_dst, uint64_t 
 int gchi_sort_workspace(uint64_t n, int key_bits, int with_values, size_t *bytes);
 int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits,
                   void *d_work, size_t work_bytes, void *stream);
+
+/* Device-side status of an asynchronous call.  gchi_sort_u64,
+ * gchi_sort_pairs_dst_src and gchi_csc_build keep a status word in the first 4
+ * bytes of their workspace: GCHI_WORK_BAD_ID when an input vertex id was out of
+ * range, GCHI_WORK_TIMEOUT when a bounded device wait (the sort's decoupled
+ * look-back) gave up -- the output is then invalid.  gchi_work_status waits for
+ * the stream and returns GCHI_OK, GCHI_ERR_INVALID or GCHI_ERR_TIMEOUT.  Calls
+ * that are host-synchronous anyway (gchi_shard_build, gchi_tc_prepare,
+ * gchi_degree_order, gchi_pr_plan_create*) check their word themselves. */
+#define GCHI_WORK_BAD_ID 1u
+#define GCHI_WORK_TIMEOUT 2u
+int gchi_work_status(const void *d_work, void *stream);
+/* Test hook: bound of every sort look-back wait in polls (0 = the default,
+ * 2^26) and, with force_stall, waits that treat every predecessor as not yet
+ * published, so that the bound is reached and the status reports it while
+ * every kernel still exits.  Process-wide; not for production use. */
+int gchi_debug_sort_lookback(uint32_t spin_limit, int force_stall);
 
 /* Sorts uint32 (src,dst) pairs by (dst,src) -- the order the sharder's shovel
  * sort (sharder.hpp:192, dst key) followed by a full tie order produces.
@@ -148,7 +184,16 @@ typedef struct gchi_pr_tune {
                               fixed list of tiles, software-pipelined (W = 256),
                               optionally with the LDS cache of hot_vertices; 3: four
                               persistent 256-thread workgroups per CU with a static
-                              tile list, each with its own (smaller) LDS cache */
+                              tile list, each with its own (smaller) LDS cache;
+                              4: the rank-space tiled sweep -- vertices relabelled by
+                              out-degree, every row's in-edges packed into 512-slot
+                              tiles reduced by one wavefront each (segmented sums),
+                              rows crossing tiles summed from per-tile partials; the
+                              first hot_vertices ranks stay in LDS.  edges_per_block
+                              and short_row_max do not apply; one graph only
+                              (gchi_pr_plan_create); a HIP-graph exec keeps the state
+                              in rank order between sweeps, a standalone sweep maps its
+                              vertex-order buffers in and out.  N < 2^31 */
     int hot_vertices;      /* 0: gathers read x by vertex id.  Otherwise the plan keeps a
                               gather layout: sources ranked by out-degree (descending,
                               ties by id), edges store the rank, and every sweep first
@@ -236,6 +281,33 @@ int gchi_pr_exec_create_ex(const gchi_pr_plan *plan, const gchi_pr_op *op, float
 int gchi_pr_exec_launch(gchi_pr_exec *exec, void *stream);
 int gchi_pr_exec_destroy(gchi_pr_exec *exec);
 
+/* Semi-synchronous sweeps: the default mode of the reference's
+ * pagerank_functional (run_functional_unweighted_semisynchronous,
+ * src/api/functional/functional_api.hpp:119-137; vertex program
+ * src/api/functional/functional_semisync.hpp:76-110; phase order
+ * src/engine/functional/functional_engine.hpp:50-121; interval and
+ * sub-interval loop src/engine/graphchi_engine.hpp:825-921).  Every edge holds
+ * one value, overwritten when its source is updated, and the vertex blocks
+ * [h_bstart[j], h_bstart[j+1]) -- the engine's sub-intervals, in execution
+ * order -- run one after another: block j gathers the x written this sweep by
+ * the blocks before it and the previous sweep's x of the rest, then applies and
+ * scatters.  One block is the synchronous sweep.  Each block gets a plan over
+ * its rows built with the tune's shape (vertex-id gathers: hot_vertices is
+ * ignored); the graph's arrays must outlive the object. */
+typedef struct gchi_pr_blocked gchi_pr_blocked;
+int gchi_pr_blocked_create(const gchi_dgraph *g, const uint32_t *h_bstart, uint32_t nblocks, const gchi_pr_tune *tune,
+                           gchi_pr_blocked **out, void *stream);
+int gchi_pr_blocked_destroy(gchi_pr_blocked *b);
+/* Iteration 0: r = init and x = r/outdeg, or with d_seed_bits r = seed_value
+ * on the seeds and 0 elsewhere. */
+int gchi_pr_blocked_init(const gchi_pr_blocked *b, float init, const uint32_t *d_seed_bits, float seed_value,
+                         float *d_x, float *d_r, void *stream);
+/* One sweep: d_x is updated in place, block by block; ranks go from d_r_in to
+ * d_r_out (distinct); d_stats[0..1] = sum |r_out - r_in| and sum r_out (FP64,
+ * block by block in order).  op->d_graph_teleport must be NULL. */
+int gchi_pr_blocked_sweep(const gchi_pr_blocked *b, const gchi_pr_op *op, float *d_x, const float *d_r_in,
+                          float *d_r_out, double *d_stats, void *stream);
+
 /* Autotuner: times every candidate launch shape (edges_per_block in
  * {512..4096}, short_row_max in {8..64}, persistent 0/1, the
  * wavefront-tiled sweep, W = 256, with hot_vertices in {0, -1, 16384, max},
@@ -256,7 +328,12 @@ int gchi_autotune_pr(const gchi_dgraph *g, int sweeps_per_candidate, gchi_pr_tun
  * bf16_sweeps sweeps or until every query's L1 residual <= bf16_switch_residual
  * or stops falling; phase 2 refines in FP32 until every query satisfies
  * residual * d/(1-d) <= tol (a bound on the L1 distance to the fixed point,
- * iterates being monotone).  Residuals are FP64. */
+ * iterates being monotone).  Residuals are FP64.  Sweeps run in chunks of
+ * check_every (rounded down to even, at least 2), each chunk one HIP-graph
+ * launch (captured once per phase and batch) whose last sweep computes the
+ * residuals, the only host readback; a query block whose 64 queries all meet
+ * the bound leaves the work queue, so later chunks sweep only the rest.
+ * max_sweeps is honoured in whole chunks. */
 typedef struct gchi_ppr_config {
     double damping;
     double tol;
@@ -270,7 +347,7 @@ typedef struct gchi_ppr_result {
     int sweeps_bf16;
     int sweeps_fp32;
     int converged;
-    int reserved;
+    int block_sweeps_fp32;  /* FP32 sweeps summed over the query blocks (a converged block stops) */
     double max_residual;
 } gchi_ppr_result;
 typedef struct gchi_ppr_batch gchi_ppr_batch;
@@ -307,6 +384,13 @@ int gchi_cc_labelprop_init(uint32_t *d_labels, uint32_t nvertices, void *stream)
 int gchi_cc_labelprop_sweep(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                             const uint32_t *d_labels_in, uint32_t *d_labels_out, uint64_t *d_changed, void *stream);
 
+/* The same synchronous sweep pulled over a symmetric CSR (d_off[N+1], d_adj;
+ * every edge in both rows): d_labels_out[v] = min(d_labels_in[v], d_labels_in[u]
+ * for every neighbour u), one write per vertex, no atomics on labels;
+ * *d_changed = labels that dropped. */
+int gchi_cc_pull_sweep(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, const uint32_t *d_labels_in,
+                       uint32_t *d_labels_out, uint64_t *d_changed, void *stream);
+
 /* The reference's CC under its selective scheduler (ConnectedComponentsProgram
  * with scheduler on, connectedcomponents.cpp:79-121 and :81,85,112; the two
  * bitsets of src/engine/bitset_scheduler.hpp:38-104): round 0 evaluates every
@@ -323,6 +407,27 @@ int gchi_cc_scheduled_workspace(uint32_t nvertices, uint64_t nedges, size_t *byt
 int gchi_cc_scheduled(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
                       uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, void *d_work, size_t work_bytes,
                       void *stream);
+
+/* The same labels from a symmetric CSR (d_off[N+1], d_adj; every edge in both
+ * rows) by union-find with subgraph sampling (Afforest, Sutton et al., IPDPS
+ * 2018): each vertex is linked to its first two neighbours, the component most
+ * sampled vertices fall in is found, and only the vertices outside it link
+ * their remaining neighbours -- the giant component's edges are never read.
+ * Host-synchronous (reads 1,024 sampled roots, then a status word). */
+int gchi_cc_afforest_workspace(uint32_t nvertices, size_t *bytes);
+int gchi_cc_afforest(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_labels,
+                     void *d_work, size_t work_bytes, void *stream);
+
+/* As gchi_cc_scheduled, reporting each round to on_round (host, between
+ * rounds): round index from 0, labels that dropped in it, vertices evaluated in
+ * it.  A nonzero return stops after that round (not an error; the labels are
+ * those of the rounds run).  This is the engine's per-iteration hook
+ * (before/after_iteration and set_last_iteration of
+ * src/engine/graphchi_engine.hpp:779-956). */
+typedef int (*gchi_cc_round_fn)(void *user, uint32_t round, uint64_t dropped, uint64_t evaluated);
+int gchi_cc_scheduled_ex(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t max_rounds,
+                         uint32_t *d_labels, uint32_t *h_rounds, uint64_t *h_evaluated, gchi_cc_round_fn on_round,
+                         void *user, void *d_work, size_t work_bytes, void *stream);
 
 /* ------------------------------------------------------------ triangle counting */
 /* Degree ordering (order_by_degree, src/preprocessing/util/orderbydegree.hpp:58-100:
@@ -373,6 +478,14 @@ int gchi_shard_encode(const gchi_shards *shards, int shard, uint8_t *h_adj, uint
  * uncompressed default-valued edge-data blocks + .size. */
 int gchi_shard_write_files(const gchi_shards *shards, const char *basefilename, uint32_t edge_value_bytes,
                            void *stream);
+/* As gchi_shard_write_files; flags GCHI_SHARD_ZLIB writes the edge-data blocks
+ * zlib-compressed (deflate level 1, one stream per block) into the ".edata..Z."
+ * directories -- write_compressed of src/util/ioutil.hpp:144-199, what the
+ * reference's sharder does unless the application defines
+ * GRAPHCHI_DISABLE_COMPRESSION (connectedcomponents does not). */
+#define GCHI_SHARD_ZLIB 1
+int gchi_shard_write_files_ex(const gchi_shards *shards, const char *basefilename, uint32_t edge_value_bytes,
+                              int flags, void *stream);
 /* Decodes one shard's .adj byte stream, already in HBM, into (src, dst) pairs
  * in stream order -- the parse of memory_shard::load_vertices
  * (src/shards/memoryshard.hpp:448-497): a 0 byte and the byte nz after it skip

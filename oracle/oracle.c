@@ -9,6 +9,11 @@
  *                            (sync mode) and of the 0.15+0.85*s harness kernel,
  *                            built from the reference sources (oracle/Makefile);
  *                            closed forms (no in-edges -> teleport; chain/star)
+ *   orc_pagerank_semisync_f32 - golden .vout of the reference pagerank_functional
+ *                            in its default (semisync) mode, one and several
+ *                            sub-intervals per interval; one block == sync
+ *   orc_semisync_blocks    - the same goldens (a wrong cut changes the ranks);
+ *                            hand cases of the window rule
  *   orc_pagerank_sync_f64  - equals the f32 variant to FP32 rounding; linear-
  *                            system residual of its fixed point
  *   orc_ppr_f64            - fixed-point residual ||r - (t + aP'r)||_1; sum rule
@@ -108,6 +113,86 @@ int orc_pagerank_sync_f32(uint32_t n, const uint32_t *in_off, const uint32_t *in
     free(xn);
     free(r);
     return 0;
+}
+
+/* Semi-synchronous form: run_functional_unweighted_semisynchronous
+ * (src/api/functional/functional_api.hpp:119-137) with the vertex program of
+ * src/api/functional/functional_semisync.hpp:76-110 under the interval loop of
+ * src/engine/graphchi_engine.hpp:825-921 and the phase order of
+ * src/engine/functional/functional_engine.hpp:50-121.  Every edge holds one
+ * value, overwritten when its source is updated; blocks (the engine's
+ * sub-intervals) run in order, each gathering all its in-edges before any of
+ * its vertices is updated and scattered.  So block j sees this sweep's x of
+ * sources in earlier blocks and the previous sweep's x of the others.
+ * bstart[0..nblocks] are the block boundaries (bstart[nblocks] = n). */
+int orc_pagerank_semisync_f32(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                              uint32_t nblocks, const uint32_t *bstart, double teleport, double damping, float init,
+                              int nsweeps, float *r_out, double *stats) {
+    if (nblocks == 0 || bstart[0] != 0 || bstart[nblocks] != n) return -1;
+    float *x = (float *)malloc((size_t)n * 4), *r = (float *)malloc((size_t)n * 4);
+    float *rn = (float *)malloc((size_t)n * 4);
+    if (!x || !r || !rn) return -1;
+    for (uint32_t v = 0; v < n; v++) {
+        r[v] = init; /* iteration 0: first_iteration() then the scatter, :86-88, :104-106 */
+        x[v] = outdeg[v] ? init / (float)outdeg[v] : 0.0f;
+    }
+    for (int it = 0; it < nsweeps; it++) {
+        double res = 0, sum = 0;
+        for (uint32_t j = 0; j < nblocks; j++) {
+            const uint32_t a = bstart[j], b = bstart[j + 1];
+            /* gather + apply of the whole block, then its scatter */
+#pragma omp parallel for schedule(dynamic, 4096)
+            for (uint32_t v = a; v < b; v++) {
+                float s = 0.0f;
+                for (uint32_t k = in_off[v]; k < in_off[v + 1]; k++) s = s + x[in_src[k]];
+                rn[v] = pr_apply_f32(teleport, damping, s);
+            }
+            for (uint32_t v = a; v < b; v++) {
+                res += fabs((double)rn[v] - (double)r[v]);
+                sum += (double)rn[v];
+                r[v] = rn[v];
+                x[v] = outdeg[v] ? rn[v] / (float)outdeg[v] : 0.0f;
+            }
+        }
+        if (stats) {
+            stats[2 * it] = res;
+            stats[2 * it + 1] = sum;
+        }
+    }
+    memcpy(r_out, r, (size_t)n * 4);
+    free(x);
+    free(r);
+    free(rn);
+    return 0;
+}
+
+/* The engine's sub-intervals for a functional kernel: each interval
+ * [intervals[p-1]+1, intervals[p]] (the first from 0) is cut into windows of
+ * at most maxwindow + 1 vertices (src/engine/graphchi_engine.hpp:862-869 with
+ * determine_next_window returning its upper bound for computational edges,
+ * :311-313, and maxwindow = membudget_mb * 2^20 / 3 / 100, :729-732).  Writes
+ * the block starts and n to bstart (capacity cap) and returns the block count,
+ * or -1. */
+int orc_semisync_blocks(uint32_t n, uint32_t nintervals, const uint32_t *interval_ends, int membudget_mb,
+                        uint32_t *bstart, uint32_t cap) {
+    const uint64_t maxwindow = (uint64_t)membudget_mb * 1024 * 1024 / 3 / 100;
+    uint32_t nb = 0;
+    uint64_t st = 0;
+    for (uint32_t p = 0; p < nintervals; p++) {
+        const uint64_t en = interval_ends[p];
+        if (st > en) continue; /* empty interval (:841) */
+        uint64_t sub = st;
+        while (sub <= en) {
+            uint64_t sub_en = sub + maxwindow < en ? sub + maxwindow : en;
+            if (nb >= cap) return -1;
+            bstart[nb++] = (uint32_t)sub;
+            sub = sub_en + 1;
+        }
+        st = en + 1;
+    }
+    if (nb >= cap) return -1;
+    bstart[nb] = n;
+    return (int)nb;
 }
 
 int orc_pagerank_sync_f64(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,

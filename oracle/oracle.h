@@ -80,6 +80,17 @@ int orc_degree_order(uint32_t n, const uint32_t *deg, uint32_t *vertexmap);
 int orc_triangles(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, uint32_t *vertexmap,
                   uint32_t *per_vertex, uint64_t *total);
 
+/* Semi-synchronous PageRank-form sweeps (the reference pagerank_functional's
+ * default mode): blocks [bstart[j], bstart[j+1]) in order, each gathering this
+ * sweep's x of earlier blocks and the previous sweep's x of the rest. */
+int orc_pagerank_semisync_f32(uint32_t n, const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                              uint32_t nblocks, const uint32_t *bstart, double teleport, double damping, float init,
+                              int nsweeps, float *r_out, double *stats);
+/* The engine's sub-interval starts for functional kernels (windows of at most
+ * membudget_mb * 2^20 / 300 + 1 vertices inside each interval). */
+int orc_semisync_blocks(uint32_t n, uint32_t nintervals, const uint32_t *interval_ends, int membudget_mb,
+                        uint32_t *bstart, uint32_t cap);
+
 /* CPU baseline: the reference's PSW sweep with all shards preloaded
  * (functional_engine.hpp:50-121): per interval, gather over the memory shard's
  * in-edges (src-sorted) reading each edge's value, apply, then scatter r/outdeg

@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py; never by the product.
 """
 import ctypes
 import os
+import re
 
 import numpy as np
 
@@ -27,6 +28,8 @@ def lib():
             "orc_csc_build": (i32, [u32, u64, vp, vp, vp, vp, vp, vp]),
             "orc_pagerank_sync_f32": (i32, [u32, vp, vp, vp, dbl, dbl, f32, i32, vp, vp]),
             "orc_pagerank_sync_f64": (i32, [u32, vp, vp, vp, dbl, dbl, dbl, i32, vp]),
+            "orc_pagerank_semisync_f32": (i32, [u32, vp, vp, vp, u32, vp, dbl, dbl, f32, i32, vp, vp]),
+            "orc_semisync_blocks": (i32, [u32, u32, vp, i32, vp, u32]),
             "orc_ppr_f64": (i32, [u32, vp, vp, vp, vp, u32, dbl, dbl, i32, vp]),
             "orc_cc_unionfind": (i32, [u32, u64, vp, vp, vp]),
             "orc_cc_labelprop": (i32, [u32, u64, vp, vp, vp]),
@@ -72,6 +75,26 @@ def pagerank_f32(n, in_off, in_src, outdeg, teleport, damping, init, nsweeps):
                                      _p(r), _p(stats))
     assert rc == 0
     return r, stats[:2 * nsweeps].reshape(-1, 2)
+
+
+def pagerank_semisync_f32(n, in_off, in_src, outdeg, bstart, teleport, damping, init, nsweeps):
+    bstart = u32(bstart)
+    r = np.zeros(n, np.float32)
+    stats = np.zeros(max(2 * nsweeps, 2), np.float64)
+    in_src = in_src if len(in_src) else np.zeros(1, np.uint32)
+    rc = lib().orc_pagerank_semisync_f32(n, _p(in_off), _p(in_src), _p(outdeg), len(bstart) - 1, _p(bstart),
+                                         teleport, damping, init, nsweeps, _p(r), _p(stats))
+    assert rc == 0
+    return r, stats[:2 * nsweeps].reshape(-1, 2)
+
+
+def semisync_blocks(n, interval_ends, membudget_mb=800):
+    ends = u32(interval_ends)
+    cap = len(ends) + n + 2
+    b = np.zeros(cap, np.uint32)
+    nb = lib().orc_semisync_blocks(n, len(ends), _p(ends), membudget_mb, _p(b), cap)
+    assert nb > 0
+    return b[:nb + 1]
 
 
 def pagerank_f64(n, in_off, in_src, outdeg, teleport, damping, init, nsweeps):
@@ -174,3 +197,41 @@ def pr_psw_omp(n, src, dst, nshards, teleport, damping, init, nsweeps, nthreads=
                                 _p(r))
     assert secs >= 0
     return r, secs
+
+
+def _libc():
+    return ctypes.CDLL(None)
+
+
+def parse_edgelist(text):
+    """convert_edgelist of src/preprocessing/conversions.hpp:210-288 restated:
+    fgets lines of at most 1023 bytes, the trailing newline dropped, lines
+    starting with '#' or '%' skipped, strtok on tab / comma / space, from and
+    to by the C library's own atoi, self-loops dropped (sharder.hpp:428).
+    Returns (src, dst) as uint32 arrays, or raises ValueError where the
+    reference asserts (a line with fewer than two tokens)."""
+    atoi = _libc().atoi
+    atoi.argtypes = [ctypes.c_char_p]
+    atoi.restype = ctypes.c_int
+    src, dst = [], []
+    pos = 0
+    data = bytes(text)
+    while pos < len(data):
+        # fgets(s, 1024, f): up to 1023 bytes, through the first newline
+        nl = data.find(b"\n", pos, pos + 1023)
+        end = nl + 1 if nl >= 0 else min(len(data), pos + 1023)
+        line = data[pos:end]
+        pos = end
+        if line.endswith(b"\n"):
+            line = line[:-1]  # FIXLINE
+        if line[:1] in (b"#", b"%"):
+            continue
+        toks = [t for t in re.split(rb"[\t, ]", line) if t]
+        if len(toks) < 2:
+            raise ValueError(f"line {line!r} has fewer than two tokens")
+        a = atoi(toks[0]) & 0xFFFFFFFF
+        b = atoi(toks[1]) & 0xFFFFFFFF
+        if a != b:
+            src.append(a)
+            dst.append(b)
+    return np.array(src, np.uint32), np.array(dst, np.uint32)

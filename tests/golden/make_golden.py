@@ -6,6 +6,7 @@ inputs made by the product's deterministic R-MAT generator (the host variant,
 gchi_rmat_generate_host), and stores what the reference wrote:
 
   pagerank_functional mode sync (normalised kernel)  -> ranks (.4B.vout)
+  pagerank_functional default (semisync) mode        -> ranks (.4B.vout), two membudgets
   oracle/ref_affine_kernel variant 0 / 1             -> 0.15+0.85*s ranks, PPR ranks
   the sharder run inside them (PairContainer<float>)  -> .intervals, .adj, .adjidx,
                                                         _degs.bin, .numvertices, edata sizes
@@ -149,6 +150,31 @@ def pagerank_cases(rr, manifest, arrays, only):
         print("pagerank", name, n, len(src), flush=True)
 
 
+SEMISYNC_CASES = ("pr_rmat14_p3", "pr_rmat16")
+
+
+def semisync_cases(rr, manifest, arrays, only):
+    """pagerank_functional in its default mode (semisync, no `mode` argument),
+    at the conf's membudget_mb (one sub-interval per interval here) and at
+    membudget_mb 1 (windows of 3,496 vertices: several sub-intervals per
+    interval)."""
+    for case in graphs.PR_CASES:
+        name = case["name"]
+        if name not in SEMISYNC_CASES or (only and name != only):
+            continue
+        src, dst, n = graphs.make(case)
+        path = rr.graph_file(name + "_semisync", src, dst)
+        niters = case["sweeps"] + 1
+        nsh = case.get("nshards", 1)
+        rr.run("pagerank_functional", ["file", path, "filetype", "edgelist", "niters", niters, "nshards", nsh])
+        arrays[f"{name}__pr_semisync"] = read_f32(path + ".4B.vout")
+        rr.run("pagerank_functional", ["file", path, "filetype", "edgelist", "niters", niters, "nshards", nsh,
+                                       "membudget_mb", 1])
+        arrays[f"{name}__pr_semisync_mb1"] = read_f32(path + ".4B.vout")
+        manifest[name]["semisync_membudget_mb"] = [800, 1]
+        print("semisync", name, n, len(src), flush=True)
+
+
 def sharder_cases(rr, manifest, arrays, only):
     for case in graphs.SHARD_CASES:
         name = case["name"]
@@ -205,21 +231,28 @@ def tc_cases(rr, manifest, arrays, only):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=None)
+    ap.add_argument("--semisync-only", action="store_true", help="add the semisync goldens to the existing files")
     args = ap.parse_args()
     if not os.path.exists(os.path.join(REF, "pagerank_functional")):
         sys.exit("reference binaries missing: run `make -C oracle ref` first")
     mpath = os.path.join(OUT, "manifest.json")
     apath = os.path.join(OUT, "golden.npz")
     manifest, arrays = {}, {}
-    if args.only and os.path.exists(mpath):
+    if (args.only or args.semisync_only) and os.path.exists(mpath):
         manifest = json.load(open(mpath))
         arrays = dict(np.load(apath))
     rr = RefRunner()
     try:
+        if args.semisync_only:
+            semisync_cases(rr, manifest, arrays, args.only)
+            raise StopIteration
         pagerank_cases(rr, manifest, arrays, args.only)
+        semisync_cases(rr, manifest, arrays, args.only)
         sharder_cases(rr, manifest, arrays, args.only)
         cc_cases(rr, manifest, arrays, args.only)
         tc_cases(rr, manifest, arrays, args.only)
+    except StopIteration:
+        pass
     finally:
         rr.cleanup()
     with open(mpath, "w") as f:

@@ -3,9 +3,9 @@ through the C++ host layer (graphchi-cpp_amd/host) and the HIP library.
 
 * the reference's own pagerank_functional.cpp and connectedcomponents.cpp,
   compiled unchanged against the host headers (apps/_dropin, built where the
-  reference tree is present);
-* this repository's apps (apps/bin): pagerank, connectedcomponents,
-  trianglecounting, sharder.
+  reference tree is present; both modes of pagerank_functional);
+* the reference's trianglecounting.cpp, likewise;
+* this repository's apps (apps/bin): pagerank, connectedcomponents, sharder.
 Outputs (.vout files, shard files) are compared with what the reference's
 binaries wrote for the same input (golden fixtures)."""
 import hashlib
@@ -76,12 +76,53 @@ def test_reference_pagerank_functional_source_on_device(workdir, golden, manifes
     shard_shas(path, 3, manifest["pr_rmat14_p3"])
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(DROPIN, "pagerank_functional")), reason="drop-in not built")
+@pytest.mark.parametrize("name", ["pr_rmat14_p3", "pr_rmat16"])
+def test_reference_pagerank_functional_default_mode(workdir, golden, name):
+    """No `mode` argument: the reference's default, semi-synchronous mode, at
+    the default memory budget and at membudget_mb 1 (several sub-intervals per
+    interval), against what the reference binary wrote in the same runs."""
+    path, n = write_edgelist(workdir, name)
+    c = case(name)
+    args = ["file", path, "filetype", "edgelist", "niters", c["sweeps"] + 1, "nshards", c.get("nshards", 1)]
+    run(os.path.join(DROPIN, "pagerank_functional"), args, workdir)
+    assert_close(np.fromfile(path + ".4B.vout", np.float32), golden[f"{name}__pr_semisync"])
+    run(os.path.join(DROPIN, "pagerank_functional"), args + ["membudget_mb", 1], workdir)
+    assert_close(np.fromfile(path + ".4B.vout", np.float32), golden[f"{name}__pr_semisync_mb1"])
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(DROPIN, "connectedcomponents")), reason="drop-in not built")
 def test_reference_connectedcomponents_source_on_device(workdir, golden):
     path, n = write_edgelist(workdir, "cc_rmat14_p3")
-    run(os.path.join(DROPIN, "connectedcomponents"), ["file", path, "filetype", "edgelist", "nshards", 3], workdir)
+    out = run(os.path.join(DROPIN, "connectedcomponents"), ["file", path, "filetype", "edgelist", "nshards", 3],
+              workdir)
     np.testing.assert_array_equal(np.fromfile(path + ".4B.vout", np.uint32), golden["cc_rmat14_p3__labels"])
     assert os.path.exists(path + ".components")
+    # the program's before/after_iteration ran every round: its convergence
+    # message, and one progress line per iteration
+    assert "Converged!" in out
+    with open(path + ".deltalog") as f:
+        lines = [l.split(",") for l in f.read().split()]
+    assert len(lines) >= 2 and [int(l[0]) for l in lines] == list(range(len(lines)))
+    # niters bounds the iterations
+    run(os.path.join(DROPIN, "connectedcomponents"), ["file", path, "filetype", "edgelist", "nshards", 3, "niters", 1],
+        workdir)
+    with open(path + ".deltalog") as f:
+        assert len(f.read().split()) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(DROPIN, "trianglecounting")), reason="drop-in not built")
+@pytest.mark.parametrize("name", ["tc_k5", "tc_rmat10", "tc_rmat13"])
+def test_reference_trianglecounting_source_on_device(workdir, golden, manifest, name):
+    """example_apps/trianglecounting.cpp compiled unchanged: its degree
+    ordering (.vertexmap), per-vertex counts (<G>_degord.4B.vout) and printed
+    total equal what the reference binary wrote for the same edge list."""
+    path, n = write_edgelist(workdir, name)
+    out = run(os.path.join(DROPIN, "trianglecounting"), ["file", path, "filetype", "edgelist", "nshards", 2], workdir)
+    g = manifest[name]
+    assert f"Number of triangles: {g['total']}({g['sum3']})" in out
+    np.testing.assert_array_equal(np.fromfile(path + ".vertexmap", np.uint32), golden[f"{name}__vertexmap"])
+    np.testing.assert_array_equal(np.fromfile(path + "_degord.4B.vout", np.uint32), golden[f"{name}__tc_pervertex"])
 
 
 def test_app_pagerank_both_forms(workdir, golden):
@@ -98,16 +139,6 @@ def test_app_connectedcomponents(workdir, golden):
     path, n = write_edgelist(workdir, "cc_rmat12")
     run(os.path.join(BIN, "connectedcomponents"), ["file", path], workdir)
     np.testing.assert_array_equal(np.fromfile(path + ".4B.vout", np.uint32), golden["cc_rmat12__labels"])
-
-
-@pytest.mark.parametrize("name", ["tc_rmat10", "tc_rmat13"])
-def test_app_trianglecounting(workdir, golden, manifest, name):
-    path, n = write_edgelist(workdir, name)
-    out = run(os.path.join(BIN, "trianglecounting"), ["file", path, "nshards", 2], workdir)
-    g = manifest[name]
-    assert f"Number of triangles: {g['total']}({g['sum3']})" in out
-    np.testing.assert_array_equal(np.fromfile(path + ".vertexmap", np.uint32), golden[f"{name}__vertexmap"])
-    np.testing.assert_array_equal(np.fromfile(path + "_degord.4B.vout", np.uint32), golden[f"{name}__tc_pervertex"])
 
 
 @pytest.mark.parametrize("name", ["sh_rmat12_p7", "sh_hand_gaps"])

@@ -116,3 +116,66 @@ def test_cc_scheduled_rejects_bad_input(dev):
     g = _sym_csr(dev, path[:-1], path[1:], 101)
     with pytest.raises(dev.GchiError):
         dev.cc_scheduled(g.in_off, g.in_src, 101, max_rounds=1)
+
+
+@pytest.mark.parametrize("scale,ef,seed", [(12, 16, 3), (16, 8, 5), (24, 8, 3)])
+def test_cc_afforest_vs_unionfind(dev, scale, ef, seed):
+    """Sampled union-find over the symmetric CSR (the giant component's edges
+    skipped) gives the same labels as the oracle's union-find, up to the bench
+    size (R-MAT 24, both directions, 268M entries: checked against the full
+    device union-find, itself pinned to the oracle at smaller sizes)."""
+    s, d, cnt, mx = dev.rmat_device(scale, ef << scale, seed)
+    import torch
+    s2, d2 = torch.cat([s, d]), torch.cat([d, s])
+    n = mx + 1
+    g = dev.DeviceGraph(s2, d2, n)
+    lab = dev.to_host_u32(dev.cc_afforest(g.in_off, g.in_src, n), n)
+    if scale <= 16:
+        want = oracle.cc_unionfind(n, dev.to_host_u32(s, cnt), dev.to_host_u32(d, cnt))
+    else:
+        want = dev.to_host_u32(dev.cc_minlabel(s2, d2, n, symmetric=True), n)
+    np.testing.assert_array_equal(lab, want)
+
+
+def test_cc_afforest_no_giant_component(dev):
+    """Many small components and isolated vertices (no component holds most
+    samples): every vertex's remaining neighbours are linked."""
+    rng = np.random.default_rng(3)
+    n = 50000
+    a = rng.integers(0, n, 30000).astype(np.uint32)
+    b = ((a // 100) * 100 + rng.integers(0, 100, len(a))).astype(np.uint32)  # edges inside blocks of 100 ids
+    b = np.minimum(b, n - 1)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    src, dst = np.concatenate([a, b]), np.concatenate([b, a])
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)
+    lab = dev.to_host_u32(dev.cc_afforest(g.in_off, g.in_src, n), n)
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(n, a, b))
+
+
+@pytest.mark.parametrize("case", graphs.CC_CASES, ids=[c["name"] for c in graphs.CC_CASES])
+def test_cc_pull_sweeps_vs_reference_golden(dev, golden, case):
+    """Synchronous sweeps pulled over the symmetric CSR: the reference's labels,
+    in as many sweeps as the oracle's synchronous label propagation."""
+    import torch
+    src, dst, n = graphs.make(case)
+    a = np.concatenate([src, dst]).astype(np.uint32)
+    b = np.concatenate([dst, src]).astype(np.uint32)
+    g = dev.DeviceGraph(dev.to_dev_u32(a), dev.to_dev_u32(b), n)
+    lab, sweeps = dev.cc_pull(g.in_off, g.in_src, n)
+    np.testing.assert_array_equal(dev.to_host_u32(lab, n), golden[f"{case['name']}__labels"])
+    _, osweeps = oracle.cc_labelprop(n, src, dst)
+    assert sweeps == osweeps
+
+
+def test_cc_pull_scale24_equals_unionfind(dev):
+    s, d, cnt, mx = dev.rmat_device(24, 8 << 24, 3)
+    import torch
+    s2, d2 = torch.cat([s, d]), torch.cat([d, s])
+    n = mx + 1
+    want = dev.to_host_u32(dev.cc_minlabel(s2, d2, n, symmetric=True), n)
+    g = dev.DeviceGraph(s2, d2, n)
+    del s2, d2
+    lab, sweeps = dev.cc_pull(g.in_off, g.in_src, n)
+    np.testing.assert_array_equal(dev.to_host_u32(lab, n), want)
+    print(f"\npull sweeps at scale 24: {sweeps}")

@@ -93,3 +93,34 @@ def test_csc_build_bit_exact(dev, case):
     np.testing.assert_array_equal(dev.to_host_u32(g.in_src, len(src)), in_src)
     np.testing.assert_array_equal(dev.to_host_u32(g.outdeg, n), outdeg)
     np.testing.assert_array_equal(dev.to_host_u32(g.indeg, n), indeg)
+
+
+def test_sort_lookback_timeout_is_reported(dev):
+    """A look-back wait that reaches its bound must not pass silently: the
+    test hook makes every wait see unpublished predecessors and gives it a
+    bound of 8 polls, so each tile gives up (every kernel still exits, and
+    every write stays in bounds), and the status word reports the failure to
+    the asynchronous sort, the pair sort, the CSC build and the sharder."""
+    import torch
+    s, d, cnt, mx = dev.rmat_device(16, 1 << 20, 4)
+    keys = torch.arange(1 << 20, 0, -1, dtype=torch.int64, device="cuda")
+    try:
+        dev.debug_sort_lookback(8, True)
+        ws = dev.sort_u64(keys, key_bits=24, check_status=False)
+        with pytest.raises(dev.GchiError, match="TIMEOUT|timed out|wait"):
+            dev.work_status(ws)
+        with pytest.raises(dev.GchiError):
+            dev.sort_pairs_dst_src(s, d, 16)
+        with pytest.raises(dev.GchiError):
+            dev.DeviceGraph(s, d, mx + 1)
+        with pytest.raises(dev.GchiError):
+            dev.Shards(s, d, mx, 2)
+    finally:
+        dev.debug_sort_lookback(0, False)
+    # the same calls succeed again once the hook is off
+    dev.sort_u64(keys, key_bits=24)
+    assert bool((keys[1:] >= keys[:-1]).all())
+    so, do = dev.sort_pairs_dst_src(s, d, 16)
+    hs, hd = dev.to_host_u32(s, cnt), dev.to_host_u32(d, cnt)
+    o = np.lexsort((hs, hd))
+    np.testing.assert_array_equal(dev.to_host_u32(so, cnt), hs[o])

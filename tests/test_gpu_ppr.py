@@ -47,6 +47,9 @@ def test_batched_ppr_vs_fp64(dev, mode, tune):
     assert info["converged"] == 1
     if mode == "bf16_fp32":
         assert info["sweeps_bf16"] > 0
+    # chunks of 4 sweeps; converged query blocks stop being swept
+    assert info["sweeps_fp32"] % 4 == 0 and info["sweeps_bf16"] % 4 == 0
+    assert 0 < info["block_sweeps_fp32"] <= 2 * info["sweeps_fp32"]
     in_off, in_src, outdeg, _ = oracle.csc(n, src, dst)
     worst = 0.0
     for q in (0, 1, 63, 64, 69):
@@ -82,3 +85,26 @@ def test_ppr_with_gather_layout_plan(dev):
     np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     with pytest.raises(dev.GchiError):
         dev.PprBatch(hot, 64)
+
+
+def test_batched_ppr_retires_converged_blocks(dev):
+    """Query blocks converge at different sweeps: one block of 1-seed queries on
+    high-out-degree vertices, one of 64-seed queries.  Every query still meets
+    the tolerance against the FP64 oracle, and the FP32 work counted per block
+    is below (blocks x sweeps) when one block retires first."""
+    case = {"name": "ppr14r", "rmat": (14, 16, 7)}
+    src, dst, n, g, plan = graph_and_plan(dev, case)
+    in_off, in_src, outdeg, _ = oracle.csc(n, src, dst)
+    hubs = np.argsort(-np.diff(in_off.astype(np.int64)))[:64].astype(np.uint32)
+    rng = np.random.default_rng(2)
+    queries = [np.array([h], np.uint32) for h in hubs] + [rng.choice(n, 64, replace=False).astype(np.uint32)
+                                                          for _ in range(64)]
+    b = dev.PprBatch(plan, len(queries))
+    info, resid = b.run(queries, D, TOL, bf16_sweeps=0, check_every=2)
+    assert info["converged"] == 1
+    assert info["block_sweeps_fp32"] <= 2 * info["sweeps_fp32"]
+    for q in (0, 63, 64, 127):
+        r = b.ranks(q).cpu().numpy().astype(np.float64)
+        r64, _ = oracle.ppr_f64(n, in_off, in_src, outdeg, queries[q], D)
+        assert np.abs(r - r64).sum() <= TOL, q
+    assert (resid * D / (1 - D) <= TOL).all()

@@ -119,3 +119,43 @@ def test_reference_app_consumes_device_shards(dev, golden):
         np.testing.assert_array_equal(r, golden["pr_rmat14_p3__pr_norm"])
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+REF_CC = os.path.join(os.path.dirname(REF_APP), "connectedcomponents")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CC), reason="reference binary not built (needs the reference tree)")
+def test_reference_cc_consumes_device_shards_with_zlib_edata(dev, golden):
+    """connectedcomponents reads zlib-compressed edge-data blocks (it does not
+    define GRAPHCHI_DISABLE_COMPRESSION): shards written by the device sharder
+    with GCHI_SHARD_ZLIB are loaded by the reference's own binary without
+    re-sharding, and its labels equal the golden labels.  Every block inflates
+    to the uncompressed block (default edge values, the .size total)."""
+    import zlib
+    case = next(c for c in graphs.CC_CASES if c["name"] == "cc_rmat14_p3")
+    src, dst, n = graphs.make(case)
+    tmp = tempfile.mkdtemp(prefix="gchi_refcc_")
+    try:
+        os.makedirs(os.path.join(tmp, "conf"))
+        with open(os.path.join(tmp, "conf", "graphchi.cnf"), "w") as f:
+            f.write("loadthreads = 4\nexecthreads = 4\nniothreads = 2\nmembudget_mb = 800\ncachesize_mb = 0\n"
+                    "io.blocksize = 1048576\nmmap = 0\nmetrics.reporter = console\n")
+        base = os.path.join(tmp, "g")
+        sh = dev.Shards(dev.to_dev_u32(src), dev.to_dev_u32(dst), n - 1, 3)
+        sh.write_files(base, 4, zlib=True)
+        for p in range(sh.nshards):
+            ed = f"{base}.edata..Z.e4B.{p}_3"
+            total = int(open(ed + ".size").read())
+            d = f"{ed}_blockdir_1048576"
+            got = b"".join(zlib.decompress(open(os.path.join(d, str(b)), "rb").read())
+                           for b in range(len(os.listdir(d))))
+            assert got == bytes(total)
+        p = subprocess.run([REF_CC, "file", base, "filetype", "edgelist", "nshards", "3"], cwd=tmp,
+                           env=dict(os.environ, GRAPHCHI_ROOT=tmp, OMP_NUM_THREADS="4"), stdin=subprocess.DEVNULL,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+        out = p.stdout.decode(errors="replace")
+        assert p.returncode == 0, out[-2000:]
+        assert "Found preprocessed files" in out
+        np.testing.assert_array_equal(np.fromfile(base + ".4B.vout", dtype=np.uint32), golden["cc_rmat14_p3__labels"])
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)

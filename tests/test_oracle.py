@@ -36,6 +36,47 @@ def test_pagerank_f32_oracle_bit_exact_vs_reference(case, golden):
         np.testing.assert_array_equal(r, ref, err_msg=kind)
 
 
+SEMISYNC = [c for c in graphs.PR_CASES if c["name"] in ("pr_rmat14_p3", "pr_rmat16")]
+
+
+@pytest.mark.parametrize("case", SEMISYNC, ids=ids(SEMISYNC))
+@pytest.mark.parametrize("membudget_mb,key", [(800, "pr_semisync"), (1, "pr_semisync_mb1")])
+def test_pagerank_semisync_oracle_bit_exact_vs_reference(case, membudget_mb, key, golden, manifest):
+    """The reference pagerank_functional's default mode, with its sub-interval
+    cut at two memory budgets (3 and 7 blocks on the 3-shard graph, 1 and 19 on
+    the 1-shard one): bit-exact, so the block order, the per-block gather-then-
+    scatter and the window rule are all pinned; a wrong cut changes the bits."""
+    src, dst, n = graphs.make(case)
+    in_off, in_src, outdeg, _ = oracle.csc(n, src, dst)
+    b = oracle.semisync_blocks(n, manifest[case["name"]]["shards"]["intervals"], membudget_mb)
+    r, _ = oracle.pagerank_semisync_f32(n, in_off, in_src, outdeg, b, 0.15 / n, DAMPING, 1.0, case["sweeps"])
+    np.testing.assert_array_equal(r, golden[f"{case['name']}__{key}"])
+    # a cut shifted by one vertex changes the result on the multi-block runs
+    if len(b) > 2:
+        b2 = b.copy()
+        b2[1] += 1
+        r2, _ = oracle.pagerank_semisync_f32(n, in_off, in_src, outdeg, b2, 0.15 / n, DAMPING, 1.0, case["sweeps"])
+        assert not np.array_equal(r2, r)
+
+
+def test_semisync_one_block_is_sync(golden):
+    case = next(c for c in graphs.PR_CASES if c["name"] == "pr_rmat14_p3")
+    src, dst, n = graphs.make(case)
+    in_off, in_src, outdeg, _ = oracle.csc(n, src, dst)
+    r1, s1 = oracle.pagerank_semisync_f32(n, in_off, in_src, outdeg, [0, n], 0.15, DAMPING, 1.0, 5)
+    r2, s2 = oracle.pagerank_f32(n, in_off, in_src, outdeg, 0.15, DAMPING, 1.0, 5)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_allclose(s1, s2, rtol=1e-12)
+
+
+def test_semisync_window_rule():
+    # maxwindow = membudget_mb * 2^20 / 300: 3495 at 1 MB; a window holds up to
+    # maxwindow + 1 vertices; empty intervals are skipped
+    np.testing.assert_array_equal(oracle.semisync_blocks(10000, [9999], 1), [0, 3496, 6992, 10000])
+    np.testing.assert_array_equal(oracle.semisync_blocks(10, [3, 3, 9], 800), [0, 4, 10])
+    np.testing.assert_array_equal(oracle.semisync_blocks(5000, [100, 4999], 1), [0, 101, 3597, 5000])
+
+
 def test_pagerank_f64_tracks_f32(golden):
     case = next(c for c in graphs.PR_CASES if c["name"] == "pr_rmat16")
     src, dst, n = graphs.make(case)
@@ -260,3 +301,25 @@ def test_psw_cpu_baseline_matches_oracle(golden):
     ref = golden["pr_rmat14_p3__pr_affine"]
     rel = np.abs(r.astype(np.float64) - ref) / np.abs(ref)
     assert rel.max() < 2e-5
+
+
+def test_parse_edgelist_restatement():
+    """The oracle's edge-list parse (the reference's convert_edgelist) on the
+    cases its loop distinguishes: delimiters in runs, comments, CRLF, a value
+    column, signs and trailing bytes (the C library's atoi), self-loops, no
+    final newline; lines without two fields raise (the reference asserts)."""
+    text = (b"# comment\n% another\n0\t1\n2,3\n  4   5  \n6\t7\t0.5\n8 8\n9\t10\r\n"
+            b"+11,-1\n12abc 13x\n14 15")
+    s, d = oracle.parse_edgelist(text)
+    assert s.tolist() == [0, 2, 4, 6, 9, 11, 12, 14]
+    assert d.tolist() == [1, 3, 5, 7, 10, 0xFFFFFFFF, 13, 15]
+    for bad in (b"1 2\n\n3 4\n", b"7\n", b" , \n"):
+        with pytest.raises(ValueError):
+            oracle.parse_edgelist(bad)
+    # the golden inputs are written as "<src>\t<dst>\n": the parse is the identity
+    case = graphs.PR_CASES[0]
+    src, dst, n = graphs.make(case)
+    txt = "".join(f"{a}\t{b}\n" for a, b in zip(src.tolist(), dst.tolist())).encode()
+    s, d = oracle.parse_edgelist(txt)
+    np.testing.assert_array_equal(s, src)
+    np.testing.assert_array_equal(d, dst)
