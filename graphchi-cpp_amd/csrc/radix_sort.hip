@@ -247,9 +247,9 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
                 p -= used;
                 if (used == 0) {
                     if (++spins > spin_limit) {
-                        // the prefix so far is at most the true one, so every
-                        // write below stays inside [0, n); the output is wrong
-                        // and the caller's status word says so
+                        // the output is wrong from here on and the caller's
+                        // status word says so; the scatter below drops any
+                        // position outside [0, n)
                         atomicOr(err, GCHI_WORK_TIMEOUT);
                         break;
                     }
@@ -281,6 +281,11 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
             uint64_t key = s_keys[p];
             uint32_t d = (uint32_t)(key >> shift) & dmask;
             uint32_t g = s_gb[d] + p;
+            // In a correct sort g < n always holds.  After a look-back wait has
+            // given up, the passes that follow read a buffer with unwritten
+            // holes whose digits the histogram (taken from the input) never
+            // counted, so their positions can run past n: those are dropped.
+            if (g >= n) continue;
             if (IO & kIoUnpackOut) {
                 io.src_out[g] = (uint32_t)(key & ((1ull << io.id_bits) - 1ull));
                 io.dst_out[g] = (uint32_t)(key >> io.id_bits);

@@ -49,10 +49,16 @@ __global__ void tc_degree_keys(const uint32_t *deg, uint32_t n, int idb, uint64_
         keys[v] = ((uint64_t)deg[v] << idb) | v;
 }
 
+// Ids >= n appear only in the output of a sort whose look-back gave up (the
+// caller reports that and discards the result): skipped so that no write leaves
+// the arrays.  The same guard is in the two run kernels below.
 __global__ void tc_vertexmap(const uint64_t *keys, uint32_t n, int idb, uint32_t *vmap) {
     uint32_t stride = gridDim.x * blockDim.x;
     const uint64_t m = (1ull << idb) - 1ull;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) vmap[keys[i] & m] = i;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t v = keys[i] & m;
+        if (v < n) vmap[v] = i;
+    }
 }
 
 __global__ void tc_orient_keys(const uint32_t *src, const uint32_t *dst, uint64_t e, uint32_t n, const uint32_t *vmap, int idb,
@@ -81,12 +87,13 @@ __global__ void tc_unique_flags(const uint64_t *keys, uint64_t e, uint64_t senti
 // Runs of equal lo = key >> idb in the sorted oriented keys (sentinel keys,
 // which sort last, excluded).  lo_start[lo] = pos at the run's first key.
 __global__ void tc_lo_run_starts(const uint64_t *keys, const uint32_t *pos, uint64_t e, int idb, uint64_t sentinel,
-                                 uint32_t *lo_start) {
+                                 uint32_t n, uint32_t *lo_start) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += stride) {
         const uint64_t k = keys[i];
         if (k == sentinel) continue;
         const uint64_t lo = k >> idb;
+        if (lo >= n) continue;
         if (i == 0 || (keys[i - 1] >> idb) != lo) lo_start[lo] = pos[i];
     }
 }
@@ -94,7 +101,7 @@ __global__ void tc_lo_run_starts(const uint64_t *keys, const uint32_t *pos, uint
 // adj[pos] = hi of every unique key; at a run's last key, lo_count[lo] = the
 // unique keys of the run (no atomics: one writer per run).
 __global__ void tc_scatter_unique(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos, uint64_t e,
-                                  int idb, uint64_t sentinel, const uint32_t *lo_start, uint32_t *adj,
+                                  int idb, uint64_t sentinel, uint32_t n, const uint32_t *lo_start, uint32_t *adj,
                                   uint32_t *lo_count) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t m = (1ull << idb) - 1ull;
@@ -102,6 +109,7 @@ __global__ void tc_scatter_unique(const uint64_t *keys, const uint32_t *flags, c
         const uint64_t k = keys[i];
         if (k == sentinel) continue;
         const uint64_t lo = k >> idb;
+        if (lo >= n) continue;
         const uint32_t f = flags[i], p = pos[i];
         if (f) adj[p] = (uint32_t)(k & m);
         const bool last = i + 1 == e || keys[i + 1] == sentinel || (keys[i + 1] >> idb) != lo;
@@ -486,9 +494,9 @@ extern "C" int gchi_tc_prepare(const uint32_t *d_src, const uint32_t *d_dst, uin
         GCHI_TRY(exclusive_scan_u32(flags, pos, E, sw, swb, s));
         uint32_t *lo_start = reinterpret_cast<uint32_t *>(vkeys);  // vertexmap is built: vkeys is free
         hipLaunchKernelGGL(tc_lo_run_starts, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, pos, E, idb,
-                           sentinel, lo_start);
+                           sentinel, N, lo_start);
         hipLaunchKernelGGL(tc_scatter_unique, dim3(grid_for(E, 256, 65536)), dim3(256), 0, s, ekeys, flags, pos, E,
-                           idb, sentinel, lo_start, d_adj, lo_count);
+                           idb, sentinel, N, lo_start, d_adj, lo_count);
         GCHI_LAUNCH_CHECK();
     }
     GCHI_TRY(exclusive_scan_u32(lo_count, d_off, N, sw, swb, s));

@@ -98,9 +98,14 @@ def test_csc_build_bit_exact(dev, case):
 def test_sort_lookback_timeout_is_reported(dev):
     """A look-back wait that reaches its bound must not pass silently: the
     test hook makes every wait see unpublished predecessors and gives it a
-    bound of 8 polls, so each tile gives up (every kernel still exits, and
-    every write stays in bounds), and the status word reports the failure to
-    the asynchronous sort, the pair sort, the CSC build and the sharder."""
+    bound of 8 polls, so each tile gives up (every kernel still exits), and the
+    status word reports the failure to the asynchronous sort, the pair sort,
+    the CSC build, the sharder and the triangle-counting preparation.  The
+    passes after a timed-out one read buffers with unwritten holes (digits the
+    histogram never counted): the sort drops every position outside [0, n),
+    and the kernels that index by sorted ids skip ids outside the graph, so
+    nothing is written out of bounds (checked under a host address sanitizer
+    on a CPU build of the kernels)."""
     import torch
     s, d, cnt, mx = dev.rmat_device(16, 1 << 20, 4)
     keys = torch.arange(1 << 20, 0, -1, dtype=torch.int64, device="cuda")
@@ -115,9 +120,13 @@ def test_sort_lookback_timeout_is_reported(dev):
             dev.DeviceGraph(s, d, mx + 1)
         with pytest.raises(dev.GchiError):
             dev.Shards(s, d, mx, 2)
+        with pytest.raises(dev.GchiError, match="TIMEOUT|timed out|wait"):
+            dev.TrianglePrep(s, d, mx + 1)
     finally:
         dev.debug_sort_lookback(0, False)
-    # the same calls succeed again once the hook is off
+    # the same calls succeed again once the hook is off (fresh keys: the
+    # stalled sort left its output invalid)
+    keys = torch.arange(1 << 20, 0, -1, dtype=torch.int64, device="cuda")
     dev.sort_u64(keys, key_bits=24)
     assert bool((keys[1:] >= keys[:-1]).all())
     so, do = dev.sort_pairs_dst_src(s, d, 16)
