@@ -29,6 +29,7 @@ int seg_init(const gchi_seg_plan *p, float init, const uint32_t *seed_bits, floa
 
 // Takes the device address of tc.hip's allocation-free scratch on device dev
 // (once per device; gchi_device_open calls it).
+int tc_scratch_setup(int dev);
 // Waits for s, then maps a status word to GCHI_ERR_TIMEOUT / GCHI_ERR_INVALID.
 int read_work_status(const uint32_t *err, hipStream_t s);
 

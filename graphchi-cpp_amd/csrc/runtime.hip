@@ -2,6 +2,7 @@
 // selection, fill kernels and the device-wide exclusive scan used by the graph
 // builders (in-degree histogram -> CSC offsets, compaction offsets).
 #include "common.hpp"
+#include "internal.hpp"
 
 #include <cstdio>
 #include <cstring>
@@ -199,7 +200,7 @@ extern "C" int gchi_device_open(int device) {
     GCHI_CHECK(device >= 0 && device < count, GCHI_ERR_INVALID, "device index out of range");
     GCHI_HIP(hipSetDevice(device));
     GCHI_HIP(hipFree(nullptr));
-    return GCHI_OK;
+    return tc_scratch_setup(device);
 }
 
 extern "C" int gchi_device_info(int *num_cus, char *arch, size_t arch_len) {

@@ -19,6 +19,7 @@
 //       binary-searches its w in N+(a) in global memory.  Counts: a, b and w
 //       each get +1 per triangle, gathered per wavefront and flushed with
 //       uint64 atomics, converted to uint32 with an overflow flag.
+#include <mutex>
 
 #include "common.hpp"
 #include "internal.hpp"
@@ -546,6 +547,31 @@ static void tc_count_layout(C &c, uint32_t nvertices, unsigned long long **cnt, 
     *wk = c.template take<uint32_t>(kTcStripes * kTcStripeGap);
 }
 
+// ---- allocation-free counting: scratch in a file-scope device array ----
+// Counters for up to GCHI_TC_NOALLOC_MAX_VERTICES vertices, laid out as
+// tc_count_layout carves them (256-byte aligned pieces).
+constexpr size_t kTcScratchCounts = (((size_t)GCHI_TC_NOALLOC_MAX_VERTICES + 1) * 8 + 255) / 256 * 256;
+constexpr size_t kTcScratchBytes = kTcScratchCounts + 256 + (size_t)kTcStripes * kTcStripeGap * 4;
+static __device__ __align__(256) unsigned char scratch[kTcScratchBytes];
+
+// Device address of each device's copy of scratch (the array's name in host
+// code is not a device address), taken once per device with that device
+// current.
+constexpr int kTcMaxDevices = 64;
+static void *g_scratch_ptr[kTcMaxDevices];
+static std::mutex g_scratch_mu;
+
+int tc_scratch_setup(int dev) {
+    GCHI_CHECK(dev >= 0 && dev < kTcMaxDevices, GCHI_ERR_UNSUPPORTED, "tc scratch: device index >= 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return GCHI_OK;
+    void *ptr = nullptr;
+    GCHI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    GCHI_CHECK(ptr, GCHI_ERR_HIP, "tc scratch: hipGetSymbolAddress returned null");
+    g_scratch_ptr[dev] = ptr;
+    return GCHI_OK;
+}
+
 }  // namespace gchi
 
 using namespace gchi;
@@ -572,6 +598,25 @@ extern "C" int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint3
     unsigned long long *cnt, *tot;
     uint32_t *wk;
     tc_count_layout(c, nvertices, &cnt, &tot, &wk);
+    return tc_count_impl(d_off, d_adj, nvertices, d_per_vertex, d_total, d_overflow, cnt, tot, wk,
+                         as_stream(stream));
+}
+
+extern "C" int gchi_tc_count_noalloc(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices,
+                                     uint32_t *d_per_vertex, uint64_t *d_total, uint32_t *d_overflow, void *stream) {
+    GCHI_CHECK(d_off && d_adj && d_per_vertex && d_total && d_overflow && nvertices > 0, GCHI_ERR_INVALID,
+               "tc_count_noalloc: null argument");
+    GCHI_CHECK(nvertices <= GCHI_TC_NOALLOC_MAX_VERTICES, GCHI_ERR_INVALID,
+               "tc_count_noalloc: nvertices exceeds GCHI_TC_NOALLOC_MAX_VERTICES (4,194,304); use gchi_tc_count "
+               "with a workspace");
+    int dev = 0;
+    GCHI_HIP(hipGetDevice(&dev));
+    GCHI_TRY(tc_scratch_setup(dev));
+    Carver c(g_scratch_ptr[dev], kTcScratchBytes);
+    unsigned long long *cnt, *tot;
+    uint32_t *wk;
+    tc_count_layout(c, nvertices, &cnt, &tot, &wk);
+    GCHI_CHECK(c.ok(), GCHI_ERR_INVALID, "tc_count_noalloc: scratch layout exceeds its array");
     return tc_count_impl(d_off, d_adj, nvertices, d_per_vertex, d_total, d_overflow, cnt, tot, wk,
                          as_stream(stream));
 }

@@ -131,6 +131,7 @@ _SIGS = {
                                 c_vp]),
     "gchi_tc_count_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
     "gchi_tc_count": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_tc_count_noalloc": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_shard_auto_count": (c_i32, [c_u64, c_u32, c_i32, ctypes.POINTER(c_i32)]),
     "gchi_shard_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_i32, ctypes.POINTER(c_vp), c_vp]),
     "gchi_shard_destroy": (c_i32, [c_vp]),

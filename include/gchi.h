@@ -454,6 +454,19 @@ int gchi_tc_count_workspace(uint32_t nvertices, size_t *bytes);
 int gchi_tc_count(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_per_vertex,
                   uint64_t *d_total, uint32_t *d_overflow, void *d_work, size_t work_bytes, void *stream);
 
+/* gchi_tc_count without a workspace and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its counters live in a library-owned device array of about
+ * 32 MiB per GPU (held until the process ends), so nvertices may not exceed
+ * GCHI_TC_NOALLOC_MAX_VERTICES (GCHI_ERR_INVALID otherwise).  Every call on one
+ * GPU shares that array: calls that overlap in time on one GPU (other streams,
+ * other threads) race and must be serialised by the caller.  Results equal
+ * gchi_tc_count's. */
+#define GCHI_TC_NOALLOC_MAX_VERTICES (1u << 22)
+int gchi_tc_count_noalloc(const uint32_t *d_off, const uint32_t *d_adj, uint32_t nvertices, uint32_t *d_per_vertex,
+                          uint64_t *d_total, uint32_t *d_overflow, void *stream);
+
 /* ------------------------------------------------------------------- sharder */
 /* determine_number_of_shards "auto" (src/preprocessing/sharder.hpp:555-587). */
 int gchi_shard_auto_count(uint64_t nedges, uint32_t edge_value_bytes, int membudget_mb, int *nshards);
