@@ -218,11 +218,14 @@ __global__ void cc_count_changed(const uint32_t *lin, const uint32_t *lout, uint
 }
 
 // One synchronous sweep pulled over a symmetric CSR: l_out[v] = min(l_in[v],
-// l_in[u] for u in adj(v)), one write per vertex.  Rows up to kPullShort
-// entries: one lane each, reading its row in order; longer rows: the whole
-// wavefront, lane-strided, then a min across lanes.  *changed += labels that
-// dropped (one atomic per wavefront).
+// l_in[u] for u in adj(v)).  Rows up to kPullShort entries: one lane each,
+// reading its row in order; rows up to kPullHuge: the whole wavefront,
+// lane-strided, then a min across lanes; longer rows (R-MAT's hubs hold
+// millions of entries, which one wavefront would walk for milliseconds): l_out
+// gets l_in here, and cc_pull_huge splits their entries over many wavefronts.
+// *changed += labels that dropped (one atomic per wavefront).
 constexpr uint32_t kPullShort = 32;
+constexpr uint32_t kPullHuge = 4096;  // also the entries per wavefront of cc_pull_huge
 
 __global__ void cc_pull_sweep(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n,
                               const uint32_t *__restrict__ lin, uint32_t *__restrict__ lout,
@@ -237,7 +240,9 @@ __global__ void cc_pull_sweep(const uint32_t *__restrict__ off, const uint32_t *
         if (v < n) {
             const uint32_t b = off[v], e = off[v + 1];
             uint32_t m = lin[v];
-            if (e - b > kPullShort) {
+            if (e - b > kPullHuge) {
+                lout[v] = m;  // cc_pull_huge lowers it
+            } else if (e - b > kPullShort) {
                 is_long = true;
             } else {
                 for (uint32_t k = b; k < e; k++) {
@@ -265,6 +270,67 @@ __global__ void cc_pull_sweep(const uint32_t *__restrict__ off, const uint32_t *
             if (lane == j) {
                 lout[w] = m;
                 dropped += m != own;
+            }
+        }
+    }
+    dropped = wave_sum(dropped);
+    if (lane == 0 && dropped) atomicAdd(changed, (unsigned long long)dropped);
+}
+
+// Row holding entry k (the last r with off[r] <= k, off[0] = 0 <= k < off[n]):
+// a 64-ary search, one load per lane per step.  Call with all lanes.
+GCHI_DEV uint32_t pull_row_of(const uint32_t *__restrict__ off, uint32_t n, uint32_t k, int lane) {
+    uint32_t lo = 0, hi = n;  // the row is in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t span = hi - lo;
+        const uint32_t step = (span + kWave - 1) / kWave;
+        const uint32_t p = lo + (uint32_t)lane * step;
+        const bool le = p < hi && off[p] <= k;
+        const uint64_t m = __ballot(le);  // lanes 0..j hold probes <= k (lane 0 always)
+        const int j = 63 - __builtin_clzll(m);
+        lo = lo + (uint32_t)j * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
+// Rows of more than kPullHuge entries, split by entry range: wavefront c takes
+// entries [c * kPullHuge, (c + 1) * kPullHuge).  Such a row is longer than the
+// range, so the only rows of it the range can meet are the row of its first
+// entry and the row of its last one.  Each takes the min of its part and lowers
+// l_out with atomicMin (exact for integers, so the result does not depend on
+// the order); the atomic that first takes a row below l_in counts its drop.
+__global__ void cc_pull_huge(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t n,
+                             const uint32_t *__restrict__ lin, uint32_t *lout, unsigned long long *changed) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    const uint32_t E = off[n];
+    const uint64_t nchunks = ((uint64_t)E + kPullHuge - 1) / kPullHuge;
+    uint32_t dropped = 0;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; c < nchunks; c += nwv) {
+        const uint32_t c0 = (uint32_t)(c * kPullHuge);
+        const uint32_t c1 = (uint32_t)min<uint64_t>((uint64_t)E, c * kPullHuge + kPullHuge);
+        const uint32_t ra = pull_row_of(off, n, c0, lane);
+        const uint32_t rb = pull_row_of(off, n, c1 - 1, lane);
+        for (int t = 0; t < 2; t++) {
+            const uint32_t r = t ? rb : ra;
+            if (t && rb == ra) break;
+            const uint32_t b = off[r], e = off[r + 1];
+            if (e - b <= kPullHuge) continue;
+            const uint32_t k0 = max(b, c0), k1 = min(e, c1);
+            uint32_t m = 0xffffffffu;
+            for (uint32_t k = k0 + lane; k < k1; k += kWave) {
+                const uint32_t u = adj[k];
+                m = min(m, lin[u < n ? u : r]);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
+            if (lane == 0) {
+                const uint32_t own = lin[r];
+                if (m < own) {
+                    const uint32_t old = atomicMin(&lout[r], m);
+                    dropped += old == own && m < old;
+                }
             }
         }
     }
@@ -589,6 +655,8 @@ extern "C" int gchi_cc_pull_sweep(const uint32_t *d_off, const uint32_t *d_adj, 
     launch_fill_u64(d_changed, 0ull, 1, s);
     hipLaunchKernelGGL(cc_pull_sweep, dim3(grid_for(nvertices, 256, 16384)), dim3(256), 0, s, d_off, d_adj, nvertices,
                        d_labels_in, d_labels_out, reinterpret_cast<unsigned long long *>(d_changed));
+    hipLaunchKernelGGL(cc_pull_huge, dim3(4096), dim3(256), 0, s, d_off, d_adj, nvertices, d_labels_in, d_labels_out,
+                       reinterpret_cast<unsigned long long *>(d_changed));
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
 }

@@ -28,7 +28,7 @@
 //     the row at most boundaries) is "split": each tile writes two partials,
 //     head (slots before its first row start) and tail (from its last row start
 //     to its end, when that row goes on into the next tile), and the finalize
-//     sums a split row's partials in tile order and applies it.
+//     sums a split row's partials in a fixed order and applies it.
 // Tiles are equal work, so each wavefront of the persistent launch takes a
 // fixed contiguous range of them; no work queue, no per-row loops, no padding
 // beyond the last tile.
@@ -49,7 +49,7 @@ constexpr int kSegPer = 8;                    // slots per lane
 constexpr uint32_t kSegTile = kWave * kSegPer;  // slots per tile
 constexpr int kSegThreads = 1024;             // workgroup of the sweep (16 wavefronts, one per CU)
 constexpr int kSegWaves = kSegThreads / kWave;
-constexpr int kSegFinThreads = 256;
+constexpr int kSegFinThreads = 1024;
 constexpr size_t kSegStaticLds = 1024;
 constexpr uint32_t kSegCont = 0x80000000u;  // tile_row0 bit: the tile's last row goes on into the next tile
 
@@ -72,7 +72,7 @@ struct SegArgs {
     const float *rs_in;         // [N]
     float *xs_out, *rs_out;
     float *part;                // [2T] head / tail partials
-    double *tile_stat;          // [2T] residual, rank sum of the tile's whole rows
+    double *wave_stat;          // [2 * waves] residual, rank sum of the whole rows of each wavefront's tiles
     uint32_t T, N;
 };
 
@@ -114,28 +114,82 @@ GCHI_DEV uint32_t lane_prefix4(uint32_t f) {
     return pre;
 }
 
-// One tile, one wavefront.  rowsum: this wavefront's 512-float LDS slice.
-template <bool HOT>
-GCHI_DEV void seg_tile(const SegArgs &a, const PrConstsSeg &c, const float *hot, uint32_t H, float *rowsum,
-                       uint32_t t, int lane, double &res, double &sum) {
+// The loads of one tile: its slot ids, its row-start flags and its metadata.
+struct SegIds {
+    u32x4 q0, q1;
+    uint32_t fb, meta;
+};
+
+GCHI_DEV void seg_load_ids(const SegArgs &a, uint32_t t, int lane, SegIds &d) {
     const uint64_t sbase = (uint64_t)t * kSegTile + (uint64_t)lane * kSegPer;
-    const u32x4 q0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.slots + sbase));
-    const u32x4 q1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.slots + sbase + 4));
-    const uint32_t fb = a.flags[(uint64_t)t * kWave + lane];
-    const uint32_t meta = a.tile_row0[t];
-    const uint32_t ids[kSegPer] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-    float val[kSegPer];
+    d.q0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.slots + sbase));
+    d.q1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.slots + sbase + 4));
+    d.fb = a.flags[(uint64_t)t * kWave + lane];
+    d.meta = a.tile_row0[t];
+}
+
+// x of the tile's sources: the first H ranks from LDS, the others from HBM.
+// Both loads are issued for every slot (at a valid address) and the choice is
+// made where the value is used (seg_pick).  The same step loads the previous
+// rank and the out-degree of the tile's first 64 whole rows for the apply.
+struct SegVals {
+    float g[kSegPer];  // from HBM
+    float h[kSegPer];  // from the LDS cache
+    uint32_t lds;      // bit j: slot j's source is cached
+    float ro;          // previous rank of row row0 + lo + lane (clamped below N)
+    uint32_t od;       // its out-degree
+};
+
+template <bool HOT>
+GCHI_DEV void seg_gather(const SegArgs &a, const float *hot, uint32_t H, const SegIds &d, SegVals &v, int lane) {
+    const uint32_t ids[kSegPer] = {d.q0.x, d.q0.y, d.q0.z, d.q0.w, d.q1.x, d.q1.y, d.q1.z, d.q1.w};
+    v.lds = 0;
 #pragma unroll
     for (int j = 0; j < kSegPer; j++) {
         const uint32_t u = ids[j];
         const bool in_lds = HOT && u < H;
-        float g = a.xs_in[in_lds ? 0u : u];
+        v.g[j] = a.xs_in[in_lds ? 0u : u];
         if (HOT) {
-            const float h = hot[in_lds ? u : 0u];
-            g = in_lds ? h : g;
+            v.h[j] = hot[in_lds ? u : 0u];
+            v.lds |= (in_lds ? 1u : 0u) << j;
+        }
+    }
+    const uint32_t lo = (__builtin_amdgcn_readfirstlane(d.fb) & 1u) ? 0u : 1u;
+    const uint32_t i = min((d.meta & ~kSegCont) + lo + (uint32_t)lane, a.N - 1);
+    v.ro = a.rs_in[i];
+    v.od = a.outdeg_r[i];
+}
+
+// The gathered values of a tile.  The empty asm statements hand the select
+// two opaque registers: without them the compiler turns "cached ? h : g" into
+// a select of addresses and moves the arrays to scratch memory.
+template <bool HOT>
+GCHI_DEV void seg_pick(const SegVals &v, float *val) {
+#pragma unroll
+    for (int j = 0; j < kSegPer; j++) {
+        float g = v.g[j];
+        if (HOT) {
+            float h = v.h[j];
+            asm volatile("" : "+v"(g));
+            asm volatile("" : "+v"(h));
+            g = ((v.lds >> j) & 1u) ? h : g;
         }
         val[j] = g;
     }
+}
+
+// What the scan of a tile leaves for its apply.
+struct SegRows {
+    uint32_t row0, lo, hi;  // the tile's whole rows are local rows [lo, hi) from rank row0
+    float ro;               // previous rank of row row0 + lo + lane (loaded with the gathers)
+    uint32_t od;            // its out-degree
+};
+
+// One tile, one wavefront, from its gathered values: the row sums (whole rows
+// into rowsum, this wavefront's 512-float LDS slice; head and tail partials of
+// split rows into part).
+GCHI_DEV void seg_scan(const SegArgs &a, float *rowsum, uint32_t t, int lane, const float *val, uint32_t fb,
+                       uint32_t meta, SegRows &R) {
     const uint32_t row0 = meta & ~kSegCont;
     const bool cont = (meta & kSegCont) != 0;
     // the tile's slot 0 starts a row: that row is local row 0, not a new one
@@ -196,14 +250,36 @@ GCHI_DEV void seg_tile(const SegArgs &a, const PrConstsSeg &c, const float *hot,
     }
     seg_lds_sync();
     // rows wholly inside the tile: local rows [lo, hi)
-    const uint32_t lo = s0 ? 0u : 1u;
-    const uint32_t hi = anyf ? last + (cont ? 0u : 1u) : 0u;
-    for (uint32_t r0 = lo; r0 < hi; r0 += kWave) {
+    R.row0 = row0;
+    R.lo = s0 ? 0u : 1u;
+    R.hi = anyf ? last + (cont ? 0u : 1u) : 0u;
+}
+
+// The apply of the rows the scan left in LDS: r from the row sum, the
+// residual against the previous rank, x = r / outdeg.  The first 64 rows are
+// computed unconditionally (stores masked) from the values loaded with the
+// tile's gathers, so the compiler keeps those loads where they were issued.
+GCHI_DEV void seg_apply_rows(const SegArgs &a, const PrConstsSeg &c, const float *rowsum, int lane,
+                             const SegRows &R, double &res, double &sum) {
+    {
+        const uint32_t li = R.lo + (uint32_t)lane;
+        const bool ok = li < R.hi;
+        const uint32_t i = R.row0 + (ok ? li : 0u);
+        const float r = seg_apply(seg_tele(c, a.perm, i), c.damping, rowsum[ok ? li : 0u]);
+        const float x = seg_scatter(r, R.od);
+        const double d = fabs((double)r - (double)R.ro);
+        if (ok) {
+            a.rs_out[i] = r;
+            a.xs_out[i] = x;
+            res += d;
+            sum += (double)r;
+        }
+    }
+    for (uint32_t r0 = R.lo + kWave; r0 < R.hi; r0 += kWave) {
         const uint32_t li = r0 + (uint32_t)lane;
-        if (li < hi) {
-            const uint32_t i = row0 + li;
-            const float s = rowsum[li];
-            const float r = seg_apply(seg_tele(c, a.perm, i), c.damping, s);
+        if (li < R.hi) {
+            const uint32_t i = R.row0 + li;
+            const float r = seg_apply(seg_tele(c, a.perm, i), c.damping, rowsum[li]);
             const float ro = a.rs_in[i];
             a.rs_out[i] = r;
             a.xs_out[i] = seg_scatter(r, a.outdeg_r[i]);
@@ -212,6 +288,20 @@ GCHI_DEV void seg_tile(const SegArgs &a, const PrConstsSeg &c, const float *hot,
         }
     }
     seg_lds_sync();
+}
+
+// Reduces tile t from its gathered values; its rows' statistics are added to
+// the lane's res and sum.
+template <bool HOT>
+GCHI_DEV void seg_tile(const SegArgs &a, const PrConstsSeg &c, float *rowsum, uint32_t t, int lane,
+                       const SegVals &v, uint32_t fb, uint32_t meta, double &res, double &sum) {
+    float val[kSegPer];
+    seg_pick<HOT>(v, val);
+    SegRows R;
+    seg_scan(a, rowsum, t, lane, val, fb, meta, R);
+    R.ro = v.ro;
+    R.od = v.od;
+    seg_apply_rows(a, c, rowsum, lane, R, res, sum);
 }
 
 template <bool HOT>
@@ -243,29 +333,58 @@ __global__ __launch_bounds__(kSegThreads) void pr_seg_sweep_kernel(SegArgs a, Pr
     const uint64_t per = ((uint64_t)a.T + nwv - 1) / nwv;
     const uint32_t t0 = (uint32_t)min<uint64_t>(a.T, per * gw);
     const uint32_t t1 = (uint32_t)min<uint64_t>(a.T, per * gw + per);
-    for (uint32_t t = t0; t < t1; t++) {
-        double res = 0.0, sum = 0.0;
-        seg_tile<HOT>(a, c, hot, H, rowsum, t, lane, res, sum);
-        res = wave_sum(res);
-        sum = wave_sum(sum);
-        if (lane == 0) {
-            a.tile_stat[2 * (uint64_t)t] = res;
-            a.tile_stat[2 * (uint64_t)t + 1] = sum;
+    double res = 0.0, sum = 0.0;  // this lane's share of the wavefront's statistics
+    if (t0 < t1) {
+        // Two-stage pipeline: the ids of tile t+2 and the gathers (and apply
+        // loads) of tile t+1 are issued before tile t is reduced, so every wait
+        // of the reduce is for loads issued a whole tile earlier (vector-memory
+        // waits complete in issue order).  Two fixed register sets (A, B) take
+        // turns, so no value is copied between iterations (a copy would make the
+        // compiler wait for the loads in flight).  Past the wavefront's last tile
+        // the loads repeat that tile (in bounds, unused).
+        const uint32_t tl = t1 - 1;
+        SegIds iA, iB;
+        SegVals vA, vB;
+        seg_load_ids(a, t0, lane, iA);
+        seg_load_ids(a, min(t0 + 1, tl), lane, iB);
+        seg_gather<HOT>(a, hot, H, iA, vA, lane);
+        for (uint32_t t = t0;; t += 2) {
+            {  // tile t in A: ids of t+2 into A, gathers of t+1 into B, then reduce t
+                const uint32_t fb = iA.fb, meta = iA.meta;
+                seg_load_ids(a, min(t + 2, tl), lane, iA);
+                seg_gather<HOT>(a, hot, H, iB, vB, lane);
+                seg_tile<HOT>(a, c, rowsum, t, lane, vA, fb, meta, res, sum);
+            }
+            if (t + 1 >= t1) break;
+            {  // tile t+1 in B: ids of t+3 into B, gathers of t+2 into A, then reduce t+1
+                const uint32_t fb = iB.fb, meta = iB.meta;
+                seg_load_ids(a, min(t + 3, tl), lane, iB);
+                seg_gather<HOT>(a, hot, H, iA, vA, lane);
+                seg_tile<HOT>(a, c, rowsum, t + 1, lane, vB, fb, meta, res, sum);
+            }
+            if (t + 2 >= t1) break;
         }
+    }
+    res = wave_sum(res);
+    sum = wave_sum(sum);
+    if (lane == 0) {  // every wavefront writes its pair, zeros when it had no tile
+        a.wave_stat[2 * (uint64_t)gw] = res;
+        a.wave_stat[2 * (uint64_t)gw + 1] = sum;
     }
 }
 
-// Split rows: partials part[beg, end) added in tile order, then the apply.
-// One lane per row; rows of many partials by the whole wavefront (the same
-// additions in the same order).  Then the deterministic reduction of the
-// tiles' statistics, and the last workgroup writes d_stats.
+// Split rows: partials part[beg, end) summed, then the apply.  One lane per
+// row, in tile order; rows of more than kSegLaneParts partials by the whole
+// wavefront (lane-strided, then across the lanes).  Then the deterministic
+// reduction of the sweep wavefronts' statistics, and the last workgroup writes
+// d_stats.
 constexpr uint32_t kSegLaneParts = 32;
 
 __global__ __launch_bounds__(kSegFinThreads) void pr_seg_finalize_kernel(
     const uint32_t *__restrict__ split_v, const uint32_t *__restrict__ split_beg,
     const uint32_t *__restrict__ split_end, uint32_t nsplit, const float *__restrict__ part,
     const uint32_t *__restrict__ outdeg_r, const uint32_t *__restrict__ perm, const float *__restrict__ rs_in,
-    float *__restrict__ rs_out, float *__restrict__ xs_out, const double *__restrict__ tile_stat, uint32_t T,
+    float *__restrict__ rs_out, float *__restrict__ xs_out, const double *__restrict__ wave_stat, uint32_t nws,
     double *fin_stat, uint32_t *counter, double *d_stats, PrConstsSeg c) {
     __shared__ double s_red[2 * (kSegFinThreads / kWave)];
     __shared__ bool s_last;
@@ -275,7 +394,9 @@ __global__ __launch_bounds__(kSegFinThreads) void pr_seg_finalize_kernel(
     double res = 0.0, sum = 0.0;
     const uint32_t nwv = nthreads / kWave, wv = gt / kWave;
     for (uint32_t base = wv; base < nsplit; base += nthreads) {
-        // lane slots spread over all wavefronts (long rows are the lowest ranks)
+        // lane slots spread over all wavefronts: the rows of many partials are
+        // among the lowest ranks, and consecutive slots would put them in a few
+        // wavefronts one after another (measured 14% slower sweeps)
         const uint32_t h = base + (uint32_t)lane * nwv;
         const bool valid = h < nsplit;
         uint32_t b = 0, e = 0, i = 0;
@@ -302,14 +423,12 @@ __global__ __launch_bounds__(kSegFinThreads) void pr_seg_finalize_kernel(
             m &= m - 1;
             const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b, j);
             const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)e, j);
+            // lane-strided partial sums, then a butterfly: a fixed order, so
+            // sweeps stay bit-reproducible, in (b1 - b0) / 64 + 6 steps
             float w = 0.0f;
-            for (uint32_t k0 = b0; k0 < b1; k0 += kWave) {
-                const uint32_t k = k0 + lane;
-                const float pv = k < b1 ? part[k] : 0.0f;
-                const uint32_t cnt = min((uint32_t)kWave, b1 - k0);
-                for (uint32_t q = 0; q < cnt; q++)
-                    w += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), (int)q));
-            }
+            for (uint32_t k = b0 + lane; k < b1; k += kWave) w += part[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, kWave);
             if (lane == j) s = w;
         }
         if (valid) {
@@ -321,9 +440,9 @@ __global__ __launch_bounds__(kSegFinThreads) void pr_seg_finalize_kernel(
             sum += (double)r;
         }
     }
-    for (uint32_t k = gt; k < T; k += nthreads) {
-        res += tile_stat[2 * (uint64_t)k];
-        sum += tile_stat[2 * (uint64_t)k + 1];
+    for (uint32_t k = gt; k < nws; k += nthreads) {
+        res += wave_stat[2 * (uint64_t)k];
+        sum += wave_stat[2 * (uint64_t)k + 1];
     }
     res = wave_sum(res);
     sum = wave_sum(sum);
@@ -524,7 +643,7 @@ struct gchi_seg_plan {
     uint8_t *flags = nullptr;
     uint32_t *split_v = nullptr, *split_beg = nullptr, *split_end = nullptr;
     float *part = nullptr;
-    double *tile_stat = nullptr, *fin_stat = nullptr;
+    double *wave_stat = nullptr, *fin_stat = nullptr;
     uint32_t *counter = nullptr;
     float *xs[2] = {nullptr, nullptr};
     float *rs[2] = {nullptr, nullptr};
@@ -617,7 +736,7 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     z.take<uint32_t>(nsplit + 1);        // split_beg
     z.take<uint32_t>(nsplit + 1);        // split_end
     z.take<float>(2 * (size_t)T);        // part
-    z.take<double>(2 * (size_t)T);       // tile_stat
+    z.take<double>(2 * ((size_t)T + kSegWaves));  // wave_stat (at most one wavefront per tile, rounded up)
     z.take<double>(2 * 1024);            // fin_stat
     z.take<uint32_t>(64);                // counter
     z.take<float>((size_t)N + 1);        // xs[0]
@@ -640,7 +759,7 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     p->split_beg = c.take<uint32_t>(nsplit + 1);
     p->split_end = c.take<uint32_t>(nsplit + 1);
     p->part = c.take<float>(2 * (size_t)T);
-    p->tile_stat = c.take<double>(2 * (size_t)T);
+    p->wave_stat = c.take<double>(2 * ((size_t)T + kSegWaves));
     p->fin_stat = c.take<double>(2 * 1024);
     p->counter = c.take<uint32_t>(64);
     p->xs[0] = c.take<float>((size_t)N + 1);
@@ -688,7 +807,7 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     p->lds = stage + 4 * (size_t)H;
     p->nwg = (uint32_t)std::min<uint64_t>((uint64_t)ncu, ((uint64_t)T + kSegWaves - 1) / kSegWaves);
     if (p->nwg == 0) p->nwg = 1;
-    p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(64, (nsplit + kSegFinThreads - 1) / kSegFinThreads));
+    p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(16, (nsplit + kSegFinThreads - 1) / kSegFinThreads));
     const void *fn = H ? (const void *)pr_seg_sweep_kernel<true> : (const void *)pr_seg_sweep_kernel<false>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
     if (e != hipSuccess) return fail_free(hip_fail(e, "hipFuncSetAttribute", __FILE__, __LINE__));
@@ -718,14 +837,14 @@ int seg_sweep_rank(const gchi_seg_plan *p, const gchi_pr_op *op, int src, int ds
     GCHI_CHECK(!op->d_graph_teleport, GCHI_ERR_UNSUPPORTED, "pr_sweep (form 4): one graph only");
     const PrConstsSeg c{op->teleport, op->damping, op->d_seed_bits, op->seed_teleport};
     SegArgs a{p->slots, p->flags, p->tile_row0, p->outdeg_r, p->perm, p->xs[src], p->rs[src], p->xs[dst],
-              p->rs[dst], p->part, p->tile_stat, p->T, p->N};
+              p->rs[dst], p->part, p->wave_stat, p->T, p->N};
     if (p->H)
         hipLaunchKernelGGL(pr_seg_sweep_kernel<true>, dim3(p->nwg), dim3(kSegThreads), p->lds, s, a, c, p->H);
     else
         hipLaunchKernelGGL(pr_seg_sweep_kernel<false>, dim3(p->nwg), dim3(kSegThreads), p->lds, s, a, c, 0u);
     hipLaunchKernelGGL(pr_seg_finalize_kernel, dim3(p->nfin), dim3(kSegFinThreads), 0, s, p->split_v, p->split_beg,
                        p->split_end, p->nsplit, p->part, p->outdeg_r, p->perm, p->rs[src], p->rs[dst], p->xs[dst],
-                       p->tile_stat, p->T, p->fin_stat, p->counter, d_stats, c);
+                       p->wave_stat, p->nwg * (uint32_t)kSegWaves, p->fin_stat, p->counter, d_stats, c);
     GCHI_LAUNCH_CHECK();
     return GCHI_OK;
 }

@@ -168,6 +168,36 @@ def test_cc_pull_sweeps_vs_reference_golden(dev, golden, case):
     assert sweeps == osweeps
 
 
+def test_cc_pull_rows_split_across_wavefronts(dev):
+    """Rows longer than one wavefront's 4096-entry range (split over many
+    wavefronts, combined with atomicMin), rows that start and end inside such
+    ranges, and a path that needs many sweeps: the oracle's labels in the
+    oracle's number of sweeps."""
+    rng = np.random.default_rng(11)
+    n = 40000
+    src, dst = [], []
+    for hub, deg in ((5, 4097), (17, 9000), (30001, 25000), (39999, 4096), (123, 12345)):
+        nb = rng.choice(n, size=deg, replace=False)
+        nb = nb[nb != hub]
+        src.append(np.full(len(nb), hub)), dst.append(nb)
+    path = np.arange(20000, 20300)  # a path of 300 vertices: labels travel one hop per sweep
+    src.append(path[1:]), dst.append(path[:-1])
+    rnd = rng.integers(0, n, (2, 30000))
+    src.append(rnd[0]), dst.append(rnd[1])
+    src = np.concatenate(src).astype(np.uint32)
+    dst = np.concatenate(dst).astype(np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    a = np.concatenate([src, dst]).astype(np.uint32)
+    b = np.concatenate([dst, src]).astype(np.uint32)
+    g = dev.DeviceGraph(dev.to_dev_u32(a), dev.to_dev_u32(b), n)
+    assert int(np.bincount(b, minlength=n).max()) > 3 * 4096
+    lab, sweeps = dev.cc_pull(g.in_off, g.in_src, n)
+    want, osweeps = oracle.cc_labelprop(n, src, dst)
+    np.testing.assert_array_equal(dev.to_host_u32(lab, n), want)
+    assert sweeps == osweeps
+
+
 def test_cc_pull_scale24_equals_unionfind(dev):
     s, d, cnt, mx = dev.rmat_device(24, 8 << 24, 3)
     import torch
