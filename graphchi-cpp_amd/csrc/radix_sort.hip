@@ -31,9 +31,10 @@ constexpr int kRsThreadsK = 512;
 constexpr int kRsTileK = kRsThreadsK * kRsItems;
 constexpr int kRsRadix = 256;
 constexpr int kRsMaxPasses = 8;
-constexpr uint32_t kFlagAgg = 1u << 30;
-constexpr uint32_t kFlagInc = 2u << 30;
-constexpr uint32_t kValMask = (1u << 30) - 1;
+// Look-back status words: 2 flag bits above a 62-bit count.
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 26;  // default bound of one look-back wait
 constexpr int kRsLookback = 4;
 constexpr int kRsHistKeys = 8;  // keys per thread in flight in the histogram pass  // predecessor tiles read per look-back round trip
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
                                                                uint64_t *__restrict__ kout,
                                                                uint32_t *__restrict__ vout, uint64_t n, int shift,
                                                                int nbits, const uint32_t *__restrict__ gbase,
-                                                               uint32_t *status, uint32_t *tile_counter,
+                                                               uint64_t *status, uint32_t *tile_counter,
                                                                uint32_t *err, PairIO io, uint32_t spin_limit,
                                                                int force_stall) {
     constexpr int NW = T / kWave;
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
 
     // per-digit: wave exclusive offsets and tile count (thread tid owns digit tid)
     uint32_t tcount = 0, incl = 0;
-    uint32_t *my_status = status + (size_t)tile * kRsRadix + (dig ? tid : 0);
+    uint64_t *my_status = status + (size_t)tile * kRsRadix + (dig ? tid : 0);
     if (dig) {
         uint32_t run = 0;
 #pragma unroll
@@ -196,9 +197,9 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
         }
         tcount = run;
         if (tile == 0)
-            st_relaxed_agent(my_status, kFlagInc | tcount);
+            st_relaxed_agent(my_status, kFlagInc | (uint64_t)tcount);
         else
-            st_relaxed_agent(my_status, kFlagAgg | tcount);
+            st_relaxed_agent(my_status, kFlagAgg | (uint64_t)tcount);
         // tile-local exclusive scan over the 256 digits
         incl = tcount;
 #pragma unroll
@@ -220,12 +221,12 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
         // (aggregates summed) up to the first inclusive one; a word not yet
         // published ends the round and is loaded again.  Tiles below 0 read as an
         // inclusive zero.
-        uint32_t prefix = 0;
+        uint64_t prefix = 0;
         if (tile > 0) {
             int64_t p = (int64_t)tile - 1;
             uint32_t spins = 0;
             for (;;) {
-                uint32_t sv[kRsLookback];
+                uint64_t sv[kRsLookback];
 #pragma unroll
                 for (int q = 0; q < kRsLookback; q++)
                     sv[q] = p - q >= 0 ? ld_relaxed_agent(status + (size_t)(p - q) * kRsRadix + tid) : kFlagInc;
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
 #pragma unroll
                 for (int q = 0; q < kRsLookback; q++) {
                     if (inc || stalled) continue;
-                    const uint32_t flag = force_stall ? 0u : sv[q] & ~kValMask;
+                    const uint64_t flag = force_stall ? 0ull : sv[q] & ~kValMask;
                     if (flag == 0) {
                         stalled = true;
                         continue;
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
             }
             st_relaxed_agent(my_status, kFlagInc | ((prefix + tcount) & kValMask));
         }
-        s_gb[tid] = gbase[tid] + prefix - toff;
+        s_gb[tid] = gbase[tid] + (uint32_t)prefix - toff;
     }
     __syncthreads();
 
@@ -313,7 +314,7 @@ struct SortWork {
     uint64_t *ktmp;
     uint32_t *vtmp;
     uint32_t *ghist;
-    uint32_t *status;
+    uint64_t *status;  // one pass's look-back words, cleared before every pass
     uint32_t *counters;  // [passes] tile counters
 };
 
@@ -325,7 +326,7 @@ static void sort_layout(C &c, uint64_t n, int key_bits, bool vals, SortWork *w) 
     auto *kt = c.template take<uint64_t>(n);
     auto *vt = c.template take<uint32_t>(vals ? n : 1);
     auto *gh = c.template take<uint32_t>((size_t)np * kRsRadix);
-    auto *st = c.template take<uint32_t>((size_t)np * ntiles * kRsRadix);
+    auto *st = c.template take<uint64_t>((size_t)ntiles * kRsRadix);
     auto *ct = c.template take<uint32_t>(np);
     if (w) *w = SortWork{er, kt, vt, gh, st, ct};
 }
@@ -349,7 +350,7 @@ size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
 // gives the same order as sorting on [0, hi) at once (every pass is stable).
 // Key-only pass with the pair reads/writes chosen at run time.
 static void launch_key_pass(int io_mode, const uint64_t *kin, uint64_t *kout, uint64_t n, int shift, int bits,
-                            const uint32_t *gbase, uint32_t *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
+                            const uint32_t *gbase, uint64_t *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
                             uint64_t ntiles, hipStream_t s) {
     const uint32_t sl = g_spin_limit;
     const int fs = g_force_stall;
@@ -386,7 +387,7 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
     if (out) *out = keys;
     if ((n <= 1 || hi <= lo) && !work) return GCHI_OK;  // nothing to sort and no status word to clear
     GCHI_CHECK(lo >= 0 && hi <= 64, GCHI_ERR_INVALID, "sort: key bits outside [0, 64)");
-    GCHI_CHECK(n < (1ull << 30), GCHI_ERR_INVALID, "sort: n must be < 2^30");
+    GCHI_CHECK(n < (1ull << 32), GCHI_ERR_INVALID, "sort: n must be < 2^32");
     GCHI_CHECK(!(out && vals), GCHI_ERR_INVALID, "sort: values are always copied back");
     GCHI_CHECK(n <= 1 || hi <= lo || keys, GCHI_ERR_INVALID, "sort: null keys");
     const bool hv = vals != nullptr;
@@ -409,7 +410,6 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
         pd.bits[p] = p < np ? std::min(8, bits - 8 * p) : 0;
     }
     launch_fill_u32(w.ghist, 0u, (size_t)np * kRsRadix, s);
-    launch_fill_u32(w.status, 0u, (size_t)np * ntiles * kRsRadix, s);
     launch_fill_u32(w.counters, 0u, np, s);
     unsigned hblocks = (unsigned)std::min<uint64_t>(ntiles, 1024);
     const PairIO io = pio ? *pio : PairIO{};
@@ -423,7 +423,8 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
     uint64_t *ka = keys, *kb = alt ? alt : w.ktmp;
     uint32_t *va = vals, *vb = w.vtmp;
     for (int p = 0; p < np; p++) {
-        uint32_t *st = w.status + (size_t)p * ntiles * kRsRadix;
+        uint64_t *st = w.status;
+        launch_fill_u64(st, 0ull, (size_t)ntiles * kRsRadix, s);  // this pass's look-back words
         if (hv)
             hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads>), dim3((unsigned)ntiles), dim3(kRsThreads), 0, s,
                                ka, va, kb, vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st,

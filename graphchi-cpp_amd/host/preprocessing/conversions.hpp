@@ -1,7 +1,7 @@
 // Graph input conversion (src/preprocessing/conversions.hpp:210-288, 611-790):
 // edge lists ("<from> <to> [value]" with tab, comma or space delimiters, '#'
-// and '%' comments) are read and handed to the device sharder, unless the
-// graph is already sharded.
+// and '%' comments) are parsed on the device and handed to the device sharder,
+// unless the graph is already sharded.
 #pragma once
 #include <cstdio>
 #include <cstdlib>
@@ -9,31 +9,44 @@
 #include <string>
 
 #include "api/chifilenames.hpp"
+#include "engine/device.hpp"
 #include "logger/logger.hpp"
 #include "preprocessing/sharder.hpp"
 #include "util/cmdopts.hpp"
 
 namespace graphchi {
 
+// convert_edgelist's loop (conversions.hpp:210-288) on the device: the file is
+// read into pinned host memory, uploaded once and parsed by gchi_parse_edgelist
+// (the same fields, comments, atoi values and self-loop filter); the pairs stay
+// in HBM for the sharder.  A line without two fields stops with the fatal log,
+// as the reference's assertion does.
 template <typename EdgeDataType, typename FinalEdgeDataType>
 void convert_edgelist(std::string inputfile, sharder<EdgeDataType, FinalEdgeDataType> &sh) {
-    FILE *f = fopen(inputfile.c_str(), "r");
+    device::open();
+    FILE *f = fopen(inputfile.c_str(), "rb");
     if (!f) logstream(LOG_FATAL) << "Could not load: " << inputfile << std::endl;
-    char line[1024];
-    while (fgets(line, sizeof(line), f)) {
-        if (line[0] == '#' || line[0] == '%') continue;
-        char *p = line, *end = nullptr;
-        while (*p == ' ' || *p == '\t' || *p == ',') p++;
-        if (*p == '\n' || *p == '\r' || *p == 0) continue;
-        unsigned long a = strtoul(p, &end, 10);
-        if (end == p) logstream(LOG_FATAL) << "Input file is not in right format: " << line << std::endl;
-        p = end;
-        while (*p == ' ' || *p == '\t' || *p == ',') p++;
-        unsigned long b = strtoul(p, &end, 10);
-        if (end == p) logstream(LOG_FATAL) << "Input file is not in right format: " << line << std::endl;
-        sh.preprocessing_add_edge((vid_t)a, (vid_t)b);
-    }
+    fseek(f, 0, SEEK_END);
+    const long len = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    const uint64_t n = len > 0 ? (uint64_t)len : 0;
+    device::pinned<uint8_t> text(n);
+    if (n && fread(text.get(), 1, n, f) != n) logstream(LOG_FATAL) << "short read from " << inputfile << std::endl;
     fclose(f);
+    device::buffer<uint8_t> d_text(n);
+    if (n) d_text.upload(text.get(), n);
+    size_t wb = 0;
+    device::check(gchi_parse_edgelist_workspace(n, &wb), "gchi_parse_edgelist_workspace");
+    device::workspace w(wb);
+    uint64_t ne = 0;
+    uint32_t maxid = 0;
+    device::check(gchi_parse_edgelist(d_text.get(), n, nullptr, nullptr, 0, &ne, &maxid, w.get(), wb, nullptr),
+                  "Input file is not in right format (gchi_parse_edgelist)");
+    device::buffer<uint32_t> src(ne), dst(ne);
+    if (ne)
+        device::check(gchi_parse_edgelist(d_text.get(), n, src.get(), dst.get(), ne, &ne, &maxid, w.get(), wb, nullptr),
+                      "Input file is not in right format (gchi_parse_edgelist)");
+    sh.add_device_edges(std::move(src), std::move(dst), ne, maxid);
 }
 
 // Binary pairs of uint32 (from, to), back to back.

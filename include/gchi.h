@@ -100,9 +100,9 @@ int gchi_parse_edgelist(const uint8_t *d_text, uint64_t nbytes, uint32_t *d_src,
 
 /* ---------------------------------------------------------------- radix sort */
 /* Stable LSD onesweep radix sort of 64-bit keys on their low key_bits bits
- * (8-bit digits; optional 32-bit payload).  Replaces iSort/radixStep/eBits of
- * src/util/radixSort.hpp:49-116.  d_keys/d_vals are sorted in place (the
- * library ping-pongs through the workspace). */
+ * (8-bit digits; optional 32-bit payload), n < 2^32.  Replaces
+ * iSort/radixStep/eBits of src/util/radixSort.hpp:49-116.  d_keys/d_vals are
+ * sorted in place (the library ping-pongs through the workspace). */
 int gchi_sort_workspace(uint64_t n, int key_bits, int with_values, size_t *bytes);
 int gchi_sort_u64(uint64_t *d_keys, uint32_t *d_vals, uint64_t n, int key_bits,
                   void *d_work, size_t work_bytes, void *stream);

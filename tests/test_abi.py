@@ -44,8 +44,9 @@ def test_status_strings_and_invalid_arguments():
     assert L.gchi_sort_u64(None, None, 10, 64, None, 0, None) == -1
     nb = ctypes.c_size_t()
     assert L.gchi_sort_workspace(1 << 20, 44, 0, ctypes.byref(nb)) == 0 and nb.value > (1 << 23)
-    # sort refuses sizes its 30-bit look-back counters cannot hold
-    assert L.gchi_sort_u64(ctypes.c_void_p(1), None, 1 << 30, 64, ctypes.c_void_p(1), 1 << 40, None) == -1
+    # sort refuses sizes its 32-bit scatter positions cannot hold
+    assert L.gchi_sort_u64(ctypes.c_void_p(1), None, 1 << 32, 64, ctypes.c_void_p(1), 1 << 40, None) == -1
+    assert L.gchi_last_error(buf, 256) == 0 and b"2^32" in buf.value
 
 
 def test_host_generator_deterministic():

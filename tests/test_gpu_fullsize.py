@@ -132,3 +132,25 @@ def test_multigraph_2048_vs_oracle(dev, multigraph2048, gi):
           f"L1 vs FP64 {np.abs(got - r64).sum():.3e}")
     assert rel64 <= 5e-5, rel64
     assert rel32 <= 2e-5, rel32
+
+
+def test_sort_beyond_2p30_keys(dev):
+    """gchi_sort_u64 on 2^30 + 12,345 keys (more than the 30-bit look-back
+    counts of earlier versions allowed), 40-bit keys in five passes: sorted,
+    and a permutation of the input (the count of every top-byte value and the
+    wrapping sum of the keys unchanged).  Size-independent checks: numpy would
+    take minutes."""
+    import torch
+    torch.cuda.empty_cache()
+    n = (1 << 30) + 12345
+    g = torch.Generator(device="cuda").manual_seed(7)
+    keys = torch.randint(0, 1 << 40, (n,), dtype=torch.int64, device="cuda", generator=g)
+    hist0 = torch.bincount(keys >> 32, minlength=256)
+    sum0 = int(keys.sum().item())
+    ws = dev.sort_u64(keys, key_bits=40)
+    del ws
+    assert bool((keys[1:] >= keys[:-1]).all())
+    assert torch.equal(torch.bincount(keys >> 32, minlength=256), hist0)
+    assert int(keys.sum().item()) == sum0
+    del keys
+    torch.cuda.empty_cache()
