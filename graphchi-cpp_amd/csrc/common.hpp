@@ -14,6 +14,9 @@
 namespace gchi {
 
 constexpr int kWave = 64;  // CDNA wavefront width
+// Largest edge count of one graph: edge offsets are 32-bit, and a 32-bit edge
+// index plus a grid stride (at most 2^24) must not wrap past 2^32.
+constexpr uint64_t kMaxEdges = (1ull << 32) - (1ull << 25);
 
 // ---- error reporting (thread-local last error message) ----
 void set_error(const std::string &msg);

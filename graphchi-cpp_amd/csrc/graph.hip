@@ -101,7 +101,7 @@ static void csc_layout(C &c, uint64_t E, uint32_t N, uint64_t **keys, uint32_t *
 
 int csc_build(const uint32_t *src, const uint32_t *dst, uint64_t E, uint32_t N, uint32_t *in_off, uint32_t *in_src,
               uint32_t *outdeg, uint32_t *indeg_out, void *work, size_t work_bytes, hipStream_t s) {
-    GCHI_CHECK(E < (1ull << 32), GCHI_ERR_UNSUPPORTED, "csc_build: E must be < 2^32 (uint32 offsets)");
+    GCHI_CHECK(E <= kMaxEdges, GCHI_ERR_UNSUPPORTED, "csc_build: E must be <= 2^32 - 2^25 (uint32 offsets)");
     GCHI_CHECK(in_off && (E == 0 || (src && dst && in_src)), GCHI_ERR_INVALID, "csc_build: null pointer");
     Carver c(work, work_bytes);
     uint64_t *keys;

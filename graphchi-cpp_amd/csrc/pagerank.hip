@@ -1701,7 +1701,7 @@ static int check_csc(const gchi_dgraph *g) {
     GCHI_CHECK(g->nvertices > 0, GCHI_ERR_INVALID, "pr_plan_create: graph has no vertices");
     GCHI_CHECK(g->in_off && g->outdeg && (g->nedges == 0 || g->in_src), GCHI_ERR_INVALID,
                "pr_plan_create: graph arrays missing");
-    GCHI_CHECK(g->nedges < (1ull << 32), GCHI_ERR_UNSUPPORTED, "pr_plan_create: a graph's E must be < 2^32");
+    GCHI_CHECK(g->nedges <= kMaxEdges, GCHI_ERR_UNSUPPORTED, "pr_plan_create: a graph's E must be <= 2^32 - 2^25");
     return GCHI_OK;
 }
 

@@ -25,13 +25,16 @@
 //     lanes.  Rows wholly inside the tile are applied by the same wavefront
 //     (one lane per row, from sums staged in LDS);
 //   * a row that crosses a tile boundary (every row longer than a tile, and
-//     the row at most boundaries) is "split": each tile writes two partials,
-//     head (slots before its first row start) and tail (from its last row start
-//     to its end, when that row goes on into the next tile), and the finalize
-//     sums a split row's partials in a fixed order and applies it.
-// Tiles are equal work, so each wavefront of the persistent launch takes a
-// fixed contiguous range of them; no work queue, no per-row loops, no padding
-// beyond the last tile.
+//     the row at most boundaries) is "split".  Tiles are equal work, so each
+//     wavefront of the persistent launch takes a fixed contiguous range of
+//     them (no work queue, no per-row loops, no padding beyond the last tile)
+//     and carries the sum of the row in progress from tile to tile: a split
+//     row that starts and ends inside one wavefront's range is applied by that
+//     wavefront.  For the rows that cross from one range into the next (at
+//     most one per range boundary) every tile also writes two partials, head
+//     (slots before its first row start) and tail (from its last row start to
+//     its end, when that row goes on into the next tile), and the finalize
+//     sums such a row's partials in the same order and applies it.
 //
 // State lives in rank space: the plan owns xs[2] (N+1 floats, xs[N] = 0) and
 // rs[2].  A HIP-graph exec keeps every sweep there and permutes only on entry
@@ -72,8 +75,9 @@ struct SegArgs {
     const float *rs_in;         // [N]
     float *xs_out, *rs_out;
     float *part;                // [2T] head / tail partials
-    double *wave_stat;          // [2 * waves] residual, rank sum of the whole rows of each wavefront's tiles
+    double *wave_stat;          // [2 * waves] residual, rank sum of the rows each wavefront applied
     uint32_t T, N;
+    uint32_t per;               // tiles per wavefront: wavefront w takes tiles [w*per, w*per + per)
 };
 
 GCHI_DEV double seg_tele(const PrConstsSeg &c, const uint32_t *perm, uint32_t i) {
@@ -130,13 +134,13 @@ GCHI_DEV void seg_load_ids(const SegArgs &a, uint32_t t, int lane, SegIds &d) {
 
 // x of the tile's sources: the first H ranks from LDS, the others from HBM
 // (lanes with a cached source skip the HBM load); the choice is made where the
-// value is used (seg_pick).  The same step loads the previous
-// rank and the out-degree of the tile's first 64 whole rows for the apply.
+// value is used (seg_pick).  The same step loads the previous rank and the
+// out-degree of the tile's first 64 rows (local rows 0..63) for the apply.
 struct SegVals {
     float g[kSegPer];  // from HBM
     float h[kSegPer];  // from the LDS cache
     uint32_t lds;      // bit j: slot j's source is cached
-    float ro;          // previous rank of row row0 + lo + lane (clamped below N)
+    float ro;          // previous rank of row row0 + lane (clamped below N)
     uint32_t od;       // its out-degree
 };
 
@@ -156,8 +160,7 @@ GCHI_DEV void seg_gather(const SegArgs &a, const float *hot, uint32_t H, const S
             v.lds |= (in_lds ? 1u : 0u) << j;
         }
     }
-    const uint32_t lo = (__builtin_amdgcn_readfirstlane(d.fb) & 1u) ? 0u : 1u;
-    const uint32_t i = min((d.meta & ~kSegCont) + lo + (uint32_t)lane, a.N - 1);
+    const uint32_t i = min((d.meta & ~kSegCont) + (uint32_t)lane, a.N - 1);
     v.ro = a.rs_in[i];
     v.od = a.outdeg_r[i];
 }
@@ -182,16 +185,24 @@ GCHI_DEV void seg_pick(const SegVals &v, float *val) {
 
 // What the scan of a tile leaves for its apply.
 struct SegRows {
-    uint32_t row0, lo, hi;  // the tile's whole rows are local rows [lo, hi) from rank row0
-    float ro;               // previous rank of row row0 + lo + lane (loaded with the gathers)
+    uint32_t row0, lo, hi;  // the rows the tile applies are local rows [lo, hi) from rank row0
+    float ro;               // previous rank of row row0 + lane (loaded with the gathers)
     uint32_t od;            // its out-degree
 };
 
-// One tile, one wavefront, from its gathered values: the row sums (whole rows
-// into rowsum, this wavefront's 512-float LDS slice; head and tail partials of
-// split rows into part).
+// The row a wavefront carries from one tile into the next: its sum so far,
+// and whether it started inside the wavefront's range (then the wavefront
+// applies it; otherwise it is a row of the finalize).  Wavefront-uniform.
+struct SegCarry {
+    float sum = 0.0f;
+    bool open = false;
+};
+
+// One tile, one wavefront, from its gathered values: the row sums (rows the
+// tile applies into rowsum, this wavefront's 512-float LDS slice; head and
+// tail partials into part), and the carry into the next tile.
 GCHI_DEV void seg_scan(const SegArgs &a, float *rowsum, uint32_t t, int lane, const float *val, uint32_t fb,
-                       uint32_t meta, SegRows &R) {
+                       uint32_t meta, SegCarry &k, SegRows &R) {
     const uint32_t row0 = meta & ~kSegCont;
     const bool cont = (meta & kSegCont) != 0;
     // the tile's slot 0 starts a row: that row is local row 0, not a new one
@@ -230,12 +241,18 @@ GCHI_DEV void seg_scan(const SegArgs &a, float *rowsum, uint32_t t, int lane, co
     const uint64_t anyf = __ballot(fb != 0);
     const float total = __shfl(v, kWave - 1, kWave);  // sum since the tile's last row start
     const uint32_t last = __shfl(loc, kWave - 1, kWave);  // local index of the tile's last row
+    // the row coming from the previous tile ends here and began in this range
+    // the carried row (begun in this range) ends here: at the tile's first row
+    // start, or with the tile when one row covers it and does not go on
+    const bool ends = k.open && anyf != 0;
+    const bool ends_whole = k.open && !anyf && !cont;
     if (fb != 0) {
         // this lane's head ends the row in progress at its start
         const uint32_t hrow = pre;
         const float hs = carry + head;
-        if (hrow == 0 && !s0) {
-            a.part[2 * (uint64_t)t] = hs;  // the row coming from the previous tile
+        if (hrow == 0 && !s0) {  // the row coming from the previous tile
+            if (ends) rowsum[0] = k.sum + hs;
+            else a.part[2 * (uint64_t)t] = hs;
         } else if (!(lane == 0 && s0 && hrow == 0)) {
             rowsum[hrow] = hs;
         }
@@ -244,17 +261,28 @@ GCHI_DEV void seg_scan(const SegArgs &a, float *rowsum, uint32_t t, int lane, co
         if (!anyf) {  // one row covers the whole tile
             a.part[2 * (uint64_t)t] = total;
             a.part[2 * (uint64_t)t + 1] = 0.0f;
+            if (ends_whole) rowsum[0] = k.sum + total;
         } else {
             if (s0) a.part[2 * (uint64_t)t] = 0.0f;
             a.part[2 * (uint64_t)t + 1] = cont ? total : 0.0f;
             if (!cont) rowsum[last] = total;
         }
     }
+    // a carried row is open only after a tile whose last row goes on (so a
+    // tile with one is never s0), and stays open across the tiles it covers
+    if (!anyf) {
+        k.sum = k.sum + total;
+        k.open = k.open && cont;
+    } else {
+        k.sum = total;
+        k.open = cont;
+    }
     seg_lds_sync();
-    // rows wholly inside the tile: local rows [lo, hi)
+    // local rows [lo, hi): the rows ending in the tile, less the one that came
+    // from before this wavefront's range
     R.row0 = row0;
-    R.lo = s0 ? 0u : 1u;
-    R.hi = anyf ? last + (cont ? 0u : 1u) : 0u;
+    R.lo = (s0 || ends || ends_whole) ? 0u : 1u;
+    R.hi = anyf ? last + (cont ? 0u : 1u) : (ends_whole ? 1u : 0u);
 }
 
 // The apply of the rows the scan left in LDS: r from the row sum, the
@@ -264,8 +292,8 @@ GCHI_DEV void seg_scan(const SegArgs &a, float *rowsum, uint32_t t, int lane, co
 GCHI_DEV void seg_apply_rows(const SegArgs &a, const PrConstsSeg &c, const float *rowsum, int lane,
                              const SegRows &R, double &res, double &sum) {
     {
-        const uint32_t li = R.lo + (uint32_t)lane;
-        const bool ok = li < R.hi;
+        const uint32_t li = (uint32_t)lane;
+        const bool ok = li >= R.lo && li < R.hi;
         const uint32_t i = R.row0 + (ok ? li : 0u);
         const float r = seg_apply(seg_tele(c, a.perm, i), c.damping, rowsum[ok ? li : 0u]);
         const float x = seg_scatter(r, R.od);
@@ -277,7 +305,7 @@ GCHI_DEV void seg_apply_rows(const SegArgs &a, const PrConstsSeg &c, const float
             sum += (double)r;
         }
     }
-    for (uint32_t r0 = R.lo + kWave; r0 < R.hi; r0 += kWave) {
+    for (uint32_t r0 = kWave; r0 < R.hi; r0 += kWave) {
         const uint32_t li = r0 + (uint32_t)lane;
         if (li < R.hi) {
             const uint32_t i = R.row0 + li;
@@ -296,11 +324,11 @@ GCHI_DEV void seg_apply_rows(const SegArgs &a, const PrConstsSeg &c, const float
 // the lane's res and sum.
 template <bool HOT>
 GCHI_DEV void seg_tile(const SegArgs &a, const PrConstsSeg &c, float *rowsum, uint32_t t, int lane,
-                       const SegVals &v, uint32_t fb, uint32_t meta, double &res, double &sum) {
+                       const SegVals &v, uint32_t fb, uint32_t meta, SegCarry &k, double &res, double &sum) {
     float val[kSegPer];
     seg_pick<HOT>(v, val);
     SegRows R;
-    seg_scan(a, rowsum, t, lane, val, fb, meta, R);
+    seg_scan(a, rowsum, t, lane, val, fb, meta, k, R);
     R.ro = v.ro;
     R.od = v.od;
     seg_apply_rows(a, c, rowsum, lane, R, res, sum);
@@ -330,12 +358,12 @@ __global__ __launch_bounds__(kSegThreads) void pr_seg_sweep_kernel(SegArgs a, Pr
         }
     }
     __syncthreads();
-    const uint32_t nwv = gridDim.x * kSegWaves;
     const uint32_t gw = blockIdx.x * kSegWaves + wave;
-    const uint64_t per = ((uint64_t)a.T + nwv - 1) / nwv;
+    const uint64_t per = a.per;
     const uint32_t t0 = (uint32_t)min<uint64_t>(a.T, per * gw);
     const uint32_t t1 = (uint32_t)min<uint64_t>(a.T, per * gw + per);
     double res = 0.0, sum = 0.0;  // this lane's share of the wavefront's statistics
+    SegCarry k;
     if (t0 < t1) {
         // Two-stage pipeline: the ids of tile t+2 and the gathers (and apply
         // loads) of tile t+1 are issued before tile t is reduced, so every wait
@@ -355,14 +383,14 @@ __global__ __launch_bounds__(kSegThreads) void pr_seg_sweep_kernel(SegArgs a, Pr
                 const uint32_t fb = iA.fb, meta = iA.meta;
                 seg_load_ids(a, min(t + 2, tl), lane, iA);
                 seg_gather<HOT>(a, hot, H, iB, vB, lane);
-                seg_tile<HOT>(a, c, rowsum, t, lane, vA, fb, meta, res, sum);
+                seg_tile<HOT>(a, c, rowsum, t, lane, vA, fb, meta, k, res, sum);
             }
             if (t + 1 >= t1) break;
             {  // tile t+1 in B: ids of t+3 into B, gathers of t+2 into A, then reduce t+1
                 const uint32_t fb = iB.fb, meta = iB.meta;
                 seg_load_ids(a, min(t + 3, tl), lane, iB);
                 seg_gather<HOT>(a, hot, H, iA, vA, lane);
-                seg_tile<HOT>(a, c, rowsum, t + 1, lane, vB, fb, meta, res, sum);
+                seg_tile<HOT>(a, c, rowsum, t + 1, lane, vB, fb, meta, k, res, sum);
             }
             if (t + 2 >= t1) break;
         }
@@ -375,8 +403,8 @@ __global__ __launch_bounds__(kSegThreads) void pr_seg_sweep_kernel(SegArgs a, Pr
     }
 }
 
-// Split rows: partials part[beg, end) summed, then the apply.  One lane per
-// row, in tile order; rows of more than kSegLaneParts partials by the whole
+// Rows crossing from one sweep wavefront's range into the next: partials
+// part[beg, end) summed, then the apply.  One lane per row, in tile order; rows of more than kSegLaneParts partials by the whole
 // wavefront (lane-strided, then across the lanes).  Then the deterministic
 // reduction of the sweep wavefronts' statistics, and the last workgroup writes
 // d_stats.
@@ -613,11 +641,15 @@ __global__ void seg_tiles(const uint64_t *soff, uint32_t n, uint64_t S, uint32_t
     }
 }
 
-// split[i] = 1 when row i's first and last slot lie in different tiles
-__global__ void seg_split_flags(const uint64_t *soff, uint32_t n, uint32_t *split) {
+// split[i] = 1 when row i's first and last slot lie in tiles of different
+// wavefronts' ranges (per tiles each); a row crossing tiles inside one range
+// is summed and applied by its wavefront
+__global__ void seg_split_flags(const uint64_t *soff, uint32_t n, uint64_t per, uint32_t *split) {
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        split[i] = (soff[i] / kSegTile != (soff[i + 1] - 1) / kSegTile) ? 1u : 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t ta = soff[i] / kSegTile, tb = (soff[i + 1] - 1) / kSegTile;
+        split[i] = ta / per != tb / per ? 1u : 0u;  // every row has at least one slot
+    }
 }
 
 // Split row h: partials part[2*t0 + 1 .. 2*t1] inclusive (its first tile's
@@ -638,7 +670,7 @@ __global__ void seg_split_list(const uint64_t *soff, const uint32_t *split, cons
 }  // namespace gchi
 
 struct gchi_seg_plan {
-    uint32_t N = 0, T = 0, H = 0, nsplit = 0, nwg = 0, nfin = 0;
+    uint32_t N = 0, T = 0, H = 0, nsplit = 0, nwg = 0, nfin = 0, per = 1;
     uint64_t S = 0, E = 0;
     size_t lds = 0;
     uint32_t *perm = nullptr, *outdeg_r = nullptr, *slots = nullptr, *tile_row0 = nullptr;
@@ -720,8 +752,23 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     const uint32_t T = (uint32_t)T64;
     p->S = S;
     p->T = T;
-    // split rows
-    hipLaunchKernelGGL(seg_split_flags, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, soff, N, split);
+    // launch shape: one 1024-thread workgroup per CU, each wavefront a fixed
+    // range of per tiles
+    int dev = 0, ncu = 256;
+    size_t lds_cu = 160 * 1024, lds_wg = 160 * 1024;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        ncu = prop.multiProcessorCount;
+        lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+        lds_wg = prop.sharedMemPerBlock;
+    }
+    p->nwg = (uint32_t)std::min<uint64_t>((uint64_t)ncu, ((uint64_t)T + kSegWaves - 1) / kSegWaves);
+    if (p->nwg == 0) p->nwg = 1;
+    const uint64_t nwv = (uint64_t)p->nwg * kSegWaves;
+    p->per = (uint32_t)std::max<uint64_t>(1, (T + nwv - 1) / nwv);
+    // the rows of the finalize: those crossing from one wavefront's range into the next
+    hipLaunchKernelGGL(seg_split_flags, dim3(grid_for(N, 256, 65536)), dim3(256), 0, s, soff, N, (uint64_t)p->per,
+                       split);
     if ((st = exclusive_scan_u32(split, spos, N, sw, swb, s)) != GCHI_OK) return fail_free(st);
     uint32_t nsplit = 0;
     if ((st = seg_read(&nsplit, spos + N, 4, s)) != GCHI_OK) return fail_free(st);
@@ -790,16 +837,7 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     uint32_t h_bad = 0;
     if ((st = seg_read(&h_bad, bad, 4, s)) != GCHI_OK) return fail_free(st);
     if (h_bad) return fail_free(fail(GCHI_ERR_INVALID, "pr_plan_create: an in-edge source id >= nvertices"));
-    // launch shape: one 1024-thread workgroup per CU; LDS holds the H hottest
-    // ranks plus a 512-float row-sum slice per wavefront
-    int dev = 0, ncu = 256;
-    size_t lds_cu = 160 * 1024, lds_wg = 160 * 1024;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-        ncu = prop.multiProcessorCount;
-        lds_cu = prop.maxSharedMemoryPerMultiProcessor;
-        lds_wg = prop.sharedMemPerBlock;
-    }
+    // LDS holds the H hottest ranks plus a 512-float row-sum slice per wavefront
     const size_t cap = std::min(lds_cu, lds_wg);
     const size_t stage = (size_t)kSegWaves * kSegTile * 4;
     const size_t hmax = cap > stage + kSegStaticLds ? (cap - stage - kSegStaticLds) / 4 / 1024 * 1024 : 0;
@@ -807,8 +845,6 @@ int seg_plan_create(const gchi_dgraph *g, int hot, gchi_seg_plan **out, hipStrea
     H = std::min<uint32_t>(H, N / 1024 * 1024);
     p->H = H;
     p->lds = stage + 4 * (size_t)H;
-    p->nwg = (uint32_t)std::min<uint64_t>((uint64_t)ncu, ((uint64_t)T + kSegWaves - 1) / kSegWaves);
-    if (p->nwg == 0) p->nwg = 1;
     p->nfin = std::min<uint32_t>(1024, std::max<uint32_t>(16, (nsplit + kSegFinThreads - 1) / kSegFinThreads));
     const void *fn = H ? (const void *)pr_seg_sweep_kernel<true> : (const void *)pr_seg_sweep_kernel<false>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
@@ -839,7 +875,7 @@ int seg_sweep_rank(const gchi_seg_plan *p, const gchi_pr_op *op, int src, int ds
     GCHI_CHECK(!op->d_graph_teleport, GCHI_ERR_UNSUPPORTED, "pr_sweep (form 4): one graph only");
     const PrConstsSeg c{op->teleport, op->damping, op->d_seed_bits, op->seed_teleport};
     SegArgs a{p->slots, p->flags, p->tile_row0, p->outdeg_r, p->perm, p->xs[src], p->rs[src], p->xs[dst],
-              p->rs[dst], p->part, p->wave_stat, p->T, p->N};
+              p->rs[dst], p->part, p->wave_stat, p->T, p->N, p->per};
     if (p->H)
         hipLaunchKernelGGL(pr_seg_sweep_kernel<true>, dim3(p->nwg), dim3(kSegThreads), p->lds, s, a, c, p->H);
     else

@@ -279,7 +279,7 @@ extern "C" int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, ui
                                 int nshards, gchi_shards **out, void *stream) {
     GCHI_CHECK(out && nshards >= 1 && nshards <= 65536, GCHI_ERR_INVALID, "shard_build: bad argument");
     GCHI_CHECK(nedges == 0 || (d_src && d_dst), GCHI_ERR_INVALID, "shard_build: null edges");
-    GCHI_CHECK(nedges < (1ull << 32), GCHI_ERR_UNSUPPORTED, "shard_build: E must be < 2^32");
+    GCHI_CHECK(nedges <= kMaxEdges, GCHI_ERR_UNSUPPORTED, "shard_build: E must be <= 2^32 - 2^25");
     hipStream_t s = as_stream(stream);
     const uint32_t N = max_id + 1;
     GCHI_CHECK(N > max_id, GCHI_ERR_UNSUPPORTED, "shard_build: max_id too large");

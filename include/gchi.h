@@ -143,14 +143,15 @@ int gchi_sort_pairs_dst_src(const uint32_t *d_src, const uint32_t *d_dst, uint64
 typedef struct gchi_dgraph {
     uint32_t nvertices;        /* N = 1 + max vertex id */
     uint32_t reserved;
-    uint64_t nedges;           /* E (< 2^32) */
+    uint64_t nedges;           /* E (<= 2^32 - 2^25) */
     const uint32_t *in_off;    /* [N+1] */
     const uint32_t *in_src;    /* [E]   */
     const uint32_t *outdeg;    /* [N]   */
     const uint32_t *indeg;     /* [N]   (optional, may be NULL) */
 } gchi_dgraph;
 
-/* Builds the CSC arrays from an edge list already resident in HBM. */
+/* Builds the CSC arrays from an edge list already resident in HBM;
+ * nedges <= 2^32 - 2^25. */
 int gchi_csc_build_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
 int gchi_csc_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                    uint32_t *d_in_off, uint32_t *d_in_src, uint32_t *d_outdeg, uint32_t *d_indeg,
@@ -240,7 +241,7 @@ int gchi_pr_plan_create_multi(const gchi_dgraph *g, const uint32_t *h_vbase, uin
  * its own in_off/in_src/outdeg, any device allocations): graph k owns plan
  * vertices [V_k, V_k + N_k), V_k = N_0 + ... + N_{k-1}, in the x/r vectors;
  * d_graph_teleport and d_stats are per graph as for gchi_pr_plan_create_multi.
- * Each graph needs E_k < 2^32; the total may exceed 2^32 (offsets are 32-bit
+ * Each graph needs E_k <= 2^32 - 2^25; the total may exceed 2^32 (offsets are 32-bit
  * per graph, 64-bit across graphs), total vertices < 2^32.  The plan copies
  * what it needs: the graphs may be freed once it is created.  All graphs of a
  * batch are swept by one launch of the sweep kernel. */
@@ -473,7 +474,7 @@ int gchi_shard_auto_count(uint64_t nedges, uint32_t edge_value_bytes, int membud
 /* Shards an edge list resident in HBM (self-loops already dropped): interval
  * cut of sharder::add/createnextshard/done (:861-927), per-shard (src,dst)
  * order of finish_shard (:640), degrees (:748-751).  The handle owns the
- * shard-major edge arrays in HBM. */
+ * shard-major edge arrays in HBM.  nedges <= 2^32 - 2^25. */
 typedef struct gchi_shards gchi_shards;
 int gchi_shard_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t max_id, int nshards,
                      gchi_shards **out, void *stream);

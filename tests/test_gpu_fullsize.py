@@ -1,6 +1,8 @@
 """GPU parity at the sizes the benchmark runs: the (dst,src) sort of the 67.1M
 scale-22 pairs, the 1,024-query batched PPR on the scale-22 graph, and the
-2,048-graph batch (scales 10-20, edge factors 8-32), each against the oracle.
+2,048-graph batch (scales 10-20, edge factors 8-32), each against the oracle;
+then sizes beyond 2^30: a sort of 2^30 + 12,345 keys and a 1.5G-edge graph in
+one PageRank plan, through size-independent checks.
 
 Sort: bit-exact against numpy.  PPR: L1 <= 1e-6 per query against the FP64
 oracle (run to a 1e-11 fixed-point residual, far below the tolerance).  Many
@@ -153,4 +155,52 @@ def test_sort_beyond_2p30_keys(dev):
     assert torch.equal(torch.bincount(keys >> 32, minlength=256), hist0)
     assert int(keys.sum().item()) == sum0
     del keys
+    torch.cuda.empty_cache()
+
+
+def test_pagerank_one_plan_beyond_2p30_edges(dev):
+    """R-MAT 27 with 1.5G edges (more than 2^30) in one CSC and one form-4
+    plan, one sweep from r = 1, x = 1/outdeg.  That sweep has a closed form:
+    r_v = 0.15 + 0.85 * sum over in-edges u->v of 1/outdeg(u), so the rank
+    sum is 0.15 N + 0.85 * (vertices with out-edges).  Checked: that sum
+    against the sweep's FP64 statistics and the ranks, and 40 rows (the 8
+    largest in-degrees, rows without in-edges, random rows) against an FP64
+    host sum over the raw edge list."""
+    import torch
+    torch.cuda.empty_cache()
+    s, d, e, maxid = dev.rmat_device(27, 1_500_000_000, 5)
+    n = maxid + 1
+    assert e > (1 << 30)
+    g = dev.DeviceGraph(s, d, n)
+    outdeg = dev.to_host_u32(g.outdeg).astype(np.int64)
+    indeg = dev.to_host_u32(g.indeg).astype(np.int64)
+    assert int(outdeg.sum()) == e and int(indeg.sum()) == e
+    plan = dev.PageRankPlan(g, persistent=4, hot_vertices=1 << 20)
+    x = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(2)]
+    r = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(2)]
+    st = torch.zeros(2, dtype=torch.float64, device="cuda")
+    plan.init(1.0, x[0], r[0])
+    plan.sweep(dev.pr_op(0.15, D), x[0], r[0], x[1], r[1], st)
+    got = r[1].cpu().numpy().astype(np.float64)
+    stats = st.cpu().numpy()
+    want_sum = 0.15 * n + D * float(np.count_nonzero(outdeg))
+    assert abs(stats[1] - want_sum) <= 1e-6 * want_sum, (stats[1], want_sum)
+    assert abs(got.sum() - want_sum) <= 1e-6 * want_sum
+    rng = np.random.default_rng(3)
+    rows = np.concatenate([np.argsort(indeg)[-8:], np.flatnonzero(indeg == 0)[:8],
+                           rng.integers(0, n, 24)]).astype(np.int64)
+    mark = torch.zeros(n, dtype=torch.bool, device="cuda")
+    mark[torch.from_numpy(rows).cuda()] = True
+    sel = mark[d.long()]
+    hs = s[sel].cpu().numpy().view(np.uint32).astype(np.int64)
+    hd = d[sel].cpu().numpy().view(np.uint32).astype(np.int64)
+    del sel, mark
+    acc = np.zeros(n, np.float64)
+    np.add.at(acc, hd, 1.0 / outdeg[hs])
+    want = 0.15 + D * acc[rows]
+    err = np.abs(got[rows] - want)
+    print(f"\nR-MAT 27: N {n}, E {e}, max in-degree {indeg.max()}, {plan.info()['regular_blocks']} tiles; "
+          f"sampled rows max rel {float((err / want).max()):.2e}")
+    assert (err <= 2e-5 * want).all()
+    del plan, g, s, d, x, r
     torch.cuda.empty_cache()

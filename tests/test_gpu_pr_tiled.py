@@ -128,6 +128,36 @@ def test_tiled_rows_across_lanes_and_tiles(dev):
         assert plan.info()["hub_rows"] > 30  # split rows: the long ones and rows at tile boundaries
 
 
+def test_tiled_split_rows_inside_and_across_wavefront_ranges(dev):
+    """More than 8,192 tiles, so each of the 4,096 sweep wavefronts takes a
+    range of at least 3 tiles: rows crossing tiles inside a range are summed
+    and applied by their wavefront, rows crossing a range boundary by the
+    finalize (at most one per boundary).  vs the FP32 oracle and the FP64
+    statistics, and the captured exec bit for bit on replays."""
+    rng = np.random.default_rng(11)
+    n = 250_000
+    deg = rng.integers(0, 30, n)
+    deg[rng.choice(n, 300, replace=False)] = rng.integers(600, 12_000, 300)
+    dst = np.repeat(rng.permutation(n), deg).astype(np.uint32)
+    src = rng.integers(0, n, len(dst)).astype(np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    g = dev.DeviceGraph(dev.to_dev_u32(src), dev.to_dev_u32(dst), n)
+    in_off, in_src, outdeg, _ = oracle.csc(n, src, dst)
+    want, ostats = oracle.pagerank_f32(n, in_off, in_src, outdeg, 0.15, DAMPING, 1.0, 6)
+    for tune in FORMS:
+        r, stats, plan, ex = run(dev, g, tune, 0.15, 6)
+        info = plan.info()
+        assert info["regular_blocks"] > 2 * 4096 and info["persistent_workgroups"] == 256
+        assert 0 < info["hub_rows"] < 4096  # only rows across a range boundary go to the finalize
+        assert_close(r, want)
+        np.testing.assert_allclose(stats[:, 1], ostats[:, 1], rtol=2e-6)
+        np.testing.assert_allclose(stats[:, 0], ostats[:, 0], rtol=1e-3, atol=1e-6 * ostats[0, 0])
+        for _ in range(2):
+            ex.launch()
+            np.testing.assert_array_equal(ex.ranks().cpu().numpy(), r)
+
+
 def test_tiled_personalized_matches_vertex_layout(dev):
     """A seed-set operator (personalized form) through the tiled sweep equals
     the workgroup-tiled form within tolerance: the seed test maps ranks back
