@@ -117,6 +117,8 @@ _SIGS = {
     "gchi_cc_labelprop_sweep": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_scheduled_workspace": (c_i32, [c_u32, c_u64, ctypes.POINTER(c_sz)]),
     "gchi_cc_afforest_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_psw_graph_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
+    "gchi_psw_graph_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "gchi_cc_pull_sweep": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_afforest": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_sz, c_vp]),
     "gchi_shard_decode_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
@@ -557,6 +559,22 @@ def cc_pull(off, adj, n, max_sweeps=100000):
         if int(ch.item()) == 0:
             break
     return a, sweeps
+
+
+def psw_graph_build(src, dst, n):
+    """Adjacency of the generic update() engine: (out_off, out_dst, in_off,
+    in_src, in_eidx, lower) device tensors (gchi_psw_graph_build)."""
+    L = lib()
+    e = int(src.numel())
+    out_off, in_off, lower = empty_u32(n + 1), empty_u32(n + 1), empty_u32(n)
+    out_dst, in_src, in_eidx = empty_u32(max(e, 1)), empty_u32(max(e, 1)), empty_u32(max(e, 1))
+    nb = c_sz()
+    check(L.gchi_psw_graph_workspace(e, n, ctypes.byref(nb)))
+    ws = workspace(nb.value)
+    check(L.gchi_psw_graph_build(dptr(src), dptr(dst), e, n, dptr(out_off), dptr(out_dst), dptr(in_off), dptr(in_src),
+                                 dptr(in_eidx), dptr(lower), dptr(ws), nb.value, stream()), "psw_graph_build")
+    work_status(ws, "psw_graph_build")
+    return out_off, out_dst[:e], in_off, in_src[:e], in_eidx[:e], lower
 
 
 def cc_afforest(off, adj, n):

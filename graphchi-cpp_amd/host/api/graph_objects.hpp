@@ -1,43 +1,52 @@
 // Vertex and edge objects of the program API (src/api/graph_objects.hpp:55-376).
-// They exist so that update() functions compile unchanged; the device engines
-// run registered programs without constructing them per vertex.
+//
+// A vertex is a view: its id, a pointer to its value, and its in- and
+// out-edges as arrays of edge objects (each an adjacent vertex id and a pointer
+// to the edge's value) owned by whoever built the view.  The generic device
+// engine (engine/psw_engine.hpp) builds these views in HBM, so every member a
+// program's update() calls is callable on the device when the program is
+// compiled with hipcc; with a host compiler they are ordinary inline members.
 #pragma once
-#include <algorithm>
-#include <vector>
-
 #include "graphchi_types.hpp"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GCHI_HD __host__ __device__
+#else
+#define GCHI_HD
+#endif
 
 namespace graphchi {
 
 // Deleted edges carry a marker value in their edge data (the convention of
 // src/api/graph_objects.hpp:96-140 under SUPPORT_DELETIONS): true for bool,
 // all ones for 32-bit integers, zero for float.
-inline bool is_deleted_edge_value(bool v) { return v; }
-inline bool is_deleted_edge_value(int v) { return (unsigned int)v == 0xffffffffu; }
-inline bool is_deleted_edge_value(unsigned int v) { return v == 0xffffffffu; }
-inline bool is_deleted_edge_value(float v) { return !(v < 0 || v > 0); }
+GCHI_HD inline bool is_deleted_edge_value(bool v) { return v; }
+GCHI_HD inline bool is_deleted_edge_value(int v) { return (unsigned int)v == 0xffffffffu; }
+GCHI_HD inline bool is_deleted_edge_value(unsigned int v) { return v == 0xffffffffu; }
+GCHI_HD inline bool is_deleted_edge_value(float v) { return !(v < 0 || v > 0); }
 
 template <typename T>
-inline T deleted_edge_value();
+GCHI_HD inline T deleted_edge_value();
 template <>
-inline bool deleted_edge_value<bool>() { return true; }
+GCHI_HD inline bool deleted_edge_value<bool>() { return true; }
 template <>
-inline int deleted_edge_value<int>() { return -1; }
+GCHI_HD inline int deleted_edge_value<int>() { return -1; }
 template <>
-inline unsigned int deleted_edge_value<unsigned int>() { return 0xffffffffu; }
+GCHI_HD inline unsigned int deleted_edge_value<unsigned int>() { return 0xffffffffu; }
 template <>
-inline float deleted_edge_value<float>() { return 0.0f; }
+GCHI_HD inline float deleted_edge_value<float>() { return 0.0f; }
 
 template <typename EdgeDataType>
 class graphchi_edge {
   public:
     vid_t vertexid = 0;
     EdgeDataType *data_ptr = nullptr;
-    graphchi_edge() {}
-    graphchi_edge(vid_t v, EdgeDataType *p) : vertexid(v), data_ptr(p) {}
-    EdgeDataType get_data() { return *data_ptr; }
-    void set_data(EdgeDataType x) { *data_ptr = x; }
-    vid_t vertex_id() const { return vertexid; }
+    GCHI_HD graphchi_edge() {}
+    GCHI_HD graphchi_edge(vid_t v, EdgeDataType *p) : vertexid(v), data_ptr(p) {}
+    GCHI_HD EdgeDataType get_data() { return *data_ptr; }
+    GCHI_HD void set_data(EdgeDataType x) { *data_ptr = x; }
+    GCHI_HD vid_t vertex_id() const { return vertexid; }
 };
 
 template <typename VertexDataType, typename EdgeDataType>
@@ -48,40 +57,67 @@ class graphchi_vertex {
     VertexDataType *dataptr = nullptr;
     bool scheduled = true;
     bool parallel_safe = true;
-    std::vector<graphchi_edge<EdgeDataType>> inc, outc;
-    VertexDataType local = VertexDataType();
+    graphchi_edge<EdgeDataType> *inc = nullptr, *outc = nullptr;  // edge arrays of the view
+    int nin = 0, nout = 0;
+    VertexDataType local = VertexDataType();  // the value when the view has no storage for it
 
-    graphchi_vertex() {}
-    graphchi_vertex(vid_t id, graphchi_edge<EdgeDataType> *, graphchi_edge<EdgeDataType> *, int, int)
-        : vertexid(id) {}
-    vid_t id() const { return vertexid; }
-    int num_inedges() const { return (int)inc.size(); }
-    int num_outedges() const { return (int)outc.size(); }
-    int num_edges() const { return (int)(inc.size() + outc.size()); }
-    graphchi_edge<EdgeDataType> *inedge(int i) { return &inc[i]; }
-    graphchi_edge<EdgeDataType> *outedge(int i) { return &outc[i]; }
-    graphchi_edge<EdgeDataType> *edge(int i) {
-        return i < (int)inc.size() ? &inc[i] : &outc[i - inc.size()];
-    }
-    graphchi_edge<EdgeDataType> *random_outedge() { return outc.empty() ? nullptr : &outc[0]; }
-    VertexDataType get_data() { return dataptr ? *dataptr : local; }
-    void set_data(VertexDataType d) {
+    GCHI_HD graphchi_vertex() {}
+    // The reference's constructor: id, in-edge and out-edge storage, counts.
+    GCHI_HD graphchi_vertex(vid_t id, graphchi_edge<EdgeDataType> *iptr, graphchi_edge<EdgeDataType> *optr, int indeg,
+                            int outdeg)
+        : vertexid(id), inc(iptr), outc(optr), nin(indeg), nout(outdeg) {}
+    GCHI_HD vid_t id() const { return vertexid; }
+    GCHI_HD int num_inedges() const { return nin; }
+    GCHI_HD int num_outedges() const { return nout; }
+    GCHI_HD int num_edges() const { return nin + nout; }
+    GCHI_HD graphchi_edge<EdgeDataType> *inedge(int i) { return &inc[i]; }
+    GCHI_HD graphchi_edge<EdgeDataType> *outedge(int i) { return &outc[i]; }
+    // In-edges first, then out-edges (graph_objects.hpp:283-286).
+    GCHI_HD graphchi_edge<EdgeDataType> *edge(int i) { return i < nin ? &inc[i] : &outc[i - nin]; }
+    GCHI_HD graphchi_edge<EdgeDataType> *random_outedge() { return nout ? &outc[0] : nullptr; }
+    GCHI_HD VertexDataType get_data() { return dataptr ? *dataptr : local; }
+    GCHI_HD void set_data(VertexDataType d) {
         if (dataptr) *dataptr = d;
         else local = d;
     }
-    void add_inedge(vid_t src, EdgeDataType *ptr, bool) { inc.push_back(graphchi_edge<EdgeDataType>(src, ptr)); }
-    void add_outedge(vid_t dst, EdgeDataType *ptr, bool) { outc.push_back(graphchi_edge<EdgeDataType>(dst, ptr)); }
-    void sort_edges_indirect() {
-        auto by_id = [](const graphchi_edge<EdgeDataType> &a, const graphchi_edge<EdgeDataType> &b) {
-            return a.vertexid < b.vertexid;
-        };
-        std::sort(inc.begin(), inc.end(), by_id);
-        std::sort(outc.begin(), outc.end(), by_id);
+    // Appends to the view's edge storage, which the caller sized.
+    GCHI_HD void add_inedge(vid_t src, EdgeDataType *ptr, bool) { inc[nin++] = graphchi_edge<EdgeDataType>(src, ptr); }
+    GCHI_HD void add_outedge(vid_t dst, EdgeDataType *ptr, bool) {
+        outc[nout++] = graphchi_edge<EdgeDataType>(dst, ptr);
     }
-    // Marks edge i deleted (its data takes the deletion marker).
-    void remove_edge(int i) { edge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
-    bool computational_edges() { return false; }
-    static bool read_outedges() { return true; }
+    // Orders the in- and out-edges by adjacent vertex id (stable).
+    GCHI_HD void sort_edges_indirect() {
+        sort_by_id(inc, nin);
+        sort_by_id(outc, nout);
+    }
+    // Marks edges deleted (their data takes the deletion marker).
+    GCHI_HD void remove_edge(int i) { edge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
+    GCHI_HD void remove_inedge(int i) { inedge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
+    GCHI_HD void remove_outedge(int i) { outedge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
+    GCHI_HD void remove_alledges() {
+        for (int i = 0; i < num_edges(); i++) remove_edge(i);
+    }
+    GCHI_HD void remove_alloutedges() {
+        for (int i = 0; i < nout; i++) remove_outedge(i);
+    }
+    GCHI_HD void remove_allinedges() {
+        for (int i = 0; i < nin; i++) remove_inedge(i);
+    }
+    GCHI_HD bool computational_edges() { return false; }
+    GCHI_HD static bool read_outedges() { return true; }
+
+  private:
+    GCHI_HD static void sort_by_id(graphchi_edge<EdgeDataType> *e, int n) {
+        for (int i = 1; i < n; i++) {
+            graphchi_edge<EdgeDataType> x = e[i];
+            int j = i - 1;
+            while (j >= 0 && e[j].vertexid > x.vertexid) {
+                e[j + 1] = e[j];
+                j--;
+            }
+            e[j + 1] = x;
+        }
+    }
 };
 
 }  // namespace graphchi

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "api/graph_objects.hpp"
 #include "graphchi_types.hpp"
 
 namespace graphchi {
@@ -18,6 +19,9 @@ class ischeduler {
     virtual bool is_scheduled(vid_t) { return true; }
 };
 
+// The device engine copies the context into HBM for every pass and reads only
+// its plain fields there; log_change() adds to device_delta when it is called
+// on the device.
 struct graphchi_context {
     size_t nvertices = 0;
     size_t nedges = 0;
@@ -26,11 +30,18 @@ struct graphchi_context {
     int last_iteration = -1;
     int execthreads = 1;
     ischeduler *scheduler = nullptr;
+    double *device_delta = nullptr;
     std::vector<double> deltas = std::vector<double>(1, 0.0);
 
-    void set_last_iteration(int it) { last_iteration = it; }
-    bool isfinal() const { return last_iteration >= 0 && iteration >= last_iteration; }
-    void log_change(double delta) { deltas[0] += delta; }
+    GCHI_HD void set_last_iteration(int it) { last_iteration = it; }
+    GCHI_HD bool isfinal() const { return last_iteration >= 0 && iteration >= last_iteration; }
+    GCHI_HD void log_change(double delta) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (device_delta) atomicAdd(device_delta, delta);
+#else
+        deltas[0] += delta;
+#endif
+    }
     double get_delta() const { return deltas[0]; }
     void reset_deltas(int) { deltas.assign(1, 0.0); }
 };

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "api/chifilenames.hpp"
+#include "api/graph_objects.hpp"
 #include "logger/logger.hpp"
 
 namespace graphchi {
@@ -13,8 +14,10 @@ namespace graphchi {
 template <typename VertexDataType>
 class VCallback {
   public:
-    virtual ~VCallback() {}
-    virtual void callback(vid_t vertex_id, VertexDataType &value) = 0;
+    // host-and-device with hipcc, so that callbacks defined in an application
+    // compiled with it still override these
+    GCHI_HD virtual ~VCallback() {}
+    GCHI_HD virtual void callback(vid_t vertex_id, VertexDataType &value) = 0;
 };
 
 template <typename VertexDataType>

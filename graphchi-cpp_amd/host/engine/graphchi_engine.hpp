@@ -3,13 +3,16 @@
 // device.
 //
 // The reference runs any GraphChiProgram through the parallel-sliding-windows
-// loop on the CPU.  Here a program runs on the device when a device
-// implementation is registered for its type:
-//     GCHI_DEVICE_PROGRAM(MyProgram, graphchi::device::min_label_components);
-// The programs of the reference's connectedcomponents and trianglecounting apps
-// (::ConnectedComponentsProgram, ::TriangleCountingProgram) are registered
-// here by name, so those sources compile and run unchanged.  A program without
-// a registration stops with a fatal log: there is no CPU fallback.
+// loop on the CPU.  Here every program runs on the device:
+//   * a program with a registered device algorithm runs that algorithm:
+//         GCHI_DEVICE_PROGRAM(MyProgram, graphchi::device::min_label_components);
+//     the programs of the reference's connectedcomponents and trianglecounting
+//     apps (::ConnectedComponentsProgram, ::TriangleCountingProgram) are
+//     registered here by name, so those sources compile and run unchanged;
+//   * any other program, in an application compiled with hipcc, runs its own
+//     update() on the device in the reference's order (engine/psw_engine.hpp);
+//   * compiled with a host compiler, an unregistered program stops with a
+//     fatal log: there is no CPU fallback.
 //
 // The graph is read from its shard files once (the adjacency decode of
 // memory_shard), uploaded to HBM, and the whole computation runs there; the
@@ -31,6 +34,7 @@
 #include "engine/device.hpp"
 #include "logger/logger.hpp"
 #include "metrics/metrics.hpp"
+#include "engine/psw_engine.hpp"
 #include "shards/shard_reader.hpp"
 #include "util/cmdopts.hpp"
 
@@ -111,9 +115,17 @@ class graphchi_engine {
     }
 
     template <class ProgramT>
-    void run_dispatch(ProgramT &, std::false_type) {
+    void run_dispatch(ProgramT &program, std::false_type) {
+#if defined(__HIPCC__)
+        ctx_.last_iteration = -1;
+        psw::run_program<VertexDataType, EdgeDataType>(program, base_, nshards_, (uint32_t)nvertices_,
+                                                       ctx_.num_iterations, ctx_, m_, nedges_);
+#else
+        (void)program;
         logstream(LOG_FATAL) << "no device implementation is registered for this program "
-                             << "(GCHI_DEVICE_PROGRAM); this engine has no CPU path" << std::endl;
+                             << "(GCHI_DEVICE_PROGRAM), and its update() runs on the device only in an "
+                             << "application compiled with hipcc; this engine has no CPU path" << std::endl;
+#endif
     }
 
     template <class ProgramT>
@@ -169,8 +181,8 @@ class graphchi_engine {
         pc.scheduler = &probe_sched;
         pc.nvertices = 10;
         uint32_t in_val = 7, out_val = 8, label = 0;
-        graphchi_vertex<VertexDataType, EdgeDataType> v;
-        v.vertexid = 5;
+        graphchi_edge<EdgeDataType> ein[1], eout[1];
+        graphchi_vertex<VertexDataType, EdgeDataType> v(5, ein, eout, 0, 0);
         v.dataptr = reinterpret_cast<VertexDataType *>(&label);
         v.add_inedge(3, reinterpret_cast<EdgeDataType *>(&in_val), false);
         v.add_outedge(9, reinterpret_cast<EdgeDataType *>(&out_val), false);

@@ -157,6 +157,22 @@ int gchi_csc_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges
                    uint32_t *d_in_off, uint32_t *d_in_src, uint32_t *d_outdeg, uint32_t *d_indeg,
                    void *d_work, size_t work_bytes, void *stream);
 
+/* ------------------------------------------------- generic vertex programs */
+/* Adjacency for running a program's own update() on the device in the
+ * reference's PSW order (graphchi_engine::exec_updates,
+ * src/engine/graphchi_engine.hpp:396-454; the order basic_smoketest.cpp:65
+ * checks: a vertex sees what lower ids wrote this iteration).  From an edge
+ * list in HBM: out-edges in stable (src, dst) order (d_out_off[N+1],
+ * d_out_dst[E]; edge values are stored in this order), in-edges in stable
+ * (dst, src) order (d_in_off[N+1], d_in_src[E]) with d_in_eidx[E] = the
+ * index of each in-edge's value in out-edge order, and d_lower[N] = the number
+ * of incident edges whose other end has a smaller id.  nedges <= 2^32 - 2^25.
+ * Status word in the workspace (gchi_work_status): a bad id or a sort wait. */
+int gchi_psw_graph_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes);
+int gchi_psw_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                         uint32_t *d_out_off, uint32_t *d_out_dst, uint32_t *d_in_off, uint32_t *d_in_src,
+                         uint32_t *d_in_eidx, uint32_t *d_lower, void *d_work, size_t work_bytes, void *stream);
+
 /* ------------------------------------------------------------------ PageRank */
 /* Affine functional kernel of example_apps/pagerank_functional.cpp:48-83:
  *   value_to_neighbor : x_u = r_u / outdeg_u            (FP32, :78-81)

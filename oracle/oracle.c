@@ -30,6 +30,11 @@
  *   orc_triangles          - golden totals/per-vertex .vout of the reference
  *                            trianglecounting; brute force on tiny graphs; K_n
  *   orc_pr_psw_omp         - equals orc_pagerank_sync_f32 within FP32 tolerance
+ *   orc_psw_run            - the reference engine's own vertex values for the
+ *                            probe program (oracle/Makefile builds it against the
+ *                            reference headers; golden fixtures, 1-3 shards);
+ *                            zero violations of basic_smoketest.cpp:65's
+ *                            invariant, and violations when run in reverse
  */
 #include "oracle.h"
 
@@ -663,4 +668,90 @@ int64_t orc_shard_decode(uint64_t nbytes, const uint8_t *adj, uint32_t *src, uin
         vid++;
     }
     return (int64_t)ne;
+}
+
+/* ---- orc_psw_run: updates one after another in id order ---- */
+static uint32_t psw_mix(uint32_t a, uint32_t b) {
+    uint32_t h = (a * 0x9E3779B1u) ^ (b + 0x7F4A7C15u + (a << 6) + (a >> 2));
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h;
+}
+
+long orc_psw_run(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int program, int niters,
+                 int reverse, uint32_t *vdata_out) {
+    if (program != 0 && program != 1) return -1;
+    /* incidence lists by edge index (the programs here fold them order-free) */
+    uint64_t *ioff = calloc((size_t)n + 1, 8), *ooff = calloc((size_t)n + 1, 8);
+    uint64_t *iedge = malloc((e ? e : 1) * 8), *oedge = malloc((e ? e : 1) * 8);
+    uint32_t *ev = calloc(e ? e : 1, 4), *vv = calloc(n ? n : 1, 4);
+    if (!ioff || !ooff || !iedge || !oedge || !ev || !vv) return -2;
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) {
+            ioff[dst[k] + 1]++;
+            ooff[src[k] + 1]++;
+        }
+    for (uint32_t v = 0; v < n; v++) {
+        ioff[v + 1] += ioff[v];
+        ooff[v + 1] += ooff[v];
+    }
+    uint64_t *ic = malloc(((size_t)n + 1) * 8), *oc = malloc(((size_t)n + 1) * 8);
+    memcpy(ic, ioff, ((size_t)n + 1) * 8);
+    memcpy(oc, ooff, ((size_t)n + 1) * 8);
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) {
+            iedge[ic[dst[k]]++] = k;
+            oedge[oc[src[k]]++] = k;
+        }
+    long violations = 0;
+    for (int it = 0; it < niters; it++) {
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t v = reverse ? n - 1 - j : j;
+            if (program == 0) { /* basic_smoketest.cpp:47-72 */
+                if (it == 0) {
+                    for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = v;
+                } else {
+                    for (uint64_t q = ioff[v]; q < ioff[v + 1]; q++) {
+                        const uint32_t u = src[iedge[q]];
+                        const uint32_t expected = u + (uint32_t)it - (u > v ? 1u : 0u);
+                        if (ev[iedge[q]] != expected) violations++;
+                    }
+                    for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = v + (uint32_t)it;
+                }
+                vv[v] = (uint32_t)it + 1u;
+                continue;
+            }
+            /* psw_probe.cpp */
+            if (it == 0) {
+                for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = psw_mix(v, dst[oedge[q]]);
+                vv[v] = v;
+                continue;
+            }
+            uint32_t s_in = 0, x_in = 0, s_out = 0, x_out = 0;
+            for (uint64_t q = ioff[v]; q < ioff[v + 1]; q++) {
+                s_in += ev[iedge[q]];
+                x_in ^= psw_mix(ev[iedge[q]], 1u);
+            }
+            for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) {
+                s_out += ev[oedge[q]];
+                x_out ^= psw_mix(ev[oedge[q]], 2u);
+            }
+            const uint32_t acc = psw_mix(psw_mix(vv[v], s_in ^ x_out), x_in + s_out);
+            for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = psw_mix(acc, dst[oedge[q]]);
+            for (uint64_t q = ioff[v]; q < ioff[v + 1]; q++)
+                if (src[iedge[q]] > v) ev[iedge[q]] = psw_mix(acc ^ 0x5bd1e995u, src[iedge[q]]);
+            vv[v] = acc;
+        }
+    }
+    if (vdata_out) memcpy(vdata_out, vv, (size_t)n * 4);
+    free(ioff);
+    free(ooff);
+    free(iedge);
+    free(oedge);
+    free(ic);
+    free(oc);
+    free(ev);
+    free(vv);
+    return violations;
 }

@@ -99,6 +99,18 @@ int orc_semisync_blocks(uint32_t n, uint32_t nintervals, const uint32_t *interva
 double orc_pr_psw_omp(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int nshards,
                       double teleport, double damping, float init, int nsweeps, int nthreads, float *r_out);
 
+/* The reference engine's update order (graphchi_engine::exec_updates,
+ * src/engine/graphchi_engine.hpp:396-454): one iteration runs update() on
+ * vertices 0..n-1 one after another; every edge holds one value, seen from both
+ * ends.  `program` 0 is src/tests/basic_smoketest.cpp's SmokeTestProgram: it
+ * returns the number of in-edge reads that broke the test's invariant (in-edge
+ * u->v holds u + iteration - (u > v)) and writes the vertex values; `program` 1
+ * is graphchi-cpp_amd/apps/psw_probe.cpp's PswProbeProgram (returns 0).
+ * `reverse` runs the vertices n-1..0 (a planted ordering mistake for tests).
+ * Self-loops are skipped, as the reference's preprocessing drops them. */
+long orc_psw_run(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int program, int niters,
+                 int reverse, uint32_t *vdata_out);
+
 #ifdef __cplusplus
 }
 #endif

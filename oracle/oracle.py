@@ -41,6 +41,7 @@ def lib():
             "orc_degree_order": (i32, [u32, vp, vp]),
             "orc_triangles": (i32, [u32, u64, vp, vp, vp, vp, vp]),
             "orc_pr_psw_omp": (dbl, [u32, u64, vp, vp, i32, dbl, dbl, f32, i32, i32, vp]),
+            "orc_psw_run": (ctypes.c_long, [u32, u64, vp, vp, i32, i32, i32, vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(_lib, k)
@@ -235,3 +236,17 @@ def parse_edgelist(text):
             src.append(a)
             dst.append(b)
     return np.array(src, np.uint32), np.array(dst, np.uint32)
+
+
+PSW_SMOKETEST, PSW_PROBE = 0, 1
+
+
+def psw_run(n, src, dst, program, niters, reverse=False):
+    """The reference engine's update order, one vertex after another in id
+    order (orc_psw_run): (vertex values, invariant violations of the smoke
+    test's program)."""
+    src, dst = u32(src), u32(dst)
+    vd = np.zeros(max(n, 1), np.uint32)
+    viol = lib().orc_psw_run(n, len(src), _p(src), _p(dst), program, niters, 1 if reverse else 0, _p(vd))
+    assert viol >= 0
+    return vd[:n], int(viol)

@@ -13,6 +13,8 @@ gchi_rmat_generate_host), and stores what the reference wrote:
   connectedcomponents                                 -> labels (.4B.vout)
   trianglecounting nshards 2                          -> total, per-vertex counts
                                                         (_degord.4B.vout), .vertexmap
+  oracle/_ref/psw_probe (graphchi-cpp_amd/apps/psw_probe.cpp
+  built against the reference's engine)               -> vertex values (.4B.vout)
 
 Inputs are not stored: tests regenerate them and check their SHA-256 here.
 Usage: python tests/golden/make_golden.py [--only NAME]
@@ -228,17 +230,34 @@ def tc_cases(rr, manifest, arrays, only):
         print("tc", name, n, len(src), total, flush=True)
 
 
+def psw_cases(rr, manifest, arrays, only):
+    """The package's order-probe program run by the reference's engine."""
+    for case in graphs.PSW_CASES:
+        name = case["name"]
+        if only and name != only:
+            continue
+        src, dst, n = graphs.make(case)
+        path = rr.graph_file(name, src, dst)
+        rr.run("psw_probe", ["file", path, "filetype", "edgelist", "niters", case["niters"], "nshards",
+                             case["nshards"]])
+        arrays[f"{name}__psw_probe"] = read_u32(path + ".4B.vout")
+        manifest[name] = {"kind": "psw", "case": case, "n": n, "e": int(len(src)),
+                          "input_sha": graphs.input_sha(src, dst)}
+        print("psw", name, n, len(src), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=None)
     ap.add_argument("--semisync-only", action="store_true", help="add the semisync goldens to the existing files")
+    ap.add_argument("--psw-only", action="store_true", help="add the psw_probe goldens to the existing files")
     args = ap.parse_args()
     if not os.path.exists(os.path.join(REF, "pagerank_functional")):
         sys.exit("reference binaries missing: run `make -C oracle ref` first")
     mpath = os.path.join(OUT, "manifest.json")
     apath = os.path.join(OUT, "golden.npz")
     manifest, arrays = {}, {}
-    if (args.only or args.semisync_only) and os.path.exists(mpath):
+    if (args.only or args.semisync_only or args.psw_only) and os.path.exists(mpath):
         manifest = json.load(open(mpath))
         arrays = dict(np.load(apath))
     rr = RefRunner()
@@ -246,11 +265,15 @@ def main():
         if args.semisync_only:
             semisync_cases(rr, manifest, arrays, args.only)
             raise StopIteration
+        if args.psw_only:
+            psw_cases(rr, manifest, arrays, args.only)
+            raise StopIteration
         pagerank_cases(rr, manifest, arrays, args.only)
         semisync_cases(rr, manifest, arrays, args.only)
         sharder_cases(rr, manifest, arrays, args.only)
         cc_cases(rr, manifest, arrays, args.only)
         tc_cases(rr, manifest, arrays, args.only)
+        psw_cases(rr, manifest, arrays, args.only)
     except StopIteration:
         pass
     finally:

@@ -56,6 +56,19 @@ TC_CASES = [
 ]
 
 
+# Programs run by the generic device engine (apps/psw_probe.cpp): duplicates in
+# both directions, id gaps, a two-way path (each vertex waits for the one
+# before it: the longest chain of updates), R-MAT graphs over several shards.
+PSW_CASES = [
+    {"name": "psw_misc", "hand": [(0, 1), (0, 1), (1, 0), (1, 2), (2, 5), (3, 1), (5, 2), (7, 5), (7, 1), (4, 7), (6, 3),
+                                  (9, 0)], "niters": 4, "nshards": 1},
+    {"name": "psw_path", "hand": [(i, i + 1) for i in range(199)] + [(i + 1, i) for i in range(199)], "niters": 3,
+     "nshards": 2},
+    {"name": "psw_rmat12_p3", "rmat": (12, 8, 7), "niters": 4, "nshards": 3},
+    {"name": "psw_rmat14_p2", "rmat": (14, 16, 8), "niters": 3, "nshards": 2},
+]
+
+
 def rmat_pairs(scale, edge_factor, seed, undirected=False):
     """Raw R-MAT pairs of a workload: edge_factor * 2^scale directed records
     (halved before mirroring for undirected workloads)."""
@@ -98,4 +111,4 @@ def input_sha(src, dst):
 
 
 def all_cases():
-    return PR_CASES + SHARD_CASES + CC_CASES + TC_CASES
+    return PR_CASES + SHARD_CASES + CC_CASES + TC_CASES + PSW_CASES
