@@ -231,11 +231,15 @@ void run_functional_device(std::string filename, int niters, metrics &m, bool se
     logstream(LOG_INFO) << "device operator: r = " << f.teleport << (f.seeds.empty() ? "" : " (+seed term)") << " + "
                         << f.damping << " * sum(r/outdeg), init " << f.init << (semisync ? ", semi-synchronous" : "")
                         << std::endl;
-    // launch shape: the library's default, or the autotuner's pick (autotune 1)
-    gchi_pr_tune tune{0, 0, 0, 0};
+    // launch shape: the rank-space tiled sweep with the largest LDS cache (the
+    // fastest shape on the benchmark graph), or the autotuner's pick (autotune
+    // 1); the semi-synchronous block sweeps take every shape but that one
+    gchi_pr_tune tune{512, 0, 4, 1 << 20};
+    if (semisync) tune = gchi_pr_tune{0, 0, 0, 0};
     if (get_option_int("autotune", 0)) {
         double ms = 0;
         device::check(gchi_autotune_pr(&g.dg, 3, &tune, &ms, nullptr, nullptr, 0, nullptr, nullptr), "autotune");
+        if (semisync && tune.persistent == 4) tune = gchi_pr_tune{0, 0, 0, 0};
     }
     device::buffer<uint32_t> seed_bits;
     gchi_pr_op op{f.teleport, f.damping, nullptr, nullptr, 0.0};
