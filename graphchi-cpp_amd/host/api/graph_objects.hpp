@@ -57,38 +57,40 @@ class graphchi_vertex {
     VertexDataType *dataptr = nullptr;
     bool scheduled = true;
     bool parallel_safe = true;
-    graphchi_edge<EdgeDataType> *inc = nullptr, *outc = nullptr;  // edge arrays of the view
-    int nin = 0, nout = 0;
+    int inc = 0, outc = 0;  // in- and out-edge counts (the reference's public members)
+    graphchi_edge<EdgeDataType> *inedges_ptr = nullptr, *outedges_ptr = nullptr;  // the view's edge arrays
     VertexDataType local = VertexDataType();  // the value when the view has no storage for it
 
     GCHI_HD graphchi_vertex() {}
     // The reference's constructor: id, in-edge and out-edge storage, counts.
     GCHI_HD graphchi_vertex(vid_t id, graphchi_edge<EdgeDataType> *iptr, graphchi_edge<EdgeDataType> *optr, int indeg,
                             int outdeg)
-        : vertexid(id), inc(iptr), outc(optr), nin(indeg), nout(outdeg) {}
+        : vertexid(id), inc(indeg), outc(outdeg), inedges_ptr(iptr), outedges_ptr(optr) {}
     GCHI_HD vid_t id() const { return vertexid; }
-    GCHI_HD int num_inedges() const { return nin; }
-    GCHI_HD int num_outedges() const { return nout; }
-    GCHI_HD int num_edges() const { return nin + nout; }
-    GCHI_HD graphchi_edge<EdgeDataType> *inedge(int i) { return &inc[i]; }
-    GCHI_HD graphchi_edge<EdgeDataType> *outedge(int i) { return &outc[i]; }
+    GCHI_HD int num_inedges() const { return inc; }
+    GCHI_HD int num_outedges() const { return outc; }
+    GCHI_HD int num_edges() const { return inc + outc; }
+    GCHI_HD graphchi_edge<EdgeDataType> *inedge(int i) { return &inedges_ptr[i]; }
+    GCHI_HD graphchi_edge<EdgeDataType> *outedge(int i) { return &outedges_ptr[i]; }
     // In-edges first, then out-edges (graph_objects.hpp:283-286).
-    GCHI_HD graphchi_edge<EdgeDataType> *edge(int i) { return i < nin ? &inc[i] : &outc[i - nin]; }
-    GCHI_HD graphchi_edge<EdgeDataType> *random_outedge() { return nout ? &outc[0] : nullptr; }
+    GCHI_HD graphchi_edge<EdgeDataType> *edge(int i) { return i < inc ? &inedges_ptr[i] : &outedges_ptr[i - inc]; }
+    GCHI_HD graphchi_edge<EdgeDataType> *random_outedge() { return outc ? &outedges_ptr[0] : nullptr; }
     GCHI_HD VertexDataType get_data() { return dataptr ? *dataptr : local; }
     GCHI_HD void set_data(VertexDataType d) {
         if (dataptr) *dataptr = d;
         else local = d;
     }
     // Appends to the view's edge storage, which the caller sized.
-    GCHI_HD void add_inedge(vid_t src, EdgeDataType *ptr, bool) { inc[nin++] = graphchi_edge<EdgeDataType>(src, ptr); }
+    GCHI_HD void add_inedge(vid_t src, EdgeDataType *ptr, bool) {
+        inedges_ptr[inc++] = graphchi_edge<EdgeDataType>(src, ptr);
+    }
     GCHI_HD void add_outedge(vid_t dst, EdgeDataType *ptr, bool) {
-        outc[nout++] = graphchi_edge<EdgeDataType>(dst, ptr);
+        outedges_ptr[outc++] = graphchi_edge<EdgeDataType>(dst, ptr);
     }
     // Orders the in- and out-edges by adjacent vertex id (stable).
     GCHI_HD void sort_edges_indirect() {
-        sort_by_id(inc, nin);
-        sort_by_id(outc, nout);
+        sort_by_id(inedges_ptr, inc);
+        sort_by_id(outedges_ptr, outc);
     }
     // Marks edges deleted (their data takes the deletion marker).
     GCHI_HD void remove_edge(int i) { edge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
@@ -98,10 +100,10 @@ class graphchi_vertex {
         for (int i = 0; i < num_edges(); i++) remove_edge(i);
     }
     GCHI_HD void remove_alloutedges() {
-        for (int i = 0; i < nout; i++) remove_outedge(i);
+        for (int i = 0; i < outc; i++) remove_outedge(i);
     }
     GCHI_HD void remove_allinedges() {
-        for (int i = 0; i < nin; i++) remove_inedge(i);
+        for (int i = 0; i < inc; i++) remove_inedge(i);
     }
     GCHI_HD bool computational_edges() { return false; }
     GCHI_HD static bool read_outedges() { return true; }

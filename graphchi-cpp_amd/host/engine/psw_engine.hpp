@@ -37,7 +37,10 @@
 // repeat_updates(), dynamic edge data, and the reference's writing of edge
 // values back into the shard files.  The iteration hooks run on the host
 // between passes; before_exec_interval / after_exec_interval are called once
-// per iteration with the whole vertex range.
+// per iteration with the whole vertex range.  log_change() on the device adds
+// to one FP64 accumulator per pass (its order of additions is not fixed);
+// <base>.deltalog and set_last_iteration() follow the reference, including its
+// in-memory mode for graphs of one shard (below).
 #pragma once
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -196,6 +199,15 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
     }
 }
 
+// <base>.deltalog: iteration, updates, work, change (graphchi_engine.hpp:593-599)
+inline void write_delta_log(const std::string &base, const graphchi_context &ctx, int iteration, uint64_t updates,
+                            uint64_t work) {
+    if (FILE *df = fopen((base + ".deltalog").c_str(), iteration == 0 ? "w" : "a")) {
+        fprintf(df, "%d,%lu,%lu,%lf\n", iteration, (unsigned long)updates, (unsigned long)work, ctx.get_delta());
+        fclose(df);
+    }
+}
+
 inline void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) logstream(LOG_FATAL) << what << ": " << hipGetErrorString(e) << std::endl;
 }
@@ -259,12 +271,23 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         ncu = prop.multiProcessorCount;
     const unsigned grid = grid_for(n, (unsigned)ncu * 8);
-    uint64_t updates = 0;
+    // The reference runs a graph of one shard (and fewer than 2 * 40M vertices)
+    // in its in-memory mode (graphchi_engine.hpp:296-301, 465-523): all
+    // iterations inside one outer iteration, so log_change() accumulates over
+    // the whole run, the exec-interval hooks get (0, N), a last iteration ends
+    // the run only when it is above 0, and <base>.deltalog gets one line at the
+    // end.  Otherwise every iteration starts from zero change and logs a line.
+    const bool inmem = nshards == 1 && n < 2u * 40000000u;
+    const vid_t win_en = inmem ? n : (n ? n - 1 : 0);
+    uint64_t updates = 0, work = 0;
+    int last_done = -1;
     metrics_entry mu = m.start_time();
+    if (inmem) ctx.reset_deltas(1);
     for (int iter = 0; iter < niters; iter++) {
         ctx.iteration = iter;
+        if (!inmem) ctx.reset_deltas(1);
         program.before_iteration(iter, ctx);
-        program.before_exec_interval(0, n ? n - 1 : 0, ctx);
+        program.before_exec_interval(0, win_en, ctx);
         graphchi_context dc = ctx;  // the plain fields, for update() on the device
         dc.device_delta = d_delta.get();
         const double zero = 0.0;
@@ -290,10 +313,16 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
         d_delta.download(&delta, 1);
         ctx.log_change(delta);
         updates += n;
-        program.after_exec_interval(0, n ? n - 1 : 0, ctx);
+        work += inmem ? E : 2 * E;  // the reference's count: shard edges, or both ends of every edge
+        program.after_exec_interval(0, win_en, ctx);
         program.after_iteration(iter, ctx);
-        if (ctx.last_iteration >= 0 && iter >= ctx.last_iteration) break;
+        last_done = iter;
+        if (!inmem) write_delta_log(base, ctx, iter, updates, work);
+        if (inmem ? (ctx.last_iteration > 0 && ctx.last_iteration <= iter)
+                  : (ctx.last_iteration >= 0 && iter >= ctx.last_iteration))
+            break;
     }
+    if (inmem && last_done >= 0) write_delta_log(base, ctx, last_done, updates, work);
     m.stop_time(mu, "execute-updates");
     m.set("updates", (double)updates);
     write_vertex_values<VT>(base, vdata.to_host(n));

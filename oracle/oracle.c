@@ -30,6 +30,8 @@
  *   orc_triangles          - golden totals/per-vertex .vout of the reference
  *                            trianglecounting; brute force on tiny graphs; K_n
  *   orc_pr_psw_omp         - equals orc_pagerank_sync_f32 within FP32 tolerance
+ *   orc_psw_pagerank       - the reference engine's ranks, iteration count and
+ *                            change log for the same program (golden fixtures)
  *   orc_psw_run            - the reference engine's own vertex values for the
  *                            probe program (oracle/Makefile builds it against the
  *                            reference headers; golden fixtures, 1-3 shards);
@@ -754,4 +756,78 @@ long orc_psw_run(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *ds
     free(ev);
     free(vv);
     return violations;
+}
+
+/* ---- orc_psw_pagerank ---- */
+static const uint32_t *g_psw_key_src, *g_psw_key_dst;
+static int psw_in_cmp(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    const uint64_t kx = ((uint64_t)g_psw_key_dst[x] << 32) | g_psw_key_src[x];
+    const uint64_t ky = ((uint64_t)g_psw_key_dst[y] << 32) | g_psw_key_src[y];
+    if (kx != ky) return kx < ky ? -1 : 1;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+int orc_psw_pagerank(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int niters, double tol,
+                     int inmem, float *vdata_out, int *iters_run, double *deltas_out) {
+    uint64_t m = 0;
+    uint64_t *in_sorted = malloc((e ? e : 1) * 8);
+    uint64_t *ioff = calloc((size_t)n + 1, 8), *ooff = calloc((size_t)n + 1, 8);
+    uint64_t *oedge = malloc((e ? e : 1) * 8);
+    float *ev = calloc(e ? e : 1, 4), *vv = calloc(n ? n : 1, 4);
+    if (!in_sorted || !ioff || !ooff || !oedge || !ev || !vv) return -2;
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) {
+            in_sorted[m++] = k;
+            ioff[dst[k] + 1]++;
+            ooff[src[k] + 1]++;
+        }
+    g_psw_key_src = src;
+    g_psw_key_dst = dst;
+    qsort(in_sorted, m, 8, psw_in_cmp); /* (dst, src, input index) */
+    for (uint32_t v = 0; v < n; v++) {
+        ioff[v + 1] += ioff[v];
+        ooff[v + 1] += ooff[v];
+    }
+    uint64_t *oc = malloc(((size_t)n + 1) * 8);
+    memcpy(oc, ooff, ((size_t)n + 1) * 8);
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) oedge[oc[src[k]]++] = k;
+    int it = 0;
+    double delta = 0.0;
+    for (; it < niters; it++) {
+        if (!inmem) delta = 0.0;
+        for (uint32_t v = 0; v < n; v++) {
+            const uint64_t nout = ooff[v + 1] - ooff[v];
+            if (it == 0) {
+                for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = (float)(1.0 / (double)(int)nout);
+                vv[v] = 0.15f;
+                continue;
+            }
+            float sum = 0.0f;
+            for (uint64_t q = ioff[v]; q < ioff[v + 1]; q++) sum += ev[in_sorted[q]];
+            const float rank = (float)(0.15 + 0.85 * (double)sum);
+            if (nout > 0) {
+                const float share = rank / (float)(int)nout;
+                for (uint64_t q = ooff[v]; q < ooff[v + 1]; q++) ev[oedge[q]] = share;
+            }
+            delta += (double)fabsf(rank - vv[v]);
+            vv[v] = rank;
+        }
+        if (deltas_out) deltas_out[it] = delta;
+        if (it > 0 && delta < tol) {
+            it++;
+            break;
+        }
+    }
+    if (iters_run) *iters_run = it;
+    if (vdata_out) memcpy(vdata_out, vv, (size_t)n * 4);
+    free(in_sorted);
+    free(ioff);
+    free(ooff);
+    free(oedge);
+    free(oc);
+    free(ev);
+    free(vv);
+    return 0;
 }

@@ -111,6 +111,16 @@ double orc_pr_psw_omp(uint32_t n, uint64_t e, const uint32_t *src, const uint32_
 long orc_psw_run(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int program, int niters,
                  int reverse, uint32_t *vdata_out);
 
+/* graphchi-cpp_amd/apps/psw_pagerank.cpp's program in the same order: in-edges
+ * summed in (src, input index) order in FP32, rank = 0.15 + 0.85 * sum in FP64
+ * rounded to FP32, rank / out-degree to the out-edges, |change| logged in FP64;
+ * after iteration i > 0 the run stops if the logged change is below tol.  With
+ * inmem (the reference engine's in-memory mode, a graph of one shard) the
+ * change accumulates over the run instead of restarting every iteration.
+ * Writes the ranks, the iterations run and the logged change after each. */
+int orc_psw_pagerank(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int niters, double tol,
+                     int inmem, float *vdata_out, int *iters_run, double *deltas_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,7 @@ def lib():
             "orc_triangles": (i32, [u32, u64, vp, vp, vp, vp, vp]),
             "orc_pr_psw_omp": (dbl, [u32, u64, vp, vp, i32, dbl, dbl, f32, i32, i32, vp]),
             "orc_psw_run": (ctypes.c_long, [u32, u64, vp, vp, i32, i32, i32, vp]),
+            "orc_psw_pagerank": (i32, [u32, u64, vp, vp, i32, dbl, i32, vp, vp, vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(_lib, k)
@@ -250,3 +251,15 @@ def psw_run(n, src, dst, program, niters, reverse=False):
     viol = lib().orc_psw_run(n, len(src), _p(src), _p(dst), program, niters, 1 if reverse else 0, _p(vd))
     assert viol >= 0
     return vd[:n], int(viol)
+
+
+def psw_pagerank(n, src, dst, niters, tol, inmem=False):
+    """apps/psw_pagerank.cpp's program in the reference's update order
+    (orc_psw_pagerank): (ranks, iterations run, logged change after each)."""
+    src, dst = u32(src), u32(dst)
+    r = np.zeros(max(n, 1), np.float32)
+    deltas = np.zeros(max(niters, 1), np.float64)
+    it = ctypes.c_int()
+    assert lib().orc_psw_pagerank(n, len(src), _p(src), _p(dst), niters, tol, 1 if inmem else 0, _p(r),
+                                  ctypes.byref(it), _p(deltas)) == 0
+    return r[:n], it.value, deltas[:it.value]

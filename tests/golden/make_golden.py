@@ -13,8 +13,9 @@ gchi_rmat_generate_host), and stores what the reference wrote:
   connectedcomponents                                 -> labels (.4B.vout)
   trianglecounting nshards 2                          -> total, per-vertex counts
                                                         (_degord.4B.vout), .vertexmap
-  oracle/_ref/psw_probe (graphchi-cpp_amd/apps/psw_probe.cpp
-  built against the reference's engine)               -> vertex values (.4B.vout)
+  oracle/_ref/psw_probe, psw_pagerank (graphchi-cpp_amd/
+  apps/psw_*.cpp built against the reference's engine) -> vertex values (.4B.vout),
+                                                        iterations run, .deltalog
 
 Inputs are not stored: tests regenerate them and check their SHA-256 here.
 Usage: python tests/golden/make_golden.py [--only NAME]
@@ -241,8 +242,15 @@ def psw_cases(rr, manifest, arrays, only):
         rr.run("psw_probe", ["file", path, "filetype", "edgelist", "niters", case["niters"], "nshards",
                              case["nshards"]])
         arrays[f"{name}__psw_probe"] = read_u32(path + ".4B.vout")
+        out = rr.run("psw_pagerank", ["file", path, "filetype", "edgelist", "niters", graphs.PSW_PR_MAX_ITERS,
+                                      "tol", case["pr_tol"], "nshards", case["nshards"]])
+        arrays[f"{name}__psw_pagerank"] = read_f32(path + ".4B.vout")
+        with open(path + ".deltalog") as f:
+            log = [line.split(",") for line in f.read().split()]
         manifest[name] = {"kind": "psw", "case": case, "n": n, "e": int(len(src)),
-                          "input_sha": graphs.input_sha(src, dst)}
+                          "input_sha": graphs.input_sha(src, dst),
+                          "psw_pagerank_iterations": int(re.search(r"iterations run: (\d+)", out).group(1)),
+                          "psw_pagerank_deltalog": [[int(a), int(b), int(c), float(d)] for a, b, c, d in log]}
         print("psw", name, n, len(src), flush=True)
 
 

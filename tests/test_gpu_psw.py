@@ -8,6 +8,9 @@ program's own update() run on the device in the reference engine's order.
   edge values) against the vertex values the reference's own engine computed
   for the same program (golden fixtures, 1-3 shards, duplicates both ways, a
   two-way path) and against the oracle on a 2M-edge R-MAT graph, bit for bit;
+* apps/psw_pagerank (FP32 values on the edges, log_change, a stop rule set in
+  after_iteration) against the reference engine's ranks, iteration count and
+  deltalog bookkeeping, including its in-memory mode for one shard;
 * the reference's src/tests/basic_smoketest.cpp, compiled unchanged with hipcc
   (apps/_dropin/basic_smoketest): its update() asserts the order's invariant
   on every in-edge read, on the device, and its vertex-data check passes."""
@@ -82,6 +85,31 @@ def test_psw_probe_equals_reference_engine(workdir, golden, case):
     path = write_edgelist(workdir, case["name"], src, dst)
     run(os.path.join(BIN, "psw_probe"), ["file", path, "niters", case["niters"], "nshards", case["nshards"]], workdir)
     np.testing.assert_array_equal(np.fromfile(path + ".4B.vout", np.uint32), golden[f"{case['name']}__psw_probe"])
+
+
+@pytest.mark.parametrize("case", graphs.PSW_CASES, ids=[c["name"] for c in graphs.PSW_CASES])
+def test_psw_pagerank_equals_reference_engine(workdir, golden, manifest, case):
+    """apps/psw_pagerank (values on the edges, log_change, a stop rule in
+    after_iteration): ranks bit-exact with the reference engine's, the same
+    iterations, the reference's deltalog lines (per iteration, or one line in
+    its in-memory mode for one shard) with the change equal to the oracle's
+    exact sum (device additions in any order: 1e-9 relative)."""
+    src, dst, n = graphs.make(case)
+    path = write_edgelist(workdir, case["name"], src, dst)
+    out = run(os.path.join(BIN, "psw_pagerank"), ["file", path, "niters", graphs.PSW_PR_MAX_ITERS, "tol",
+                                                   case["pr_tol"], "nshards", case["nshards"]], workdir)
+    m = manifest[case["name"]]
+    np.testing.assert_array_equal(np.fromfile(path + ".4B.vout", np.float32), golden[f"{case['name']}__psw_pagerank"])
+    assert f"iterations run: {m['psw_pagerank_iterations']}" in out
+    inmem = case["nshards"] == 1
+    _, _, deltas = oracle.psw_pagerank(n, src, dst, graphs.PSW_PR_MAX_ITERS, case["pr_tol"], inmem)
+    with open(path + ".deltalog") as f:
+        got = [line.split(",") for line in f.read().split()]
+    want = [m["psw_pagerank_deltalog"][-1]] if inmem else m["psw_pagerank_deltalog"]
+    assert len(got) == len(want)
+    for g, w, d in zip(got, want, deltas[-1:] if inmem else deltas):
+        assert [int(x) for x in g[:3]] == w[:3]
+        assert abs(float(g[3]) - d) <= 1e-9 * abs(d) + 1e-6
 
 
 def test_psw_probe_rmat17_vs_oracle(workdir):
