@@ -57,3 +57,26 @@ def test_host_generator_deterministic():
     c = gchi.rmat_host(12, 1000, 9, first=4000)
     assert (c[0] == a[0][4000:]).all() and (c[1] == a[1][4000:]).all()
     assert a[0].max() < 4096 and a[1].max() < 4096
+
+
+def test_size_limits_refused_before_device_work():
+    """Edge counts beyond each entry point's stated limit come back as
+    GCHI_ERR_UNSUPPORTED (-4) before any device memory is touched (the
+    pointers here are placeholders): TC preparation and degree ordering
+    E < 2^31; CSC build, sharder and the generic engine's adjacency
+    E <= 2^32 - 2^25 (a 32-bit edge index plus a grid stride never wraps)."""
+    L = gchi.lib()
+    one = ctypes.c_void_p(1)
+    big = (1 << 32) - (1 << 25) + 1
+    m = ctypes.c_uint64()
+    L.gchi_degree_order.restype = ctypes.c_int
+    L.gchi_degree_order.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    assert L.gchi_tc_prepare(one, one, 1 << 31, 10, one, one, one, ctypes.byref(m), one, 1 << 40, None) == -4
+    assert L.gchi_degree_order(one, one, 1 << 31, 10, one, one, 1 << 40, None) == -4
+    assert L.gchi_csc_build(one, one, big, 10, one, one, one, one, one, 1 << 40, None) == -4
+    sh = ctypes.c_void_p()
+    assert L.gchi_shard_build(one, one, big, 9, 2, ctypes.byref(sh), None) == -4
+    assert L.gchi_psw_graph_build(one, one, big, 10, one, one, one, one, one, one, one, 1 << 40, None) == -4
+    buf = ctypes.create_string_buffer(256)
+    assert L.gchi_last_error(buf, 256) == 0 and b"2^32 - 2^25" in buf.value
