@@ -23,8 +23,9 @@
 // every slot is taken.  Visibility across XCDs: an update ends with a release
 // fence at agent scope, its decrements are relaxed (eight in flight per lane),
 // the lane whose decrement reaches zero takes an acquire fence before it
-// writes the queue entry with a release store, and the queue entry is read
-// with acquire; so every update a vertex waited for happens before it runs.
+// writes the queue entry with a release store, and the lane that finds the
+// entry (polling with relaxed loads) takes an acquire fence before it runs the
+// vertex; so every update a vertex waited for happens before it runs.
 //
 // Memory.  Edge values live in one array in out-edge (src, dst) order; every
 // in-edge and out-edge is a graphchi_edge view (adjacent id, value pointer)
@@ -175,7 +176,9 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
             have = true;
         }
         if (slot >= a.n) break;
-        const uint32_t v = __hip_atomic_load(&a.queue[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        // polled with a relaxed load: an acquire on every poll invalidates the
+        // L2 each time, and thousands of waiting lanes made that 14x slower
+        const uint32_t v = __hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (v == kEmpty) {
             const uint32_t filled = __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (filled != seen) {
@@ -188,6 +191,7 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // pairs with the queue entry's release store
         const uint32_t ib = a.in_off[v], ie = a.in_off[v + 1], ob = a.out_off[v], oe = a.out_off[v + 1];
         graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
         vx.dataptr = a.vdata + v;
