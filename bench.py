@@ -396,7 +396,7 @@ def main():
                     help="also run a device copy of this many MiB (counter calibration for HBM bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one GPU only: the package has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
     only = set(x for x in args.only.split(",") if x)

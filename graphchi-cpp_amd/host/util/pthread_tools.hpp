@@ -27,8 +27,8 @@ class mutex {
     mutable pthread_mutex_t m_;
 };
 
-// Same interface; the host side of this build runs programs' callbacks on one
-// thread, so a plain mutex serves.
+// Same interface; the host layer runs programs' callbacks on one thread, so a
+// plain mutex serves.
 typedef mutex spinlock;
 
 }  // namespace graphchi

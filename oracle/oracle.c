@@ -1,7 +1,8 @@
 /*
  * oracle.c -- CPU restatement of the GraphChi hot path.  TEST INFRASTRUCTURE:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
- * always as the checker (or the timed CPU baseline), never as the product.
+ * always as what results are compared against (or the timed CPU baseline),
+ * never as the product.
  *
  * Pinning (what checks each function, beyond agreeing with the GPU):
  *   orc_csc_build          - closed forms on hand graphs; offsets == in-degree scan

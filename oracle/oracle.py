@@ -1,4 +1,4 @@
-"""ctypes wrapper of liboracle.so -- the CPU restatement used as the checker.
+"""ctypes wrapper of liboracle.so -- the CPU restatement results are compared against.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py; never by the product.
