@@ -10,13 +10,19 @@
 
 namespace graphchi {
 
+// The scheduler interface (src/api/ischeduler.hpp:38-46).  Host-and-device
+// with hipcc: the generic device engine gives update() a scheduler object
+// built on the device (engine/psw_engine.hpp).  This base class schedules
+// nothing and reports every vertex scheduled (the reference's non_scheduler).
 class ischeduler {
   public:
-    virtual ~ischeduler() {}
-    virtual void add_task(vid_t, bool = false) {}
-    virtual void remove_tasks(vid_t, vid_t) {}
-    virtual void add_task_to_all() {}
-    virtual bool is_scheduled(vid_t) { return true; }
+    GCHI_HD virtual ~ischeduler() {}
+    GCHI_HD virtual void add_task(vid_t, bool = false) {}
+    GCHI_HD virtual void remove_tasks(vid_t, vid_t) {}
+    GCHI_HD virtual void add_task_to_all() {}
+    GCHI_HD virtual bool is_scheduled(vid_t) { return true; }
+    GCHI_HD virtual size_t num_tasks() { return 0; }
+    GCHI_HD virtual void new_iteration(int) {}
 };
 
 // The device engine copies the context into HBM for every pass and reads only

@@ -73,8 +73,7 @@ template <typename VertexDataType, typename EdgeDataType,
 class graphchi_engine {
   public:
     graphchi_engine(std::string base_filename, int nshards, bool selective_scheduling, metrics &m)
-        : base_(base_filename), nshards_(nshards), m_(m) {
-        (void)selective_scheduling;
+        : base_(base_filename), nshards_(nshards), scheduling_(selective_scheduling), m_(m) {
         nvertices_ = get_num_vertices(base_);
         ctx_.nvertices = nvertices_;
         m_.set("engine", "device");
@@ -119,7 +118,7 @@ class graphchi_engine {
 #if defined(__HIPCC__)
         ctx_.last_iteration = -1;
         psw::run_program<VertexDataType, EdgeDataType>(program, base_, nshards_, (uint32_t)nvertices_,
-                                                       ctx_.num_iterations, ctx_, m_, nedges_);
+                                                       ctx_.num_iterations, scheduling_, ctx_, m_, nedges_);
 #else
         (void)program;
         logstream(LOG_FATAL) << "no device implementation is registered for this program "
@@ -286,6 +285,7 @@ class graphchi_engine {
 
     std::string base_;
     int nshards_;
+    bool scheduling_;  // selective scheduling (the generic device engine's bitset scheduler)
     metrics &m_;
     graphchi_context ctx_;
     size_t nvertices_ = 0, nedges_ = 0;

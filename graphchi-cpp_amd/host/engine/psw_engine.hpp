@@ -34,9 +34,16 @@
 // reference's engine starts from zero-filled vertex data and, for graphs
 // converted without edge values, zero edge data).
 //
-// Not supported on the device: the scheduler (selective scheduling),
-// repeat_updates(), dynamic edge data, and the reference's writing of edge
-// values back into the shard files.  The iteration hooks run on the host
+// Selective scheduling (src/engine/bitset_scheduler.hpp): two bitsets in HBM,
+// this iteration's tasks and the next's; update() reaches them through a
+// scheduler object built on the device.  As in the reference, a vertex's
+// scheduled flag is read when its interval starts (the reference reads it
+// when it loads the sub-interval), so with scheduling on, a graph of several
+// shards runs one pass per interval; add_task(v, true) then runs v in this
+// iteration exactly when v's interval has not started.
+//
+// Not supported on the device: repeat_updates(), dynamic edge data, and the
+// reference's writing of edge values back into the shard files.  The iteration hooks run on the host
 // between passes; before_exec_interval / after_exec_interval are called once
 // per iteration with the whole vertex range.  log_change() on the device adds
 // to one FP64 accumulator per pass (its order of additions is not fixed);
@@ -47,8 +54,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <new>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -89,15 +98,20 @@ GCHI_HD inline void check_assert(bool ok, const char *expr, const char *file, in
 #endif
 }
 
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) logstream(LOG_FATAL) << what << ": " << hipGetErrorString(e) << std::endl;
+}
+
 template <typename VT, typename ET>
 struct PassArgs {
-    uint32_t n;
+    uint32_t vlo, vhi;                 // the pass runs vertices [vlo, vhi)
     const uint32_t *out_off, *in_off;  // [n+1]
     const uint32_t *out_dst, *in_src;  // [E]
     graphchi_edge<ET> *out_edges, *in_edges;  // [E] views
     VT *vdata;                         // [n]
     uint32_t *dep;                     // [n] updates still awaited this pass
-    uint32_t *queue;                   // [n] vertices in the order they became ready
+    const uint8_t *sched;              // [n] 1: update() runs this pass (null: every vertex)
+    uint32_t *queue;                   // [vhi - vlo] vertices in the order they became ready
     uint32_t *counters;                // [0] slots taken, [1] slots filled, [2] a wait gave up
     uint32_t spin_limit;               // waits with no vertex queued meanwhile before giving up
 };
@@ -118,25 +132,189 @@ __global__ void edge_views(const uint32_t *out_dst, const uint32_t *in_src, cons
     }
 }
 
-__global__ void pass_reset(const uint32_t *lower, uint32_t n, uint32_t *dep, uint32_t *queue, uint32_t *counters) {
+// dep[v] = the updates v waits for in a pass over [vlo, vhi): its lower-id
+// incident edges, less those to vertices below vlo (cross, when given: done in
+// an earlier pass)
+__global__ void pass_reset(const uint32_t *lower, const uint32_t *cross, uint32_t vlo, uint32_t vhi, uint32_t *dep,
+                           uint32_t *queue, uint32_t *counters) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t v = gid; v < n; v += stride) {
-        dep[v] = lower[v];
-        queue[v] = kEmpty;
+    for (uint64_t v = (uint64_t)vlo + gid; v < vhi; v += stride) {
+        dep[v] = lower[v] - (cross ? cross[v] : 0u);
+        queue[v - vlo] = kEmpty;
     }
     if (gid < 3) counters[gid] = 0u;
 }
 
-__global__ void pass_seed(const uint32_t *lower, uint32_t n, uint32_t *queue, uint32_t *counters) {
+__global__ void pass_seed(const uint32_t *dep, uint32_t vlo, uint32_t vhi, uint32_t *queue, uint32_t *counters) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint64_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        if (lower[v] == 0u) {
+    for (uint64_t v = (uint64_t)vlo + blockIdx.x * blockDim.x + threadIdx.x; v < vhi; v += stride) {
+        if (dep[v] == 0u) {
             const uint32_t s = atomicAdd(&counters[1], 1u);
             __hip_atomic_store(&queue[s], (uint32_t)v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
+
+// cross[v] for v in [vlo, vhi) = incident edges of v whose other end is below vlo
+__global__ void count_cross(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
+                            const uint32_t *out_dst, uint32_t vlo, uint32_t vhi, uint32_t *cross) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)vlo + blockIdx.x * blockDim.x + threadIdx.x; v < vhi; v += stride) {
+        const uint32_t lo = vlo;
+        uint32_t c = 0;
+        for (uint32_t k = in_off[v]; k < in_off[v + 1]; k++) c += in_src[k] < lo ? 1u : 0u;
+        for (uint32_t k = out_off[v]; k < out_off[v + 1]; k++) c += out_dst[k] < lo ? 1u : 0u;
+        cross[v] = c;
+    }
+}
+
+// The scheduler update() sees on the device: bitset_scheduler.hpp:38-104 with a
+// byte per vertex instead of a bit, so that concurrent updates set flags with
+// plain stores.  cur holds this iteration's tasks, next the next iteration's,
+// has_new the reference's has_new_tasks.  Built in device memory by a kernel,
+// so its virtual functions dispatch to their device code.  With cur null it
+// is the reference's non_scheduler (nothing recorded, every vertex scheduled).
+class device_scheduler : public ischeduler {
+  public:
+    GCHI_HD device_scheduler(uint8_t *cur, uint8_t *next, uint32_t *has_new, uint32_t n)
+        : cur_(cur), next_(next), has_new_(has_new), n_(n) {}
+    GCHI_HD void add_task(vid_t v, bool also_this_iteration = false) override {
+        if (!cur_ || v >= n_) return;
+        next_[v] = 1;
+        if (also_this_iteration) cur_[v] = 1;
+        *has_new_ = 1u;
+    }
+    GCHI_HD void remove_tasks(vid_t from, vid_t to) override {  // [from, to], as dense_bitset::clear_bits
+        if (!cur_) return;
+        for (uint64_t v = from; v <= to && v < n_; v++) next_[v] = 0;
+    }
+    GCHI_HD void add_task_to_all() override {
+        if (!cur_) return;
+        for (uint32_t v = 0; v < n_; v++) cur_[v] = 1;
+        *has_new_ = 1u;
+    }
+    GCHI_HD bool is_scheduled(vid_t v) override { return !cur_ || (v < n_ && cur_[v]); }
+    GCHI_HD size_t num_tasks() override {
+        size_t c = 0;
+        if (cur_)
+            for (uint32_t v = 0; v < n_; v++) c += cur_[v];
+        return c;
+    }
+
+  private:
+    uint8_t *cur_, *next_;
+    uint32_t *has_new_;
+    uint32_t n_;
+};
+
+__global__ void make_device_scheduler(device_scheduler *p, uint8_t *cur, uint8_t *next, uint32_t *has_new,
+                                      uint32_t n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) new (p) device_scheduler(cur, next, has_new, n);
+}
+
+// The pass's scheduled flags, read when it starts (the reference reads them
+// when it loads the sub-interval, graphchi_engine.hpp:556-566): snap[v] =
+// cur[v] (every vertex when cur is null) for v in [vlo, vhi); stats[0] += the
+// scheduled vertices, stats[1] += their in- and out-edges (the reference's
+// nupdates and work).
+__global__ void snapshot_schedule(const uint8_t *cur, const uint32_t *in_off, const uint32_t *out_off, uint32_t vlo,
+                                  uint32_t vhi, uint8_t *snap, unsigned long long *stats) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long cnt = 0, edges = 0;
+    for (uint64_t v = (uint64_t)vlo + blockIdx.x * blockDim.x + threadIdx.x; v < vhi; v += stride) {
+        const uint8_t s = cur ? cur[v] : (uint8_t)1;
+        snap[v] = s;
+        if (s) {
+            cnt++;
+            edges += (in_off[v + 1] - in_off[v]) + (out_off[v + 1] - out_off[v]);
+        }
+    }
+    if (cnt) {
+        atomicAdd(&stats[0], cnt);
+        atomicAdd(&stats[1], edges);
+    }
+}
+
+// The scheduler the program's hooks see on the host.  The flags stay in HBM:
+// the first call after device work reads them back, and the engine writes
+// changed ones back before the next device step.
+class host_scheduler : public ischeduler {
+  public:
+    uint8_t *cur = nullptr, *next = nullptr;
+    uint32_t *has_new = nullptr;
+    uint32_t n = 0;
+
+    // host-and-device only because the base's virtuals are; the device never
+    // holds this object (its bodies are empty in the device pass)
+    GCHI_HD void add_task(vid_t v, bool also_this_iteration = false) override {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (v >= n) return;
+        pull();
+        hn_[v] = 1;
+        if (also_this_iteration) hc_[v] = 1;
+        hf_ = 1u;
+        dirty_ = true;
+#endif
+    }
+    GCHI_HD void remove_tasks(vid_t from, vid_t to) override {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        pull();
+        for (uint64_t v = from; v <= to && v < n; v++) hn_[v] = 0;
+        dirty_ = true;
+#endif
+    }
+    GCHI_HD void add_task_to_all() override {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        pull();
+        std::fill(hc_.begin(), hc_.end(), (uint8_t)1);
+        hf_ = 1u;
+        dirty_ = true;
+#endif
+    }
+    GCHI_HD bool is_scheduled(vid_t v) override {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        pull();
+        return v < n && hc_[v];
+#else
+        return false;
+#endif
+    }
+    GCHI_HD size_t num_tasks() override {
+        size_t c = 0;
+#if !defined(__HIP_DEVICE_COMPILE__)
+        pull();
+        for (uint8_t x : hc_) c += x;
+#endif
+        return c;
+    }
+    // the engine's side
+    void push() {
+        if (!dirty_) return;
+        hip_check(hipMemcpy(cur, hc_.data(), n, hipMemcpyHostToDevice), "scheduler upload");
+        hip_check(hipMemcpy(next, hn_.data(), n, hipMemcpyHostToDevice), "scheduler upload");
+        hip_check(hipMemcpy(has_new, &hf_, 4, hipMemcpyHostToDevice), "scheduler upload");
+        dirty_ = false;
+    }
+    void stale() {  // device work may have changed the flags
+        push();
+        valid_ = false;
+    }
+
+  private:
+    std::vector<uint8_t> hc_, hn_;
+    uint32_t hf_ = 0;
+    bool valid_ = false, dirty_ = false;
+    void pull() {
+        if (valid_) return;
+        hc_.resize(n);
+        hn_.resize(n);
+        hip_check(hipMemcpy(hc_.data(), cur, n, hipMemcpyDeviceToHost), "scheduler download");
+        hip_check(hipMemcpy(hn_.data(), next, n, hipMemcpyDeviceToHost), "scheduler download");
+        hip_check(hipMemcpy(&hf_, has_new, 4, hipMemcpyDeviceToHost), "scheduler download");
+        valid_ = true;
+    }
+};
 
 // One decrement of dep[w] for each neighbour id w > v in nb[0, cnt), eight in
 // flight; a vertex whose count reaches zero is queued.
@@ -146,7 +324,10 @@ __device__ inline void release_higher(const PassArgs<VT, ET> &a, uint32_t v, con
     for (uint32_t k0 = 0; k0 < cnt; k0 += kBatch) {
         uint32_t w[kBatch], left[kBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; q++) w[q] = k0 + q < cnt ? nb[k0 + q] : 0u;  // 0 is never above v
+        for (int q = 0; q < kBatch; q++) {
+            w[q] = k0 + q < cnt ? nb[k0 + q] : 0u;  // 0 is never above v
+            if (w[q] >= a.vhi) w[q] = 0u;            // a later pass's vertex
+        }
 #pragma unroll
         for (int q = 0; q < kBatch; q++)
             left[q] = w[q] > v ? __hip_atomic_fetch_add(&a.dep[w[q]], 0xffffffffu, __ATOMIC_RELAXED,
@@ -175,7 +356,7 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
             slot = atomicAdd(&a.counters[0], 1u);
             have = true;
         }
-        if (slot >= a.n) break;
+        if (slot >= a.vhi - a.vlo) break;
         // polled with a relaxed load: an acquire on every poll invalidates the
         // L2 each time, and thousands of waiting lanes made that 14x slower
         const uint32_t v = __hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -195,7 +376,7 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
         const uint32_t ib = a.in_off[v], ie = a.in_off[v + 1], ob = a.out_off[v], oe = a.out_off[v + 1];
         graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
         vx.dataptr = a.vdata + v;
-        prog.P::update(vx, *ctx);
+        if (!a.sched || a.sched[v]) prog.P::update(vx, *ctx);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         release_higher(a, v, a.in_src + ib, ie - ib);
         release_higher(a, v, a.out_dst + ob, oe - ob);
@@ -212,10 +393,6 @@ inline void write_delta_log(const std::string &base, const graphchi_context &ctx
     }
 }
 
-inline void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) logstream(LOG_FATAL) << what << ": " << hipGetErrorString(e) << std::endl;
-}
-
 inline unsigned grid_for(uint64_t n, unsigned cap) {
     uint64_t b = (n + kThreads - 1) / kThreads;
     if (b == 0) b = 1;
@@ -225,7 +402,7 @@ inline unsigned grid_for(uint64_t n, unsigned cap) {
 // Runs niters iterations of program on the graph stored in base's shards and
 // writes the vertex values to <base>.<sizeof VT>B.vout.
 template <typename VT, typename ET, class ProgramT>
-void run_program(ProgramT &program, const std::string &base, int nshards, uint32_t n, int niters,
+void run_program(ProgramT &program, const std::string &base, int nshards, uint32_t n, int niters, bool scheduling,
                  graphchi_context &ctx, metrics &m, size_t &nedges_out) {
     static_assert(std::is_trivially_copyable<VT>::value && std::is_trivially_copyable<ET>::value,
                   "device vertex programs need trivially copyable vertex and edge data");
@@ -236,8 +413,9 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     nedges_out = E;
     ctx.nedges = E;
     const uint64_t E1 = E ? E : 1;
+    const uint32_t n1 = n ? n : 1;
     device::buffer<uint32_t> out_off((size_t)n + 1), in_off((size_t)n + 1), out_dst(E1), in_src(E1), in_eidx(E1),
-        lower(n);
+        lower(n1);
     {
         size_t wb = 0;
         device::check(gchi_psw_graph_workspace(E, n, &wb), "gchi_psw_graph_workspace");
@@ -250,15 +428,16 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     src.release();
     dst.release();
     device::buffer<ET> edata(E1);
-    device::buffer<VT> vdata(n);
+    device::buffer<VT> vdata(n1);
     device::buffer<graphchi_edge<ET>> out_edges(E1), in_edges(E1);
-    device::buffer<uint32_t> dep(n), queue(n), counters(4);
+    device::buffer<uint32_t> dep(n1), queue(n1), counters(4);
     device::buffer<graphchi_context> d_ctx(1);
     device::buffer<double> d_delta(1);
     ET *ed = edata.get();
     VT *vd = vdata.get();
     graphchi_edge<ET> *oe = out_edges.get(), *ie = in_edges.get();
     const uint32_t *od = out_dst.get(), *is = in_src.get(), *ix = in_eidx.get(), *lw = lower.get();
+    const uint32_t *ioff = in_off.get(), *ooff = out_off.get();
     uint32_t *dp = dep.get(), *qu = queue.get(), *ct = counters.get();
     graphchi_context *dctx = d_ctx.get();
     hipLaunchKernelGGL(fill_zero<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, ed, (uint64_t)E);
@@ -267,14 +446,7 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
         hipLaunchKernelGGL(edge_views<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, od, is, ix, (uint64_t)E, ed,
                            oe, ie);
     hip_check(hipGetLastError(), "psw setup");
-    m.stop_time(me, "load_shards");
 
-    PassArgs<VT, ET> a{n, out_off.get(), in_off.get(), od, is, oe, ie, vd, dp, qu, ct, 1u << 24};
-    int dev = 0, ncu = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-        ncu = prop.multiProcessorCount;
-    const unsigned grid = grid_for(n, (unsigned)ncu * 8);
     // The reference runs a graph of one shard (and fewer than 2 * 40M vertices)
     // in its in-memory mode (graphchi_engine.hpp:296-301, 465-523): all
     // iterations inside one outer iteration, so log_change() accumulates over
@@ -282,29 +454,114 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     // the run only when it is above 0, and <base>.deltalog gets one line at the
     // end.  Otherwise every iteration starts from zero change and logs a line.
     const bool inmem = nshards == 1 && n < 2u * 40000000u;
-    const vid_t win_en = inmem ? n : (n ? n - 1 : 0);
+
+    // Passes: one over all vertices, or with scheduling on a graph of several
+    // shards, one per interval in order, so that each reads its scheduled
+    // flags after the intervals before it ran.  A pass over [vlo, vhi) waits
+    // only for neighbours in it: cross[v] counts the others (done earlier).
+    struct pass_range {
+        uint32_t lo, hi;
+    };
+    std::vector<pass_range> passes;
+    const bool per_interval = scheduling && !inmem;
+    device::buffer<uint32_t> cross(per_interval ? n1 : 1);
+    uint32_t *crs = per_interval ? cross.get() : nullptr;
+    if (per_interval) {
+        uint32_t lo = 0;
+        for (vid_t end : load_interval_ends(base, nshards)) {
+            const uint32_t hi = std::min<uint64_t>((uint64_t)end + 1, n);
+            if (hi > lo) passes.push_back({lo, hi});
+            lo = std::max(lo, hi);
+        }
+        if (lo < n) passes.push_back({lo, n});
+        for (const pass_range &r : passes)
+            hipLaunchKernelGGL(count_cross, dim3(grid_for(r.hi - r.lo, 4096)), dim3(kThreads), 0, 0, ioff, is, ooff, od,
+                               r.lo, r.hi, crs);
+    } else {
+        passes.push_back({0, n});
+    }
+
+    // the scheduler (graphchi_engine.hpp:285-291: every vertex scheduled at the start)
+    device::buffer<uint8_t> sched_a(scheduling ? n1 : 1), sched_b(scheduling ? n1 : 1), snap(scheduling ? n1 : 1);
+    device::buffer<uint32_t> has_new(1);
+    device::buffer<unsigned long long> stats(2);
+    device::buffer<uint8_t> dsched_mem(sizeof(device_scheduler));
+    device_scheduler *dsched = reinterpret_cast<device_scheduler *>(dsched_mem.get());
+    uint32_t *hnew = has_new.get();
+    uint8_t *snp = snap.get();
+    unsigned long long *sts = stats.get();
+    host_scheduler hsched;
+    ischeduler no_scheduler;
+    if (scheduling) {
+        hsched.cur = sched_a.get();
+        hsched.next = sched_b.get();
+        hsched.has_new = has_new.get();
+        hsched.n = n;
+        const uint32_t one = 1u;
+        hip_check(hipMemset(hsched.cur, 1, n), "scheduler init");
+        hip_check(hipMemset(hsched.next, 0, n), "scheduler init");
+        has_new.upload(&one, 1);
+    }
+    auto make_dsched = [&]() {
+        uint8_t *c = scheduling ? hsched.cur : nullptr, *x = scheduling ? hsched.next : nullptr;
+        hipLaunchKernelGGL(make_device_scheduler, dim3(1), dim3(1), 0, 0, dsched, c, x, hnew, n);
+        hip_check(hipGetLastError(), "scheduler setup");
+    };
+    make_dsched();
+    ctx.scheduler = scheduling ? static_cast<ischeduler *>(&hsched) : &no_scheduler;
+    auto read_has_new = [&]() {
+        hsched.stale();
+        uint32_t f = 0;
+        has_new.download(&f, 1);
+        return f != 0;
+    };
+    auto clear_has_new = [&]() {
+        hsched.stale();
+        const uint32_t zero = 0;
+        has_new.upload(&zero, 1);
+    };
+    // bitset_scheduler::new_iteration (bitset_scheduler.hpp:50-58)
+    auto new_iteration = [&](int iter) {
+        if (iter == 0) return;
+        hsched.stale();
+        std::swap(hsched.cur, hsched.next);
+        uint8_t *x = hsched.next;
+        hipLaunchKernelGGL(fill_zero<uint8_t>, dim3(grid_for(n, 4096)), dim3(kThreads), 0, 0, x, (uint64_t)n);
+        make_dsched();
+    };
+    m.stop_time(me, "load_shards");
+
+    int dev = 0, ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        ncu = prop.multiProcessorCount;
     uint64_t updates = 0, work = 0;
-    int last_done = -1;
+    bool started = false;
     metrics_entry mu = m.start_time();
     if (inmem) ctx.reset_deltas(1);
-    for (int iter = 0; iter < niters; iter++) {
-        ctx.iteration = iter;
-        if (!inmem) ctx.reset_deltas(1);
-        program.before_iteration(iter, ctx);
-        program.before_exec_interval(0, win_en, ctx);
+
+    // one pass over [lo, hi) with the given scheduled flags (null: all); adds
+    // its log_change() total to ctx
+    auto run_pass = [&](const pass_range &r, const uint8_t *sched) {
+        const uint32_t cnt_v = r.hi - r.lo;
+        PassArgs<VT, ET> a{r.lo, r.hi, ooff, ioff, od, is, oe, ie, vd, dp, sched, qu, ct, 1u << 24};
         graphchi_context dc = ctx;  // the plain fields, for update() on the device
         dc.device_delta = d_delta.get();
+        dc.scheduler = dsched;
         const double zero = 0.0;
         const unsigned int uzero = 0;
         d_ctx.upload(&dc, 1);
         d_delta.upload(&zero, 1);
         hip_check(hipMemcpyToSymbol(HIP_SYMBOL(device_assert_failed), &uzero, 4), "hipMemcpyToSymbol");
         hip_check(hipMemcpyToSymbol(HIP_SYMBOL(device_assert_line), &uzero, 4), "hipMemcpyToSymbol");
-        hipLaunchKernelGGL(pass_reset, dim3(grid_for(n, 4096)), dim3(kThreads), 0, 0, lw, n, dp, qu, ct);
-        hipLaunchKernelGGL(pass_seed, dim3(grid_for(n, 4096)), dim3(kThreads), 0, 0, lw, n, qu, ct);
-        if (n) hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(grid), dim3(kThreads), 0, 0, program, a, dctx);
+        const unsigned g = grid_for(cnt_v, 4096);
+        hipLaunchKernelGGL(pass_reset, dim3(g), dim3(kThreads), 0, 0, lw, crs, r.lo, r.hi, dp, qu, ct);
+        hipLaunchKernelGGL(pass_seed, dim3(g), dim3(kThreads), 0, 0, dp, r.lo, r.hi, qu, ct);
+        hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(grid_for(cnt_v, (unsigned)ncu * 8)), dim3(kThreads), 0, 0,
+                           program, a, dctx);
         hip_check(hipGetLastError(), "psw pass launch");
         hip_check(hipDeviceSynchronize(), "psw pass");
+        hsched.stale();
         const std::vector<uint32_t> cnt = counters.to_host(3);
         if (cnt[2]) logstream(LOG_FATAL) << "psw pass: a wait for a queued vertex gave up" << std::endl;
         unsigned int failed = 0, line = 0;
@@ -312,23 +569,82 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
         hip_check(hipMemcpyFromSymbol(&line, HIP_SYMBOL(device_assert_line), 4), "hipMemcpyFromSymbol");
         if (failed)
             logstream(LOG_FATAL) << "assertion failed in update() on the device (line " << line << "), iteration "
-                                 << iter << std::endl;
+                                 << ctx.iteration << std::endl;
         double delta = 0.0;
         d_delta.download(&delta, 1);
         ctx.log_change(delta);
-        updates += n;
-        work += inmem ? E : 2 * E;  // the reference's count: shard edges, or both ends of every edge
-        program.after_exec_interval(0, win_en, ctx);
+    };
+    // snapshot of this iteration's flags over [lo, hi); returns (vertices, edges)
+    auto take_snapshot = [&](const pass_range &r, bool all) {
+        hsched.stale();
+        const unsigned long long z[2] = {0, 0};
+        stats.upload(z, 2);
+        const uint8_t *c = all ? nullptr : hsched.cur;
+        hipLaunchKernelGGL(snapshot_schedule, dim3(grid_for(r.hi - r.lo, 4096)), dim3(kThreads), 0, 0, c, ioff, ooff,
+                           r.lo, r.hi, snp, sts);
+        hip_check(hipGetLastError(), "scheduler snapshot");
+        std::vector<unsigned long long> st = stats.to_host(2);
+        return std::make_pair((uint64_t)st[0], (uint64_t)st[1]);
+    };
+
+    for (int iter = 0; iter < niters; iter++) {
+        started = true;
+        ctx.iteration = iter;
+        if (!inmem) ctx.reset_deltas(1);
+        program.before_iteration(iter, ctx);
+        if (inmem) {
+            // graphchi_engine.hpp:465-511
+            program.before_exec_interval(0, n, ctx);
+            if (scheduling) {
+                if (iter > 0 && !read_has_new()) break;
+                new_iteration(iter);
+                const auto st = take_snapshot(passes[0], iter == 0);
+                if (st.first == 0) break;
+                clear_has_new();
+                updates += st.first;
+                work += st.second;
+                run_pass(passes[0], snp);
+            } else {
+                updates += n;
+                work += E;
+                if (n) run_pass(passes[0], nullptr);
+            }
+            program.after_exec_interval(0, n, ctx);
+            program.after_iteration(iter, ctx);
+            if (ctx.last_iteration > 0 && ctx.last_iteration <= iter) break;
+            continue;
+        }
+        // graphchi_engine.hpp:798-960
+        if (scheduling) {
+            if (!read_has_new()) break;
+            clear_has_new();
+            new_iteration(iter);
+            for (const pass_range &r : passes) {
+                program.before_exec_interval(r.lo, r.hi - 1, ctx);
+                const auto st = take_snapshot(r, false);
+                updates += st.first;
+                work += st.second;
+                if (st.first) run_pass(r, snp);
+                program.after_exec_interval(r.lo, r.hi - 1, ctx);
+            }
+        } else {
+            const vid_t win_en = n ? n - 1 : 0;
+            program.before_exec_interval(0, win_en, ctx);
+            updates += n;
+            work += 2 * E;  // both ends of every edge
+            if (n) run_pass(passes[0], nullptr);
+            program.after_exec_interval(0, win_en, ctx);
+        }
         program.after_iteration(iter, ctx);
-        last_done = iter;
-        if (!inmem) write_delta_log(base, ctx, iter, updates, work);
-        if (inmem ? (ctx.last_iteration > 0 && ctx.last_iteration <= iter)
-                  : (ctx.last_iteration >= 0 && iter >= ctx.last_iteration))
-            break;
+        write_delta_log(base, ctx, iter, updates, work);
+        if (ctx.last_iteration >= 0 && iter >= ctx.last_iteration) break;
     }
-    if (inmem && last_done >= 0) write_delta_log(base, ctx, last_done, updates, work);
+    if (inmem && started) write_delta_log(base, ctx, ctx.iteration, updates, work);
     m.stop_time(mu, "execute-updates");
     m.set("updates", (double)updates);
+    m.set("work", (double)work);
+    hsched.stale();
+    ctx.scheduler = nullptr;
     write_vertex_values<VT>(base, vdata.to_host(n));
 }
 

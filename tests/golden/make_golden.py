@@ -13,9 +13,10 @@ gchi_rmat_generate_host), and stores what the reference wrote:
   connectedcomponents                                 -> labels (.4B.vout)
   trianglecounting nshards 2                          -> total, per-vertex counts
                                                         (_degord.4B.vout), .vertexmap
-  oracle/_ref/psw_probe, psw_pagerank (graphchi-cpp_amd/
-  apps/psw_*.cpp built against the reference's engine) -> vertex values (.4B.vout),
-                                                        iterations run, .deltalog
+  oracle/_ref/psw_probe, psw_pagerank, psw_sched (graphchi-cpp_amd/
+  apps/psw_*.cpp built against the reference's engine) -> vertex values (.4B.vout, .8B.vout),
+                                                        iterations run, .deltalog,
+                                                        printed task counts
 
 Inputs are not stored: tests regenerate them and check their SHA-256 here.
 Usage: python tests/golden/make_golden.py [--only NAME]
@@ -247,10 +248,20 @@ def psw_cases(rr, manifest, arrays, only):
         arrays[f"{name}__psw_pagerank"] = read_f32(path + ".4B.vout")
         with open(path + ".deltalog") as f:
             log = [line.split(",") for line in f.read().split()]
+        # psw_sched reads edge values at iteration 0: fresh shards (zero edge values)
+        spath = rr.graph_file(name + "_sched", src, dst)
+        sout = rr.run("psw_sched", ["file", spath, "filetype", "edgelist", "niters", graphs.PSW_SCHED_MAX_ITERS,
+                                    "nshards", case["nshards"]])
+        arrays[f"{name}__psw_sched"] = np.fromfile(spath + ".8B.vout", np.uint64)
+        with open(spath + ".deltalog") as f:
+            slog = [line.split(",") for line in f.read().split()]
         manifest[name] = {"kind": "psw", "case": case, "n": n, "e": int(len(src)),
                           "input_sha": graphs.input_sha(src, dst),
                           "psw_pagerank_iterations": int(re.search(r"iterations run: (\d+)", out).group(1)),
-                          "psw_pagerank_deltalog": [[int(a), int(b), int(c), float(d)] for a, b, c, d in log]}
+                          "psw_pagerank_deltalog": [[int(a), int(b), int(c), float(d)] for a, b, c, d in log],
+                          "psw_sched_deltalog": [[int(a), int(b), int(c), float(d)] for a, b, c, d in slog],
+                          "psw_sched_tasks": [[int(a), int(b)] for a, b in
+                                              re.findall(r"psw_sched: iteration (\d+) tasks (\d+)", sout)]}
         print("psw", name, n, len(src), flush=True)
 
 

@@ -66,7 +66,11 @@ PSW_CASES = [
      "nshards": 2, "pr_tol": 1e-2},
     {"name": "psw_rmat12_p3", "rmat": (12, 8, 7), "niters": 4, "nshards": 3, "pr_tol": 0.5},
     {"name": "psw_rmat14_p2", "rmat": (14, 16, 8), "niters": 3, "nshards": 2, "pr_tol": 2.0},
+    # a path through a random permutation of the ids plus random extra edges:
+    # labels travel against the id order, so apps/psw_sched runs many iterations
+    {"name": "psw_chain_p4", "chain": (3000, 300, 7), "niters": 3, "nshards": 4, "pr_tol": 1e-2},
 ]
+PSW_SCHED_MAX_ITERS = 60  # apps/psw_sched.cpp's niters in the fixtures (its scheduler runs out of tasks sooner)
 PSW_PR_MAX_ITERS = 40  # apps/psw_pagerank.cpp's niters in the fixtures (the stop rule ends it earlier)
 
 
@@ -96,6 +100,12 @@ def make(case):
     if "hand" in case:
         e = np.array(case["hand"], dtype=np.uint32).reshape(-1, 2)
         src, dst = e[:, 0], e[:, 1]
+    elif "chain" in case:
+        nv, extra, seed = case["chain"]
+        rng = np.random.default_rng(seed)
+        p = rng.permutation(nv)
+        src = np.concatenate([p[:-1], rng.integers(0, nv, extra)])
+        dst = np.concatenate([p[1:], rng.integers(0, nv, extra)])
     else:
         scale, ef, seed = case["rmat"]
         src, dst = rmat_pairs(scale, ef, seed, undirected=case.get("undirected", False))

@@ -11,10 +11,15 @@ program's own update() run on the device in the reference engine's order.
 * apps/psw_pagerank (FP32 values on the edges, log_change, a stop rule set in
   after_iteration) against the reference engine's ranks, iteration count and
   deltalog bookkeeping, including its in-memory mode for one shard;
+* apps/psw_sched (selective scheduling: add_task for this and the next
+  iteration, remove_tasks, host calls from the hooks, num_tasks) against the
+  reference engine's vertex values, deltalog (updates and work of every
+  iteration) and printed task counts, at 1-4 shards;
 * the reference's src/tests/basic_smoketest.cpp, compiled unchanged with hipcc
   (apps/_dropin/basic_smoketest): its update() asserts the order's invariant
   on every in-edge read, on the device, and its vertex-data check passes."""
 import os
+import re
 import shutil
 import subprocess
 import tempfile
@@ -110,6 +115,21 @@ def test_psw_pagerank_equals_reference_engine(workdir, golden, manifest, case):
     for g, w, d in zip(got, want, deltas[-1:] if inmem else deltas):
         assert [int(x) for x in g[:3]] == w[:3]
         assert abs(float(g[3]) - d) <= 1e-9 * abs(d) + 1e-6
+
+
+@pytest.mark.parametrize("case", graphs.PSW_CASES, ids=[c["name"] for c in graphs.PSW_CASES])
+def test_psw_sched_equals_reference_engine(workdir, golden, manifest, case):
+    src, dst, n = graphs.make(case)
+    path = write_edgelist(workdir, case["name"], src, dst)
+    out = run(os.path.join(BIN, "psw_sched"), ["file", path, "niters", graphs.PSW_SCHED_MAX_ITERS, "nshards",
+                                                case["nshards"]], workdir)
+    m = manifest[case["name"]]
+    np.testing.assert_array_equal(np.fromfile(path + ".8B.vout", np.uint64), golden[f"{case['name']}__psw_sched"])
+    tasks = [[int(a), int(b)] for a, b in re.findall(r"psw_sched: iteration (\d+) tasks (\d+)", out)]
+    assert tasks == m["psw_sched_tasks"]
+    with open(path + ".deltalog") as f:
+        got = [[int(a), int(b), int(c), float(d)] for a, b, c, d in (line.split(",") for line in f.read().split())]
+    assert got == m["psw_sched_deltalog"]
 
 
 def test_psw_probe_rmat17_vs_oracle(workdir):
