@@ -7,6 +7,8 @@
 // program's update() calls is callable on the device when the program is
 // compiled with hipcc; with a host compiler they are ordinary inline members.
 #pragma once
+#include <stddef.h>
+
 #include "graphchi_types.hpp"
 
 #if defined(__HIPCC__)
@@ -37,16 +39,32 @@ GCHI_HD inline unsigned int deleted_edge_value<unsigned int>() { return 0xffffff
 template <>
 GCHI_HD inline float deleted_edge_value<float>() { return 0.0f; }
 
+// An edge as one of its endpoints sees it: the other endpoint's id and the
+// edge's value.  Every edge has two views, the source's out-edge and the
+// target's in-edge; each holds a copy of the value and a pointer to the other
+// copy (twin), and set_data() writes both.  A read is one load from the view
+// itself, next to the vertex's other views, rather than a load through a
+// pointer into a shared array.  A view built over outside storage,
+// graphchi_edge(v, p), starts from *p and writes through to it.
 template <typename EdgeDataType>
 class graphchi_edge {
   public:
     vid_t vertexid = 0;
-    EdgeDataType *data_ptr = nullptr;
+    EdgeDataType value = EdgeDataType();
+    EdgeDataType *twin = nullptr;
     GCHI_HD graphchi_edge() {}
-    GCHI_HD graphchi_edge(vid_t v, EdgeDataType *p) : vertexid(v), data_ptr(p) {}
-    GCHI_HD EdgeDataType get_data() { return *data_ptr; }
-    GCHI_HD void set_data(EdgeDataType x) { *data_ptr = x; }
+    GCHI_HD graphchi_edge(vid_t v, EdgeDataType *p) : vertexid(v), value(p ? *p : EdgeDataType()), twin(p) {}
+    GCHI_HD graphchi_edge(vid_t v, EdgeDataType x, EdgeDataType *t) : vertexid(v), value(x), twin(t) {}
+    GCHI_HD EdgeDataType get_data() { return value; }
+    GCHI_HD void set_data(EdgeDataType x) {
+        value = x;
+        if (twin) *twin = x;
+    }
     GCHI_HD vid_t vertex_id() const { return vertexid; }
+    // the view whose value `twin` points at, when both views are views
+    GCHI_HD static graphchi_edge *view_of(EdgeDataType *v) {
+        return reinterpret_cast<graphchi_edge *>(reinterpret_cast<char *>(v) - offsetof(graphchi_edge, value));
+    }
 };
 
 template <typename VertexDataType, typename EdgeDataType>
@@ -57,6 +75,7 @@ class graphchi_vertex {
     VertexDataType *dataptr = nullptr;
     bool scheduled = true;
     bool parallel_safe = true;
+    bool twins_are_views = false;  // the engine's views: every twin is the value of another view
     int inc = 0, outc = 0;  // in- and out-edge counts (the reference's public members)
     graphchi_edge<EdgeDataType> *inedges_ptr = nullptr, *outedges_ptr = nullptr;  // the view's edge arrays
     VertexDataType local = VertexDataType();  // the value when the view has no storage for it
@@ -87,10 +106,14 @@ class graphchi_vertex {
     GCHI_HD void add_outedge(vid_t dst, EdgeDataType *ptr, bool) {
         outedges_ptr[outc++] = graphchi_edge<EdgeDataType>(dst, ptr);
     }
-    // Orders the in- and out-edges by adjacent vertex id (stable).
+    // Orders the in- and out-edges by adjacent vertex id (stable).  A moved
+    // view's twin is pointed at its new place, one array at a time, so that a
+    // self-loop's two views (one in each array) stay linked.
     GCHI_HD void sort_edges_indirect() {
         sort_by_id(inedges_ptr, inc);
+        relink(inedges_ptr, inc);
         sort_by_id(outedges_ptr, outc);
+        relink(outedges_ptr, outc);
     }
     // Marks edges deleted (their data takes the deletion marker).
     GCHI_HD void remove_edge(int i) { edge(i)->set_data(deleted_edge_value<EdgeDataType>()); }
@@ -109,6 +132,11 @@ class graphchi_vertex {
     GCHI_HD static bool read_outedges() { return true; }
 
   private:
+    GCHI_HD void relink(graphchi_edge<EdgeDataType> *e, int n) {
+        if (!twins_are_views) return;
+        for (int i = 0; i < n; i++)
+            if (e[i].twin) graphchi_edge<EdgeDataType>::view_of(e[i].twin)->twin = &e[i].value;
+    }
     GCHI_HD static void sort_by_id(graphchi_edge<EdgeDataType> *e, int n) {
         for (int i = 1; i < n; i++) {
             graphchi_edge<EdgeDataType> x = e[i];

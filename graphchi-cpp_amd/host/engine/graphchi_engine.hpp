@@ -180,19 +180,22 @@ class graphchi_engine {
         pc.scheduler = &probe_sched;
         pc.nvertices = 10;
         uint32_t in_val = 7, out_val = 8, label = 0;
-        graphchi_edge<EdgeDataType> ein[1], eout[1];
-        graphchi_vertex<VertexDataType, EdgeDataType> v(5, ein, eout, 0, 0);
-        v.dataptr = reinterpret_cast<VertexDataType *>(&label);
-        v.add_inedge(3, reinterpret_cast<EdgeDataType *>(&in_val), false);
-        v.add_outedge(9, reinterpret_cast<EdgeDataType *>(&out_val), false);
         ProgramT p = program;
-        pc.iteration = 0;
-        p.update(v, pc);
+        // a fresh vertex per call: edge views copy their values when built
+        auto update_at = [&](int iteration) {
+            graphchi_edge<EdgeDataType> ein[1], eout[1];
+            graphchi_vertex<VertexDataType, EdgeDataType> v(5, ein, eout, 0, 0);
+            v.dataptr = reinterpret_cast<VertexDataType *>(&label);
+            v.add_inedge(3, reinterpret_cast<EdgeDataType *>(&in_val), false);
+            v.add_outedge(9, reinterpret_cast<EdgeDataType *>(&out_val), false);
+            pc.iteration = iteration;
+            p.update(v, pc);
+        };
+        update_at(0);
         const bool ok0 = label == 3 && out_val == 3;
         in_val = 1;
         out_val = 4;
-        pc.iteration = 1;
-        p.update(v, pc);
+        update_at(1);
         const bool ok1 = label == 1 && out_val == 1 && in_val == 1;
         if (!ok0 || !ok1)
             logstream(LOG_FATAL) << "the program registered as min-label connected components does not compute "

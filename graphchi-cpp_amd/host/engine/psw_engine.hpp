@@ -27,12 +27,14 @@
 // entry (polling with relaxed loads) takes an acquire fence before it runs the
 // vertex; so every update a vertex waited for happens before it runs.
 //
-// Memory.  Edge values live in one array in out-edge (src, dst) order; every
-// in-edge and out-edge is a graphchi_edge view (adjacent id, value pointer)
-// built once in HBM, so vertex.inedge(i) / outedge(i) return pointers into
-// them.  Vertex values are an array of VertexDataType.  Both start at zero (the
-// reference's engine starts from zero-filled vertex data and, for graphs
-// converted without edge values, zero edge data).
+// Memory.  Every edge has two graphchi_edge views built once in HBM, the
+// out-edge in (src, dst) order and the in-edge in (dst, src) order; each holds
+// the adjacent id, a copy of the value and a pointer to the other view's copy,
+// so vertex.inedge(i) / outedge(i) return pointers into them and a read is one
+// load next to the vertex's other views.  Vertex values are an array of
+// VertexDataType.  Both start at zero (the reference's engine starts from
+// zero-filled vertex data and, for graphs converted without edge values, zero
+// edge data).
 //
 // Selective scheduling (src/engine/bitset_scheduler.hpp): two bitsets in HBM,
 // this iteration's tasks and the next's; update() reaches them through a
@@ -122,13 +124,16 @@ __global__ void fill_zero(T *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T();
 }
 
+// The two views of every edge, each the other's twin; values start at zero.
+// in_eidx is a permutation: in-edge q is out-edge in_eidx[q].
 template <typename ET>
 __global__ void edge_views(const uint32_t *out_dst, const uint32_t *in_src, const uint32_t *in_eidx, uint64_t E,
-                           ET *edata, graphchi_edge<ET> *out_edges, graphchi_edge<ET> *in_edges) {
+                           graphchi_edge<ET> *out_edges, graphchi_edge<ET> *in_edges) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
-        out_edges[e] = graphchi_edge<ET>(out_dst[e], edata + e);
-        in_edges[e] = graphchi_edge<ET>(in_src[e], edata + in_eidx[e]);
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < E; q += stride) {
+        const uint32_t p = in_eidx[q];
+        in_edges[q] = graphchi_edge<ET>(in_src[q], ET(), &out_edges[p].value);
+        out_edges[p] = graphchi_edge<ET>(out_dst[p], ET(), &in_edges[q].value);
     }
 }
 
@@ -320,7 +325,7 @@ class host_scheduler : public ischeduler {
 // flight; a vertex whose count reaches zero is queued.
 template <typename VT, typename ET>
 __device__ inline void release_higher(const PassArgs<VT, ET> &a, uint32_t v, const uint32_t *nb, uint32_t cnt) {
-    constexpr int kBatch = 8;
+    constexpr int kBatch = 32;
     for (uint32_t k0 = 0; k0 < cnt; k0 += kBatch) {
         uint32_t w[kBatch], left[kBatch];
 #pragma unroll
@@ -376,6 +381,7 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
         const uint32_t ib = a.in_off[v], ie = a.in_off[v + 1], ob = a.out_off[v], oe = a.out_off[v + 1];
         graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
         vx.dataptr = a.vdata + v;
+        vx.twins_are_views = true;
         if (!a.sched || a.sched[v]) prog.P::update(vx, *ctx);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         release_higher(a, v, a.in_src + ib, ie - ib);
@@ -427,24 +433,21 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     }
     src.release();
     dst.release();
-    device::buffer<ET> edata(E1);
     device::buffer<VT> vdata(n1);
     device::buffer<graphchi_edge<ET>> out_edges(E1), in_edges(E1);
     device::buffer<uint32_t> dep(n1), queue(n1), counters(4);
     device::buffer<graphchi_context> d_ctx(1);
     device::buffer<double> d_delta(1);
-    ET *ed = edata.get();
     VT *vd = vdata.get();
     graphchi_edge<ET> *oe = out_edges.get(), *ie = in_edges.get();
     const uint32_t *od = out_dst.get(), *is = in_src.get(), *ix = in_eidx.get(), *lw = lower.get();
     const uint32_t *ioff = in_off.get(), *ooff = out_off.get();
     uint32_t *dp = dep.get(), *qu = queue.get(), *ct = counters.get();
     graphchi_context *dctx = d_ctx.get();
-    hipLaunchKernelGGL(fill_zero<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, ed, (uint64_t)E);
     hipLaunchKernelGGL(fill_zero<VT>, dim3(grid_for(n, 4096)), dim3(kThreads), 0, 0, vd, (uint64_t)n);
     if (E)
-        hipLaunchKernelGGL(edge_views<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, od, is, ix, (uint64_t)E, ed,
-                           oe, ie);
+        hipLaunchKernelGGL(edge_views<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, od, is, ix, (uint64_t)E, oe,
+                           ie);
     hip_check(hipGetLastError(), "psw setup");
 
     // The reference runs a graph of one shard (and fewer than 2 * 40M vertices)

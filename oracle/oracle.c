@@ -38,6 +38,9 @@
  *                            reference headers; golden fixtures, 1-3 shards);
  *                            zero violations of basic_smoketest.cpp:65's
  *                            invariant, and violations when run in reverse
+ *   orc_psw_sched          - the reference engine's vertex values, deltalog
+ *                            and printed task counts for the same program
+ *                            (golden fixtures, 1-4 shards, up to 26 iterations)
  */
 #include "oracle.h"
 
@@ -830,5 +833,169 @@ int orc_psw_pagerank(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t
     free(oc);
     free(ev);
     free(vv);
+    return 0;
+}
+
+/* ---- orc_psw_sched: apps/psw_sched.cpp under the bitset scheduler ---- */
+static uint32_t sched_mix(uint32_t a, uint32_t b) {
+    uint32_t h = (a ^ 0x85EBCA6Bu) * 0x9E3779B1u + b;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    return h;
+}
+
+typedef struct {
+    uint8_t *cur, *next;
+    int has_new;
+    uint32_t n;
+} orc_sched;
+
+static void sched_add(orc_sched *s, uint32_t v, int also) { /* bitset_scheduler.hpp:66-73 */
+    if (v >= s->n) return;
+    s->next[v] = 1;
+    if (also) s->cur[v] = 1;
+    s->has_new = 1;
+}
+
+static void sched_remove(orc_sched *s, uint32_t from, uint32_t to) { /* :86-88, [from, to] */
+    for (uint64_t v = from; v <= to && v < s->n; v++) s->next[v] = 0;
+}
+
+static void sched_new_iteration(orc_sched *s, int it) { /* :50-58 */
+    if (it == 0) return;
+    uint8_t *t = s->cur;
+    s->cur = s->next;
+    s->next = t;
+    memset(s->next, 0, s->n);
+}
+
+int orc_psw_sched(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int nivl,
+                  const uint32_t *ivl_end, int inmem, int niters, uint64_t *vdata_out, uint64_t *log_out,
+                  int *nlog, uint64_t *tasks_out, int *ntasks) {
+    /* incidence lists by edge index; the program folds them order-free */
+    uint64_t *ioff = calloc((size_t)n + 1, 8), *ooff = calloc((size_t)n + 1, 8);
+    uint64_t *iedge = malloc((e ? e : 1) * 8), *oedge = malloc((e ? e : 1) * 8);
+    uint32_t *ev = calloc(e ? e : 1, 4);
+    uint64_t *vv = calloc(n ? n : 1, 8);
+    uint8_t *ba = malloc(n ? n : 1), *bb = calloc(n ? n : 1, 1), *snap = malloc(n ? n : 1);
+    if (!ioff || !ooff || !iedge || !oedge || !ev || !vv || !ba || !bb || !snap) return -2;
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) {
+            ioff[dst[k] + 1]++;
+            ooff[src[k] + 1]++;
+        }
+    for (uint32_t v = 0; v < n; v++) {
+        ioff[v + 1] += ioff[v];
+        ooff[v + 1] += ooff[v];
+    }
+    uint64_t *ic = malloc(((size_t)n + 1) * 8), *oc = malloc(((size_t)n + 1) * 8);
+    memcpy(ic, ioff, ((size_t)n + 1) * 8);
+    memcpy(oc, ooff, ((size_t)n + 1) * 8);
+    for (uint64_t k = 0; k < e; k++)
+        if (src[k] != dst[k]) {
+            iedge[ic[dst[k]]++] = k;
+            oedge[oc[src[k]]++] = k;
+        }
+    free(ic);
+    free(oc);
+    memset(ba, 1, n); /* initialize_scheduler: add_task_to_all */
+    orc_sched S = {ba, bb, 1, n};
+    uint64_t updates = 0, work = 0;
+    int it_last = 0, started = 0;
+    *nlog = 0;
+    *ntasks = 0;
+    for (int it = 0; it < niters; it++) {
+        it_last = it;
+        started = 1;
+        if (it == 2 && n > 0) sched_add(&S, 0, 0); /* before_iteration */
+        /* passes: the intervals in order, or the whole graph in memory */
+        int np = inmem ? 1 : nivl;
+        if (inmem) {
+            if (it > 0 && !S.has_new) break;
+            sched_new_iteration(&S, it);
+        } else {
+            if (!S.has_new) break;
+            S.has_new = 0;
+            sched_new_iteration(&S, it);
+        }
+        uint32_t lo = 0;
+        int stop = 0;
+        for (int p = 0; p < np && !stop; p++) {
+            uint64_t hi64 = inmem ? n : (uint64_t)ivl_end[p] + 1;
+            uint32_t hi = hi64 > n ? n : (uint32_t)hi64;
+            uint64_t cnt = 0;
+            for (uint32_t v = lo; v < hi; v++) {
+                snap[v] = (inmem && it == 0) ? 1 : S.cur[v];
+                if (snap[v]) {
+                    cnt++;
+                    work += (ioff[v + 1] - ioff[v]) + (ooff[v + 1] - ooff[v]);
+                }
+            }
+            updates += cnt;
+            if (inmem) {
+                if (cnt == 0) {
+                    stop = 1;
+                    break;
+                }
+                S.has_new = 0;
+            }
+            for (uint32_t v = lo; v < hi; v++) {
+                if (!snap[v]) continue;
+                const uint32_t label = it == 0 ? v : (uint32_t)vv[v];
+                uint32_t trace = it == 0 ? 0u : (uint32_t)(vv[v] >> 32);
+                uint32_t m = label;
+                for (int side = 0; side < 2; side++) {
+                    const uint64_t *off = side ? ooff : ioff, *lst = side ? oedge : iedge;
+                    for (uint64_t q = off[v]; q < off[v + 1]; q++) {
+                        const uint32_t x = ev[lst[q]];
+                        if (x != 0 && x - 1 < m) m = x - 1;
+                    }
+                }
+                trace = sched_mix(trace, (uint32_t)it * 0x10001u + m);
+                for (int side = 0; side < 2; side++) {
+                    const uint64_t *off = side ? ooff : ioff, *lst = side ? oedge : iedge;
+                    for (uint64_t q = off[v]; q < off[v + 1]; q++) {
+                        const uint64_t k = lst[q];
+                        if (ev[k] != m + 1) {
+                            ev[k] = m + 1;
+                            const uint32_t nb = side ? dst[k] : src[k];
+                            if (nb != v) sched_add(&S, nb, ((nb ^ v) & 1u) != 0);
+                        }
+                    }
+                }
+                if (v % 7 == 3 && (it & 1)) sched_remove(&S, v, v);
+                vv[v] = ((uint64_t)trace << 32) | m;
+            }
+            lo = hi;
+        }
+        if (stop) break;
+        if (it == 1) sched_remove(&S, 5, 9); /* after_iteration */
+        uint64_t t = 0;
+        for (uint32_t v = 0; v < n; v++) t += S.cur[v];
+        tasks_out[(*ntasks)++] = t;
+        if (!inmem) {
+            log_out[3 * *nlog] = (uint64_t)it;
+            log_out[3 * *nlog + 1] = updates;
+            log_out[3 * *nlog + 2] = work;
+            (*nlog)++;
+        }
+    }
+    if (inmem && started) {
+        log_out[0] = (uint64_t)it_last;
+        log_out[1] = updates;
+        log_out[2] = work;
+        *nlog = 1;
+    }
+    memcpy(vdata_out, vv, (size_t)n * 8);
+    free(ioff);
+    free(ooff);
+    free(iedge);
+    free(oedge);
+    free(ev);
+    free(vv);
+    free(S.cur);
+    free(S.next);
+    free(snap);
     return 0;
 }

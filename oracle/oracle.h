@@ -121,6 +121,19 @@ long orc_psw_run(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *ds
 int orc_psw_pagerank(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int niters, double tol,
                      int inmem, float *vdata_out, int *iters_run, double *deltas_out);
 
+/* graphchi-cpp_amd/apps/psw_sched.cpp's program under the reference engine's
+ * bitset scheduler (src/engine/bitset_scheduler.hpp:38-104,
+ * src/engine/graphchi_engine.hpp:285-291, 465-511, 545-566, 798-830): the
+ * updates run in id order, interval by interval ([ivl_end[k-1]+1, ivl_end[k]]),
+ * each interval's scheduled flags read when it starts; with inmem (one shard)
+ * the in-memory mode's stop rules.  Writes the vertex values, the deltalog
+ * lines (iteration, updates, work; one line at the end with inmem) and the task
+ * count printed after every iteration; *nlog and *ntasks get their counts
+ * (at most niters each). */
+int orc_psw_sched(uint32_t n, uint64_t e, const uint32_t *src, const uint32_t *dst, int nivl,
+                  const uint32_t *ivl_end, int inmem, int niters, uint64_t *vdata_out, uint64_t *log_out,
+                  int *nlog, uint64_t *tasks_out, int *ntasks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,7 @@ def lib():
             "orc_pr_psw_omp": (dbl, [u32, u64, vp, vp, i32, dbl, dbl, f32, i32, i32, vp]),
             "orc_psw_run": (ctypes.c_long, [u32, u64, vp, vp, i32, i32, i32, vp]),
             "orc_psw_pagerank": (i32, [u32, u64, vp, vp, i32, dbl, i32, vp, vp, vp]),
+            "orc_psw_sched": (i32, [u32, u64, vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(_lib, k)
@@ -263,3 +264,18 @@ def psw_pagerank(n, src, dst, niters, tol, inmem=False):
     assert lib().orc_psw_pagerank(n, len(src), _p(src), _p(dst), niters, tol, 1 if inmem else 0, _p(r),
                                   ctypes.byref(it), _p(deltas)) == 0
     return r[:n], it.value, deltas[:it.value]
+
+
+def psw_sched(n, src, dst, interval_ends, inmem, niters):
+    """apps/psw_sched.cpp's program under the reference's bitset scheduler
+    (orc_psw_sched): (vertex values, deltalog lines [iteration, updates, work],
+    task counts printed after each iteration)."""
+    src, dst = u32(src), u32(dst)
+    ends = u32(interval_ends)
+    vd = np.zeros(max(n, 1), np.uint64)
+    log = np.zeros(3 * max(niters, 1), np.uint64)
+    tasks = np.zeros(max(niters, 1), np.uint64)
+    nlog, ntasks = ctypes.c_int(), ctypes.c_int()
+    assert lib().orc_psw_sched(n, len(src), _p(src), _p(dst), len(ends), _p(ends), 1 if inmem else 0, niters, _p(vd),
+                               _p(log), ctypes.byref(nlog), _p(tasks), ctypes.byref(ntasks)) == 0
+    return vd[:n], log[:3 * nlog.value].reshape(-1, 3).tolist(), tasks[:ntasks.value].tolist()

@@ -147,6 +147,27 @@ def test_psw_probe_rmat17_vs_oracle(workdir):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("nshards", [1, 3])
+def test_psw_sched_rmat17_vs_oracle(workdir, nshards):
+    """2.1M edges under the scheduler: vertex values, deltalog and task counts
+    equal the oracle's, with the intervals the sharder wrote."""
+    case = {"name": "psw_rmat17", "rmat": (17, 16, 11)}
+    src, dst, n = graphs.make(case)
+    path = write_edgelist(workdir, case["name"], src, dst)
+    out = run(os.path.join(BIN, "psw_sched"), ["file", path, "niters", graphs.PSW_SCHED_MAX_ITERS, "nshards", nshards],
+              workdir)
+    print("\n".join(l for l in out.splitlines() if "execute-updates" in l))
+    with open(f"{path}.{nshards}.intervals") as f:
+        ends = [int(x) for x in f.read().split()]
+    want, wlog, wtasks = oracle.psw_sched(n, src, dst, ends, nshards == 1, graphs.PSW_SCHED_MAX_ITERS)
+    np.testing.assert_array_equal(np.fromfile(path + ".8B.vout", np.uint64), want)
+    tasks = [int(b) for _, b in re.findall(r"psw_sched: iteration (\d+) tasks (\d+)", out)]
+    assert tasks == wtasks
+    with open(path + ".deltalog") as f:
+        got = [[int(x) for x in line.split(",")[:3]] for line in f.read().split()]
+    assert got == wlog
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(DROPIN, "basic_smoketest")), reason="drop-in not built")
 @pytest.mark.parametrize("name", ["psw_misc", "psw_path", "psw_rmat12_p3", "psw_rmat14_p2"])
 def test_reference_smoketest_source_on_device(workdir, name):

@@ -7,7 +7,10 @@
   built against the reference headers), for 1, 2 and 3 shards;
 * the smoke-test program of src/tests/basic_smoketest.cpp breaks none of its
   invariants (in-edge u->v holds u + iteration - (u > v)), and a planted
-  mistake in the oracle (vertices in reverse order) breaks them."""
+  mistake in the oracle (vertices in reverse order) breaks them;
+* the scheduled program (apps/psw_sched.cpp, orc_psw_sched): vertex values,
+  deltalog lines and printed task counts equal the reference engine's, with
+  the intervals of the oracle's sharder restatement."""
 import numpy as np
 import pytest
 
@@ -61,3 +64,30 @@ def test_psw_pagerank_oracle_equals_reference_engine(case, golden, manifest):
     got_deltas = deltas[-1:] if inmem else deltas
     for line, d in zip(want, got_deltas):
         assert d * (1 - 0.05) - 1e-6 <= line[3] <= d + 1e-6
+
+
+def sched_intervals(src, dst, n, nshards):
+    ends, _ = oracle.shard_intervals(np.sort(dst), n - 1, nshards)
+    return ends
+
+
+@pytest.mark.parametrize("case", graphs.PSW_CASES, ids=[c["name"] for c in graphs.PSW_CASES])
+def test_psw_sched_oracle_equals_reference_engine(case, golden, manifest):
+    src, dst, n = graphs.make(case)
+    m = manifest[case["name"]]
+    inmem = case["nshards"] == 1
+    vd, log, tasks = oracle.psw_sched(n, src, dst, sched_intervals(src, dst, n, case["nshards"]), inmem,
+                                      graphs.PSW_SCHED_MAX_ITERS)
+    np.testing.assert_array_equal(vd, golden[f"{case['name']}__psw_sched"])
+    assert log == [w[:3] for w in m["psw_sched_deltalog"]]
+    assert [[i, t] for i, t in enumerate(tasks)] == m["psw_sched_tasks"]
+
+
+def test_psw_sched_oracle_sees_intervals():
+    """With several shards, add_task(v, true) runs v in the same iteration only
+    when v's interval comes later: one interval changes the values."""
+    case = next(c for c in graphs.PSW_CASES if c["name"] == "psw_chain_p4")
+    src, dst, n = graphs.make(case)
+    a, la, _ = oracle.psw_sched(n, src, dst, sched_intervals(src, dst, n, 4), False, graphs.PSW_SCHED_MAX_ITERS)
+    b, lb, _ = oracle.psw_sched(n, src, dst, [n - 1], False, graphs.PSW_SCHED_MAX_ITERS)
+    assert (a != b).any() and la != lb
