@@ -31,16 +31,19 @@ __global__ void psw_pack(const uint32_t *__restrict__ a, const uint32_t *__restr
     }
 }
 
-// out-edge p: its dst, its source's out-degree, and where input edge idx[p] went
+// out-edge p: its dst, its source's out-degree, and where input edge idx[p]
+// went (out_eidx, when given: the input index of out-edge p)
 __global__ void psw_out_edges(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx, uint64_t n,
                               int vb, uint32_t vmask, uint32_t nv, uint32_t *__restrict__ out_dst,
-                              uint32_t *__restrict__ pos_of, uint32_t *__restrict__ outdeg) {
+                              uint32_t *__restrict__ pos_of, uint32_t *__restrict__ outdeg,
+                              uint32_t *__restrict__ out_eidx) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         const uint64_t k = keys[p];
         const uint32_t s = (uint32_t)(k >> vb);
         out_dst[p] = (uint32_t)k & vmask;
         pos_of[idx[p]] = (uint32_t)p;
+        if (out_eidx) out_eidx[p] = idx[p];
         if (s < nv) atomicAdd(&outdeg[s], 1u);
     }
 }
@@ -110,6 +113,14 @@ extern "C" int gchi_psw_graph_build(const uint32_t *d_src, const uint32_t *d_dst
                                     uint32_t nvertices, uint32_t *d_out_off, uint32_t *d_out_dst, uint32_t *d_in_off,
                                     uint32_t *d_in_src, uint32_t *d_in_eidx, uint32_t *d_lower, void *d_work,
                                     size_t work_bytes, void *stream) {
+    return gchi_psw_graph_build_ex(d_src, d_dst, nedges, nvertices, d_out_off, d_out_dst, d_in_off, d_in_src, d_in_eidx,
+                                   d_lower, nullptr, d_work, work_bytes, stream);
+}
+
+extern "C" int gchi_psw_graph_build_ex(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges,
+                                       uint32_t nvertices, uint32_t *d_out_off, uint32_t *d_out_dst,
+                                       uint32_t *d_in_off, uint32_t *d_in_src, uint32_t *d_in_eidx, uint32_t *d_lower,
+                                       uint32_t *d_out_eidx, void *d_work, size_t work_bytes, void *stream) {
     GCHI_CHECK(nedges <= kMaxEdges, GCHI_ERR_UNSUPPORTED, "psw_graph_build: E must be <= 2^32 - 2^25");
     GCHI_CHECK(nvertices > 0, GCHI_ERR_INVALID, "psw_graph_build: no vertices");
     GCHI_CHECK(d_out_off && d_in_off && d_lower && d_work, GCHI_ERR_INVALID, "psw_graph_build: null pointer");
@@ -136,7 +147,7 @@ extern "C" int gchi_psw_graph_build(const uint32_t *d_src, const uint32_t *d_dst
         GCHI_LAUNCH_CHECK();
         GCHI_TRY(radix_sort_u64(keys, idx, nedges, 2 * vb, sw, swb, s, err));
         hipLaunchKernelGGL(psw_out_edges, dim3(g), dim3(256), 0, s, keys, idx, nedges, vb, vmask, N, d_out_dst,
-                           pos_of, deg);
+                           pos_of, deg, d_out_eidx);
         GCHI_LAUNCH_CHECK();
     }
     GCHI_TRY(exclusive_scan_u32(deg, d_out_off, N, sw, swb, s));

@@ -25,7 +25,9 @@
 #include "common.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
@@ -541,10 +543,11 @@ int write_file(const std::string &path, const void *data, size_t n) {
 
 // Writes <base>.<P>.intervals, <base>.numvertices, <base>_degs.bin and, per
 // shard, <base>.edata_azv.<p>_<P>.adj/.adjidx plus the edge-data block
-// directory (<edata>_blockdir_<B>/<id>, uncompressed, default-valued records
-// of edge_value_bytes) and <edata>.size -- the layout of
-// src/api/chifilenames.hpp:65-153 that find_shards (:200-275) accepts.
-// Compressed (zlib) edge-data blocks are not written.
+// directory (<edata>_blockdir_<B>/<id>, default-valued records of
+// edge_value_bytes, zlib-compressed with GCHI_SHARD_ZLIB) and <edata>.size --
+// the layout of src/api/chifilenames.hpp:65-153 that find_shards (:200-275)
+// accepts.
+//
 // zlib deflate of one edge-data block at level 1 (Z_BEST_SPEED), one
 // Z_FINISH stream: write_compressed of src/util/ioutil.hpp:144-199.
 static int deflate_block(const uint8_t *in, size_t n, std::vector<uint8_t> &out) {
@@ -623,6 +626,99 @@ extern "C" int gchi_shard_write_files_ex(const gchi_shards *sh, const char *base
                     GCHI_TRY(write_file(dir + "/" + std::to_string(b), zeros.data(), len));
                 }
             }
+        }
+    }
+    return GCHI_OK;
+}
+
+// ------------------------------------------------------------------ edge values
+// A shard's edge values in its edge-data block directory (the reference's
+// memory_shard load and commit, src/shards/memoryshard.hpp:160-210, 268-300,
+// with read_compressed / write_compressed, src/util/ioutil.hpp:144-265): blocks
+// of B bytes (1 MiB rounded up to a multiple of the value size; the last one
+// shorter), each one zlib stream when the file is compressed, and
+// <edata>.size holding the total.
+namespace {
+
+size_t edata_block_bytes(uint32_t vb) {
+    size_t b = 1024 * 1024;
+    while (b % vb) b++;
+    return b;
+}
+
+int read_whole(const std::string &path, std::vector<uint8_t> &out) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return fail(GCHI_ERR_IO, "cannot open " + path);
+    out.clear();
+    uint8_t buf[1 << 16];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof(buf), f)) > 0) out.insert(out.end(), buf, buf + r);
+    const bool bad = ferror(f) != 0;
+    fclose(f);
+    if (bad) return fail(GCHI_ERR_IO, "read error on " + path);
+    return GCHI_OK;
+}
+
+}  // namespace
+
+extern "C" int gchi_edata_read(const char *edata_name, int compressed, uint32_t value_bytes, uint64_t nvalues,
+                               void *host_out) {
+    GCHI_CHECK(edata_name && value_bytes && (host_out || nvalues == 0), GCHI_ERR_INVALID, "edata_read: bad argument");
+    const std::string ed(edata_name);
+    const uint64_t total = nvalues * value_bytes;
+    std::vector<uint8_t> buf;
+    GCHI_TRY(read_whole(ed + ".size", buf));
+    buf.push_back(0);
+    const unsigned long long stated = strtoull(reinterpret_cast<const char *>(buf.data()), nullptr, 10);
+    GCHI_CHECK(stated == total, GCHI_ERR_FORMAT,
+               ed + ".size states " + std::to_string(stated) + " bytes, the shard has " + std::to_string(total));
+    const size_t B = edata_block_bytes(value_bytes);
+    const std::string dir = ed + "_blockdir_" + std::to_string(B);
+    uint8_t *out = static_cast<uint8_t *>(host_out);
+    for (uint64_t b = 0; b * B < total; b++) {
+        const size_t len = (size_t)std::min<uint64_t>(B, total - b * B);
+        const std::string path = dir + "/" + std::to_string(b);
+        GCHI_TRY(read_whole(path, buf));
+        if (!compressed) {
+            GCHI_CHECK(buf.size() == len, GCHI_ERR_FORMAT, path + ": block of the wrong size");
+            memcpy(out + b * B, buf.data(), len);
+            continue;
+        }
+        z_stream z;
+        memset(&z, 0, sizeof(z));
+        GCHI_CHECK(inflateInit(&z) == Z_OK, GCHI_ERR_IO, "edata_read: inflateInit failed");
+        z.next_in = buf.data();
+        z.avail_in = (uInt)buf.size();
+        z.next_out = out + b * B;
+        z.avail_out = (uInt)len;
+        const int r = inflate(&z, Z_FINISH);
+        const size_t have = len - z.avail_out;
+        inflateEnd(&z);
+        GCHI_CHECK(r == Z_STREAM_END && have == len, GCHI_ERR_FORMAT, path + ": not a zlib block of the shard's size");
+    }
+    return GCHI_OK;
+}
+
+extern "C" int gchi_edata_write(const char *edata_name, int compressed, uint32_t value_bytes, uint64_t nvalues,
+                                const void *host_in) {
+    GCHI_CHECK(edata_name && value_bytes && (host_in || nvalues == 0), GCHI_ERR_INVALID, "edata_write: bad argument");
+    const std::string ed(edata_name);
+    const uint64_t total = nvalues * value_bytes;
+    const size_t B = edata_block_bytes(value_bytes);
+    const std::string dir = ed + "_blockdir_" + std::to_string(B);
+    mkdir(dir.c_str(), 0777);
+    const std::string sz = std::to_string(total);
+    GCHI_TRY(write_file(ed + ".size", sz.data(), sz.size()));
+    const uint8_t *in = static_cast<const uint8_t *>(host_in);
+    std::vector<uint8_t> packed;
+    for (uint64_t b = 0; b * B < total; b++) {
+        const size_t len = (size_t)std::min<uint64_t>(B, total - b * B);
+        const std::string path = dir + "/" + std::to_string(b);
+        if (compressed) {
+            GCHI_TRY(deflate_block(in + b * B, len, packed));
+            GCHI_TRY(write_file(path, packed.data(), packed.size()));
+        } else {
+            GCHI_TRY(write_file(path, in + b * B, len));
         }
     }
     return GCHI_OK;

@@ -32,9 +32,8 @@
 // the adjacent id, a copy of the value and a pointer to the other view's copy,
 // so vertex.inedge(i) / outedge(i) return pointers into them and a read is one
 // load next to the vertex's other views.  Vertex values are an array of
-// VertexDataType.  Both start at zero (the reference's engine starts from
-// zero-filled vertex data and, for graphs converted without edge values, zero
-// edge data).
+// VertexDataType, starting at zero (the reference's engine starts from
+// zero-filled vertex data).
 //
 // Selective scheduling (src/engine/bitset_scheduler.hpp): two bitsets in HBM,
 // this iteration's tasks and the next's; update() reaches them through a
@@ -44,8 +43,11 @@
 // shards runs one pass per interval; add_task(v, true) then runs v in this
 // iteration exactly when v's interval has not started.
 //
-// Not supported on the device: repeat_updates(), dynamic edge data, and the
-// reference's writing of edge values back into the shard files.  The iteration hooks run on the host
+// Edge values start from the shard files' edge-data blocks (zlib-inflated on
+// the host) and, for a graph of several shards, are written back there after
+// the run, as the reference's engine commits them; a run on the same shards
+// then starts from them.  Not supported on the device: repeat_updates() and
+// dynamic edge data.  The iteration hooks run on the host
 // between passes; before_exec_interval / after_exec_interval are called once
 // per iteration with the whole vertex range.  log_change() on the device adds
 // to one FP64 accumulator per pass (its order of additions is not fixed);
@@ -64,6 +66,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "api/chifilenames.hpp"
 #include "api/graph_objects.hpp"
 #include "api/graphchi_context.hpp"
 #include "api/vertex_aggregator.hpp"
@@ -124,18 +127,35 @@ __global__ void fill_zero(T *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T();
 }
 
-// The two views of every edge, each the other's twin; values start at zero.
-// in_eidx is a permutation: in-edge q is out-edge in_eidx[q].
+// The two views of every edge, each the other's twin, holding the edge's
+// value from the shard files (val_in in shard order; out-edge p is input edge
+// out_eidx[p]).  in_eidx is a permutation: in-edge q is out-edge in_eidx[q].
 template <typename ET>
-__global__ void edge_views(const uint32_t *out_dst, const uint32_t *in_src, const uint32_t *in_eidx, uint64_t E,
-                           graphchi_edge<ET> *out_edges, graphchi_edge<ET> *in_edges) {
+__global__ void edge_views(const uint32_t *out_dst, const uint32_t *in_src, const uint32_t *in_eidx,
+                           const uint32_t *out_eidx, const ET *val_in, uint64_t E, graphchi_edge<ET> *out_edges,
+                           graphchi_edge<ET> *in_edges) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < E; q += stride) {
         const uint32_t p = in_eidx[q];
-        in_edges[q] = graphchi_edge<ET>(in_src[q], ET(), &out_edges[p].value);
-        out_edges[p] = graphchi_edge<ET>(out_dst[p], ET(), &in_edges[q].value);
+        const ET x = val_in[out_eidx[p]];
+        in_edges[q] = graphchi_edge<ET>(in_src[q], x, &out_edges[p].value);
+        out_edges[p] = graphchi_edge<ET>(out_dst[p], x, &in_edges[q].value);
     }
 }
+
+// The edge values back in shard order, for writing them to the shard files.
+template <typename ET>
+__global__ void edge_values_out(const graphchi_edge<ET> *out_edges, const uint32_t *out_eidx, uint64_t E, ET *val_out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < E; p += stride)
+        val_out[out_eidx[p]] = out_edges[p].value;
+}
+
+#ifndef GRAPHCHI_DISABLE_COMPRESSION
+constexpr int kEdataZlib = 1;  // the edge-data files' form, as filename_shard_edata names them
+#else
+constexpr int kEdataZlib = 0;
+#endif
 
 // dep[v] = the updates v waits for in a pass over [vlo, vhi): its lower-id
 // incident edges, less those to vertices below vlo (cross, when given: done in
@@ -415,24 +435,40 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     device::open();
     metrics_entry me = m.start_time();
     device::buffer<uint32_t> src, dst;
-    const uint64_t E = load_shard_edges_device(base, nshards, src, dst);
+    std::vector<uint64_t> shard_edges;
+    const uint64_t E = load_shard_edges_device(base, nshards, src, dst, &shard_edges);
     nedges_out = E;
     ctx.nedges = E;
     const uint64_t E1 = E ? E : 1;
     const uint32_t n1 = n ? n : 1;
     device::buffer<uint32_t> out_off((size_t)n + 1), in_off((size_t)n + 1), out_dst(E1), in_src(E1), in_eidx(E1),
-        lower(n1);
+        out_eidx(E1), lower(n1);
     {
         size_t wb = 0;
         device::check(gchi_psw_graph_workspace(E, n, &wb), "gchi_psw_graph_workspace");
         device::workspace w(wb);
-        device::check(gchi_psw_graph_build(src.get(), dst.get(), E, n, out_off.get(), out_dst.get(), in_off.get(),
-                                           in_src.get(), in_eidx.get(), lower.get(), w.get(), wb, nullptr),
+        device::check(gchi_psw_graph_build_ex(src.get(), dst.get(), E, n, out_off.get(), out_dst.get(), in_off.get(),
+                                              in_src.get(), in_eidx.get(), lower.get(), out_eidx.get(), w.get(), wb,
+                                              nullptr),
                       "gchi_psw_graph_build");
         device::check(gchi_work_status(w.get(), nullptr), "gchi_psw_graph_build");
     }
     src.release();
     dst.release();
+    // the edge values the shard files hold, in shard order
+    // (memory_shard::load, src/shards/memoryshard.hpp:268-300)
+    device::buffer<ET> val_in(E1);
+    {
+        std::vector<ET> h(E1);
+        uint64_t at = 0;
+        for (size_t p = 0; p < shard_edges.size(); p++) {
+            const std::string ed = filename_shard_edata<ET>(base, (int)p, nshards);
+            device::check(gchi_edata_read(ed.c_str(), kEdataZlib, (uint32_t)sizeof(ET), shard_edges[p], h.data() + at),
+                          ("reading the edge values of shard " + std::to_string(p)).c_str());
+            at += shard_edges[p];
+        }
+        val_in.upload(h.data(), E1);
+    }
     device::buffer<VT> vdata(n1);
     device::buffer<graphchi_edge<ET>> out_edges(E1), in_edges(E1);
     device::buffer<uint32_t> dep(n1), queue(n1), counters(4);
@@ -441,13 +477,15 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     VT *vd = vdata.get();
     graphchi_edge<ET> *oe = out_edges.get(), *ie = in_edges.get();
     const uint32_t *od = out_dst.get(), *is = in_src.get(), *ix = in_eidx.get(), *lw = lower.get();
+    const uint32_t *ox = out_eidx.get();
+    const ET *vin = val_in.get();
     const uint32_t *ioff = in_off.get(), *ooff = out_off.get();
     uint32_t *dp = dep.get(), *qu = queue.get(), *ct = counters.get();
     graphchi_context *dctx = d_ctx.get();
     hipLaunchKernelGGL(fill_zero<VT>, dim3(grid_for(n, 4096)), dim3(kThreads), 0, 0, vd, (uint64_t)n);
     if (E)
-        hipLaunchKernelGGL(edge_views<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, od, is, ix, (uint64_t)E, oe,
-                           ie);
+        hipLaunchKernelGGL(edge_views<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, od, is, ix, ox, vin,
+                           (uint64_t)E, oe, ie);
     hip_check(hipGetLastError(), "psw setup");
 
     // The reference runs a graph of one shard (and fewer than 2 * 40M vertices)
@@ -648,6 +686,23 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
     m.set("work", (double)work);
     hsched.stale();
     ctx.scheduler = nullptr;
+    // With several shards the reference commits the edge values to the shard
+    // files as it goes (memory_shard::commit, sliding_shard writes); in its
+    // in-memory mode it does not (graphchi_engine.hpp:923-933).
+    if (!inmem && E) {
+        ET *vout = val_in.get();
+        hipLaunchKernelGGL(edge_values_out<ET>, dim3(grid_for(E, 4096)), dim3(kThreads), 0, 0, oe, ox, (uint64_t)E,
+                           vout);
+        hip_check(hipGetLastError(), "edge values");
+        const std::vector<ET> h = val_in.to_host(E);
+        uint64_t at = 0;
+        for (size_t p = 0; p < shard_edges.size(); p++) {
+            const std::string ed = filename_shard_edata<ET>(base, (int)p, nshards);
+            device::check(gchi_edata_write(ed.c_str(), kEdataZlib, (uint32_t)sizeof(ET), shard_edges[p], h.data() + at),
+                          ("writing the edge values of shard " + std::to_string(p)).c_str());
+            at += shard_edges[p];
+        }
+    }
     write_vertex_values<VT>(base, vdata.to_host(n));
 }
 

@@ -100,7 +100,7 @@ inline void read_file_pinned(const std::string &path, uint64_t n, device::pinned
 // (gchi_shard_decode_many), and the edges stay there for the device build.
 // Same edge sequence as load_shard_edges.  Returns the edge count.
 inline uint64_t load_shard_edges_device(const std::string &base, int nshards, device::buffer<uint32_t> &src,
-                                        device::buffer<uint32_t> &dst) {
+                                        device::buffer<uint32_t> &dst, std::vector<uint64_t> *per_shard = nullptr) {
     device::open();
     std::vector<vid_t> ends = load_interval_ends(base, nshards);
     std::vector<device::buffer<uint8_t>> bytes(ends.size());
@@ -131,6 +131,7 @@ inline uint64_t load_shard_edges_device(const std::string &base, int nshards, de
                   ("decoding the shards of " + base).c_str());
     uint64_t total = 0;
     for (uint64_t c : count) total += c;
+    if (per_shard) *per_shard = count;
     return total;
 }
 

@@ -172,6 +172,12 @@ int gchi_psw_graph_workspace(uint64_t nedges, uint32_t nvertices, size_t *bytes)
 int gchi_psw_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
                          uint32_t *d_out_off, uint32_t *d_out_dst, uint32_t *d_in_off, uint32_t *d_in_src,
                          uint32_t *d_in_eidx, uint32_t *d_lower, void *d_work, size_t work_bytes, void *stream);
+/* As gchi_psw_graph_build; d_out_eidx[E] (may be NULL) gets the input index of
+ * each out-edge, which places edge values read in input (shard) order. */
+int gchi_psw_graph_build_ex(const uint32_t *d_src, const uint32_t *d_dst, uint64_t nedges, uint32_t nvertices,
+                            uint32_t *d_out_off, uint32_t *d_out_dst, uint32_t *d_in_off, uint32_t *d_in_src,
+                            uint32_t *d_in_eidx, uint32_t *d_lower, uint32_t *d_out_eidx, void *d_work,
+                            size_t work_bytes, void *stream);
 
 /* ------------------------------------------------------------------ PageRank */
 /* Affine functional kernel of example_apps/pagerank_functional.cpp:48-83:
@@ -506,6 +512,19 @@ int gchi_shard_encode(const gchi_shards *shards, int shard, uint8_t *h_adj, uint
 /* Writes the reference's on-disk layout (src/api/chifilenames.hpp:65-153):
  * .intervals, .numvertices, _degs.bin, .adj/.adjidx per shard and
  * uncompressed default-valued edge-data blocks + .size. */
+/* The edge values of one shard, nvalues records of value_bytes, in the shard's
+ * edge order: its edge-data file edata_name (filename_shard_edata,
+ * src/api/chifilenames.hpp:91-105), blocks <edata_name>_blockdir_<B>/<id> of
+ * B = 1 MiB rounded up to a multiple of value_bytes, each one zlib stream when
+ * `compressed`, and <edata_name>.size.  gchi_edata_read is the load of
+ * src/shards/memoryshard.hpp:268-300 with read_compressed
+ * (src/util/ioutil.hpp:201-265); a .size or block that does not match nvalues
+ * is GCHI_ERR_FORMAT.  gchi_edata_write rewrites the blocks and .size, as the
+ * reference's memory_shard::commit (memoryshard.hpp:160-210, write_compressed
+ * :144-199).  Host memory only; no device work. */
+int gchi_edata_read(const char *edata_name, int compressed, uint32_t value_bytes, uint64_t nvalues, void *host_out);
+int gchi_edata_write(const char *edata_name, int compressed, uint32_t value_bytes, uint64_t nvalues,
+                     const void *host_in);
 int gchi_shard_write_files(const gchi_shards *shards, const char *basefilename, uint32_t edge_value_bytes,
                            void *stream);
 /* As gchi_shard_write_files; flags GCHI_SHARD_ZLIB writes the edge-data blocks

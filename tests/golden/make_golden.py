@@ -16,7 +16,8 @@ gchi_rmat_generate_host), and stores what the reference wrote:
   oracle/_ref/psw_probe, psw_pagerank, psw_sched (graphchi-cpp_amd/
   apps/psw_*.cpp built against the reference's engine) -> vertex values (.4B.vout, .8B.vout),
                                                         iterations run, .deltalog,
-                                                        printed task counts
+                                                        printed task counts; psw_sched
+                                                        run twice on the same shards
 
 Inputs are not stored: tests regenerate them and check their SHA-256 here.
 Usage: python tests/golden/make_golden.py [--only NAME]
@@ -255,6 +256,11 @@ def psw_cases(rr, manifest, arrays, only):
         arrays[f"{name}__psw_sched"] = np.fromfile(spath + ".8B.vout", np.uint64)
         with open(spath + ".deltalog") as f:
             slog = [line.split(",") for line in f.read().split()]
+        # a second run on the same shards starts from the edge values the first
+        # wrote back (several shards; none in the in-memory mode of one shard)
+        rr.run("psw_sched", ["file", spath, "filetype", "edgelist", "niters", graphs.PSW_SCHED_MAX_ITERS, "nshards",
+                             case["nshards"]])
+        arrays[f"{name}__psw_sched_rerun"] = np.fromfile(spath + ".8B.vout", np.uint64)
         manifest[name] = {"kind": "psw", "case": case, "n": n, "e": int(len(src)),
                           "input_sha": graphs.input_sha(src, dst),
                           "psw_pagerank_iterations": int(re.search(r"iterations run: (\d+)", out).group(1)),

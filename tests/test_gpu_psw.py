@@ -14,7 +14,8 @@ program's own update() run on the device in the reference engine's order.
 * apps/psw_sched (selective scheduling: add_task for this and the next
   iteration, remove_tasks, host calls from the hooks, num_tasks) against the
   reference engine's vertex values, deltalog (updates and work of every
-  iteration) and printed task counts, at 1-4 shards;
+  iteration) and printed task counts, at 1-4 shards; and a second run on the
+  same shards, which reads the edge values the first wrote back;
 * the reference's src/tests/basic_smoketest.cpp, compiled unchanged with hipcc
   (apps/_dropin/basic_smoketest): its update() asserts the order's invariant
   on every in-edge read, on the device, and its vertex-data check passes."""
@@ -130,6 +131,14 @@ def test_psw_sched_equals_reference_engine(workdir, golden, manifest, case):
     with open(path + ".deltalog") as f:
         got = [[int(a), int(b), int(c), float(d)] for a, b, c, d in (line.split(",") for line in f.read().split())]
     assert got == m["psw_sched_deltalog"]
+    # a second run on the same shards: with several shards it starts from the
+    # edge values the first wrote back into the shard files (zlib blocks), as
+    # the reference's does; in the in-memory mode of one shard nothing is
+    # written back
+    run(os.path.join(BIN, "psw_sched"), ["file", path, "niters", graphs.PSW_SCHED_MAX_ITERS, "nshards",
+                                         case["nshards"]], workdir)
+    np.testing.assert_array_equal(np.fromfile(path + ".8B.vout", np.uint64),
+                                  golden[f"{case['name']}__psw_sched_rerun"])
 
 
 def test_psw_probe_rmat17_vs_oracle(workdir):

