@@ -15,17 +15,18 @@
 // here is one persistent kernel over a queue of ready vertices:
 //   * dep[v] starts at the number of incident edges whose other end has a
 //     smaller id (gchi_psw_graph_build); vertices at zero start in the queue;
-//   * each lane takes the next queue slot, waits until a vertex is written
-//     there, runs update() on it, then decrements dep of its higher-id
-//     neighbours, queueing those that reach zero.
+//   * each wavefront takes the next queue slot, waits until a vertex is
+//     written there, runs update() on it (one lane), then decrements dep of
+//     its higher-id neighbours (all lanes), queueing those that reach zero.
 // The vertex in slot s is queued by an update taken from a slot below s, so the
-// lane holding the lowest unfinished slot can always run: the pass ends when
-// every slot is taken.  Visibility across XCDs: an update ends with a release
-// fence at agent scope, its decrements are relaxed (eight in flight per lane),
-// the lane whose decrement reaches zero takes an acquire fence before it
-// writes the queue entry with a release store, and the lane that finds the
-// entry (polling with relaxed loads) takes an acquire fence before it runs the
-// vertex; so every update a vertex waited for happens before it runs.
+// wavefront holding the lowest unfinished slot can always run: the pass ends
+// when every slot is taken.  Visibility across XCDs: an update ends with a
+// release fence at agent scope, its decrements are relaxed (eight in flight
+// per lane), the lane whose decrement reaches zero takes an acquire fence
+// before it writes the queue entry with a release store, and the wavefront
+// that finds the entry (polling with relaxed loads) takes an acquire fence
+// before it runs the vertex; so every update a vertex waited for happens
+// before it runs.
 //
 // Memory.  Every edge has two graphchi_edge views built once in HBM, the
 // out-edge in (src, dst) order and the in-edge in (dst, src) order; each holds
@@ -80,6 +81,7 @@ namespace psw {
 
 constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr int kThreads = 256;
+constexpr int kWaveLanes = 64;
 
 // assert() in an update() on the device records the failure (and the largest
 // failing line) here; the engine stops with a fatal log after the pass.  A
@@ -341,25 +343,28 @@ class host_scheduler : public ischeduler {
     }
 };
 
-// One decrement of dep[w] for each neighbour id w > v in nb[0, cnt), eight in
-// flight; a vertex whose count reaches zero is queued.
+// One decrement of dep[w] for each neighbour id w > v in nb[0, cnt) that
+// belongs to this pass, spread over the wavefront's lanes with eight in flight
+// per lane; a vertex whose count reaches zero is queued.
+constexpr int kReleaseBatch = 8;
 template <typename VT, typename ET>
-__device__ inline void release_higher(const PassArgs<VT, ET> &a, uint32_t v, const uint32_t *nb, uint32_t cnt) {
-    constexpr int kBatch = 32;
-    for (uint32_t k0 = 0; k0 < cnt; k0 += kBatch) {
-        uint32_t w[kBatch], left[kBatch];
+__device__ inline void release_higher(const PassArgs<VT, ET> &a, uint32_t v, const uint32_t *nb, uint32_t cnt,
+                                      uint32_t lane) {
+    for (uint32_t k0 = 0; k0 < cnt; k0 += kWaveLanes * kReleaseBatch) {
+        uint32_t w[kReleaseBatch], left[kReleaseBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; q++) {
-            w[q] = k0 + q < cnt ? nb[k0 + q] : 0u;  // 0 is never above v
-            if (w[q] >= a.vhi) w[q] = 0u;            // a later pass's vertex
+        for (int q = 0; q < kReleaseBatch; q++) {
+            const uint32_t k = k0 + (uint32_t)q * kWaveLanes + lane;
+            w[q] = k < cnt ? nb[k] : 0u;  // 0 is never above v
+            if (w[q] >= a.vhi) w[q] = 0u;  // a later pass's vertex
         }
 #pragma unroll
-        for (int q = 0; q < kBatch; q++)
+        for (int q = 0; q < kReleaseBatch; q++)
             left[q] = w[q] > v ? __hip_atomic_fetch_add(&a.dep[w[q]], 0xffffffffu, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT)
                                : 0u;
 #pragma unroll
-        for (int q = 0; q < kBatch; q++) {
+        for (int q = 0; q < kReleaseBatch; q++) {
             if (w[q] > v && left[q] == 1u) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 const uint32_t s = atomicAdd(&a.counters[1], 1u);
@@ -369,44 +374,89 @@ __device__ inline void release_higher(const PassArgs<VT, ET> &a, uint32_t v, con
     }
 }
 
-// One pass.  Every lane loops on its own: take a slot, run the vertex queued
-// there once it is, repeat; lanes whose vertex is not queued yet wait while the
-// others of their wavefront go on.
+// Loads every 64-byte line of a vertex's views across the wavefront (16 per
+// lane in flight) before update() walks them one edge at a time on one lane.
+template <typename ET>
+__device__ inline void prefetch_views(const graphchi_edge<ET> *e, uint32_t cnt, uint32_t lane) {
+    const char *p = reinterpret_cast<const char *>(e);
+    const uint64_t lines = ((uint64_t)cnt * sizeof(graphchi_edge<ET>) + 63) / 64;
+    uint32_t acc = 0;
+    for (uint64_t l0 = 0; l0 < lines; l0 += 16 * kWaveLanes) {
+        uint32_t x[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint64_t l = l0 + (uint64_t)j * kWaveLanes + lane;
+            x[j] = l < lines ? *reinterpret_cast<const uint32_t *>(p + l * 64) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc += x[j];
+    }
+    asm volatile("" ::"v"(acc));  // the loads complete here
+}
+
+// One pass, one worker per wavefront: lane 0 takes the next slot and waits
+// until a vertex is queued there, lane 0 runs update() on it, and the whole
+// wavefront then decrements the counts of its higher-id neighbours.  A
+// wavefront runs one update at a time: with a vertex per lane, a lane whose
+// vertex was ready waited for the other lanes' updates (a hub's included),
+// since the lanes of a wavefront advance together.  The fences are
+// wavefront-wide instructions, so the release after lane 0's update orders its
+// stores before every lane's decrements.
 template <class P, typename VT, typename ET>
 __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a, graphchi_context *ctx) {
-    uint32_t slot = 0, spins = 0, seen = 0;
-    bool have = false;
+    const uint32_t lane = threadIdx.x & (kWaveLanes - 1);
+    const uint32_t nslots = a.vhi - a.vlo;
+    uint32_t spins = 0, seen = 0;
     for (;;) {
-        if (!have) {
-            slot = atomicAdd(&a.counters[0], 1u);
-            have = true;
-        }
-        if (slot >= a.vhi - a.vlo) break;
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(&a.counters[0], 1u);
+        slot = __shfl(slot, 0);
+        if (slot >= nslots) break;
         // polled with a relaxed load: an acquire on every poll invalidates the
-        // L2 each time, and thousands of waiting lanes made that 14x slower
-        const uint32_t v = __hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v == kEmpty) {
-            const uint32_t filled = __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (filled != seen) {
-                seen = filled;
-                spins = 0;
-            } else if (++spins > a.spin_limit) {
-                atomicOr(&a.counters[2], 1u);
+        // L2 each time, and thousands of waiting workers made that 14x slower
+        uint32_t v = kEmpty;
+        bool gave_up = false;
+        for (;;) {
+            uint32_t got = kEmpty, stop = 0;
+            if (lane == 0) {
+                got = __hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (got == kEmpty) {
+                    const uint32_t filled =
+                        __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (filled != seen) {
+                        seen = filled;
+                        spins = 0;
+                    } else if (++spins > a.spin_limit) {
+                        atomicOr(&a.counters[2], 1u);
+                        stop = 1;
+                    }
+                }
+            }
+            v = __shfl(got, 0);
+            if (v != kEmpty) break;
+            if (__shfl(stop, 0)) {
+                gave_up = true;
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
-            continue;
         }
+        if (gave_up) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // pairs with the queue entry's release store
         const uint32_t ib = a.in_off[v], ie = a.in_off[v + 1], ob = a.out_off[v], oe = a.out_off[v + 1];
-        graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
-        vx.dataptr = a.vdata + v;
-        vx.twins_are_views = true;
-        if (!a.sched || a.sched[v]) prog.P::update(vx, *ctx);
+        const bool run = !a.sched || a.sched[v];
+        if (run && ie - ib + oe - ob >= kWaveLanes) {
+            prefetch_views(a.in_edges + ib, ie - ib, lane);
+            prefetch_views(a.out_edges + ob, oe - ob, lane);
+        }
+        if (lane == 0 && run) {
+            graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
+            vx.dataptr = a.vdata + v;
+            vx.twins_are_views = true;
+            prog.P::update(vx, *ctx);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        release_higher(a, v, a.in_src + ib, ie - ib);
-        release_higher(a, v, a.out_dst + ob, oe - ob);
-        have = false;
+        release_higher(a, v, a.in_src + ib, ie - ib, lane);
+        release_higher(a, v, a.out_dst + ob, oe - ob, lane);
     }
 }
 
@@ -598,8 +648,8 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
         const unsigned g = grid_for(cnt_v, 4096);
         hipLaunchKernelGGL(pass_reset, dim3(g), dim3(kThreads), 0, 0, lw, crs, r.lo, r.hi, dp, qu, ct);
         hipLaunchKernelGGL(pass_seed, dim3(g), dim3(kThreads), 0, 0, dp, r.lo, r.hi, qu, ct);
-        hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(grid_for(cnt_v, (unsigned)ncu * 8)), dim3(kThreads), 0, 0,
-                           program, a, dctx);
+        hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(grid_for((uint64_t)cnt_v * kWaveLanes, (unsigned)ncu * 8)),
+                           dim3(kThreads), 0, 0, program, a, dctx);
         hip_check(hipGetLastError(), "psw pass launch");
         hip_check(hipDeviceSynchronize(), "psw pass");
         hsched.stale();
