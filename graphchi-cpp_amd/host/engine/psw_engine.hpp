@@ -32,7 +32,9 @@
 // out-edge in (src, dst) order and the in-edge in (dst, src) order; each holds
 // the adjacent id, a copy of the value and a pointer to the other view's copy,
 // so vertex.inedge(i) / outedge(i) return pointers into them and a read is one
-// load next to the vertex's other views.  Vertex values are an array of
+// load next to the vertex's other views.  A vertex with up to 4096 views (64 KB)
+// has them copied into its wavefront's LDS for its update() and copied back
+// after it.  Vertex values are an array of
 // VertexDataType, starting at zero (the reference's engine starts from
 // zero-filled vertex data).
 //
@@ -394,18 +396,70 @@ __device__ inline void prefetch_views(const graphchi_edge<ET> *e, uint32_t cnt, 
     asm volatile("" ::"v"(acc));  // the loads complete here
 }
 
+// A vertex's views copied into the wavefront's LDS area for its update(), when
+// they fit: update() then walks them at LDS latency.  Twins that point into the
+// vertex's own views (a self-loop) are pointed at the copies.
+constexpr uint32_t kStageBytes = 65536;  // per wavefront
+constexpr int kPassThreads = 128;        // two wavefronts, 128 KB of LDS per workgroup
+template <typename ET>
+__device__ inline uint32_t stage_capacity() {
+    return kStageBytes / (uint32_t)sizeof(graphchi_edge<ET>);
+}
+
+template <typename ET>
+__device__ inline bool in_views(const ET *t, graphchi_edge<ET> *e, uint32_t cnt, uint32_t *at) {
+    const char *b = reinterpret_cast<const char *>(e), *x = reinterpret_cast<const char *>(t);
+    if (x < b || x >= b + (uint64_t)cnt * sizeof(graphchi_edge<ET>)) return false;
+    *at = (uint32_t)((uint64_t)(x - b) / sizeof(graphchi_edge<ET>));
+    return true;
+}
+
+template <typename ET>
+__device__ inline void stage_in(graphchi_edge<ET> *stage, graphchi_edge<ET> *in, uint32_t nin, graphchi_edge<ET> *out,
+                                uint32_t nout, uint32_t lane) {
+    for (uint32_t i = lane; i < nin + nout; i += kWaveLanes) {
+        graphchi_edge<ET> e = i < nin ? in[i] : out[i - nin];
+        uint32_t at;
+        if (in_views(e.twin, in, nin, &at)) e.twin = &stage[at].value;
+        else if (in_views(e.twin, out, nout, &at)) e.twin = &stage[nin + at].value;
+        stage[i] = e;
+    }
+}
+
+// The copies back to their places after update() (which may have reordered
+// them, sort_edges_indirect): each view goes to the slot of its position, and
+// the view its twin belongs to is pointed at it.
+template <typename ET>
+__device__ inline void stage_out(graphchi_edge<ET> *stage, graphchi_edge<ET> *in, uint32_t nin,
+                                 graphchi_edge<ET> *out, uint32_t nout, uint32_t lane) {
+    for (uint32_t i = lane; i < nin + nout; i += kWaveLanes) {
+        graphchi_edge<ET> e = stage[i];
+        graphchi_edge<ET> *g = i < nin ? in + i : out + (i - nin);
+        uint32_t at;
+        const bool self = in_views(e.twin, stage, nin + nout, &at);
+        if (self) e.twin = at < nin ? &in[at].value : &out[at - nin].value;
+        // a view that moved: the other end's view follows it
+        if (!self && e.twin && e.twin != g->twin) graphchi_edge<ET>::view_of(e.twin)->twin = &g->value;
+        *g = e;
+    }
+}
+
 // One pass, one worker per wavefront: lane 0 takes the next slot and waits
-// until a vertex is queued there, lane 0 runs update() on it, and the whole
-// wavefront then decrements the counts of its higher-id neighbours.  A
+// until a vertex is queued there, the wavefront stages its views in LDS, lane 0
+// runs update() on it, and the whole wavefront copies the views back and
+// decrements the counts of its higher-id neighbours.  A
 // wavefront runs one update at a time: with a vertex per lane, a lane whose
 // vertex was ready waited for the other lanes' updates (a hub's included),
 // since the lanes of a wavefront advance together.  The fences are
 // wavefront-wide instructions, so the release after lane 0's update orders its
 // stores before every lane's decrements.
 template <class P, typename VT, typename ET>
-__global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a, graphchi_context *ctx) {
+__global__ __launch_bounds__(kPassThreads) void pass_run(P prog, PassArgs<VT, ET> a, graphchi_context *ctx) {
     const uint32_t lane = threadIdx.x & (kWaveLanes - 1);
     const uint32_t nslots = a.vhi - a.vlo;
+    HIP_DYNAMIC_SHARED(char, lds_raw)
+    graphchi_edge<ET> *stage =
+        reinterpret_cast<graphchi_edge<ET> *>(lds_raw + (size_t)(threadIdx.x / kWaveLanes) * kStageBytes);
     uint32_t spins = 0, seen = 0;
     for (;;) {
         uint32_t slot = 0;
@@ -444,15 +498,27 @@ __global__ __launch_bounds__(kThreads) void pass_run(P prog, PassArgs<VT, ET> a,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // pairs with the queue entry's release store
         const uint32_t ib = a.in_off[v], ie = a.in_off[v + 1], ob = a.out_off[v], oe = a.out_off[v + 1];
         const bool run = !a.sched || a.sched[v];
-        if (run && ie - ib + oe - ob >= kWaveLanes) {
-            prefetch_views(a.in_edges + ib, ie - ib, lane);
-            prefetch_views(a.out_edges + ob, oe - ob, lane);
+        const uint32_t nin = ie - ib, nout = oe - ob;
+        graphchi_edge<ET> *gin = a.in_edges + ib, *gout = a.out_edges + ob;
+        const bool staged = run && nin + nout >= kWaveLanes / 4 && nin + nout <= stage_capacity<ET>();
+        if (staged) {
+            stage_in(stage, gin, nin, gout, nout, lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the copies are in LDS before lane 0 reads them
+            __builtin_amdgcn_wave_barrier();
+        } else if (run && nin + nout >= kWaveLanes) {
+            prefetch_views(gin, nin, lane);
+            prefetch_views(gout, nout, lane);
         }
         if (lane == 0 && run) {
-            graphchi_vertex<VT, ET> vx(v, a.in_edges + ib, a.out_edges + ob, (int)(ie - ib), (int)(oe - ob));
+            graphchi_vertex<VT, ET> vx(v, staged ? stage : gin, staged ? stage + nin : gout, (int)nin, (int)nout);
             vx.dataptr = a.vdata + v;
-            vx.twins_are_views = true;
+            vx.twins_are_views = !staged;  // staged copies are re-linked by stage_out
             prog.P::update(vx, *ctx);
+        }
+        if (staged) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // lane 0's LDS writes before the others read
+            __builtin_amdgcn_wave_barrier();
+            stage_out(stage, gin, nin, gout, nout, lane);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         release_higher(a, v, a.in_src + ib, ie - ib, lane);
@@ -648,8 +714,18 @@ void run_program(ProgramT &program, const std::string &base, int nshards, uint32
         const unsigned g = grid_for(cnt_v, 4096);
         hipLaunchKernelGGL(pass_reset, dim3(g), dim3(kThreads), 0, 0, lw, crs, r.lo, r.hi, dp, qu, ct);
         hipLaunchKernelGGL(pass_seed, dim3(g), dim3(kThreads), 0, 0, dp, r.lo, r.hi, qu, ct);
-        hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(grid_for((uint64_t)cnt_v * kWaveLanes, (unsigned)ncu * 8)),
-                           dim3(kThreads), 0, 0, program, a, dctx);
+        // one 64 KB LDS area per wavefront: one workgroup per CU
+        static bool lds_set = false;
+        if (!lds_set) {
+            hip_check(hipFuncSetAttribute((const void *)pass_run<ProgramT, VT, ET>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (kPassThreads / kWaveLanes) * (int)kStageBytes),
+                      "psw pass LDS");
+            lds_set = true;
+        }
+        const unsigned pgrid = (unsigned)std::min<uint64_t>(((uint64_t)cnt_v + 1) / 2, (uint64_t)ncu);
+        hipLaunchKernelGGL((pass_run<ProgramT, VT, ET>), dim3(pgrid ? pgrid : 1), dim3(kPassThreads),
+                           (kPassThreads / kWaveLanes) * kStageBytes, 0, program, a, dctx);
         hip_check(hipGetLastError(), "psw pass launch");
         hip_check(hipDeviceSynchronize(), "psw pass");
         hsched.stale();
