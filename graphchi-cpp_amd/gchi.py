@@ -119,6 +119,10 @@ _SIGS = {
     "gchi_cc_afforest_workspace": (c_i32, [c_u32, ctypes.POINTER(c_sz)]),
     "gchi_psw_graph_workspace": (c_i32, [c_u64, c_u32, ctypes.POINTER(c_sz)]),
     "gchi_psw_graph_build": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "gchi_psw_graph_build_ex": (c_i32, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                        c_vp]),
+    "gchi_edata_read": (c_i32, [ctypes.c_char_p, c_i32, c_u32, c_u64, c_vp]),
+    "gchi_edata_write": (c_i32, [ctypes.c_char_p, c_i32, c_u32, c_u64, c_vp]),
     "gchi_cc_pull_sweep": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "gchi_cc_afforest": (c_i32, [c_vp, c_vp, c_u32, c_vp, c_vp, c_sz, c_vp]),
     "gchi_shard_decode_workspace": (c_i32, [c_u64, ctypes.POINTER(c_sz)]),
