@@ -300,6 +300,34 @@ def bench_tc(gchi, torch, scale):
             "algorithmic_GBps": round(algo / t / 1e9, 1), "overflow": int(ovf.item())}
 
 
+# ------------------------------------------------------------------ generic engine
+def bench_generic(gchi, scale=17, ef=16, seed=11, niters=3):
+    """A program's own update() on the device (apps/bin/psw_probe, built with
+    hipcc): the app's execute-updates timer over niters iterations in the
+    reference engine's update order, on an R-MAT edge list it shards itself."""
+    import re
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "graphchi-cpp_amd", "apps", "bin", "psw_probe")
+    if not os.path.exists(exe):
+        return {"skipped": "apps/bin/psw_probe not built"}
+    s, d = gchi.rmat_host(scale, ef << scale, seed)
+    keep = s != d
+    s, d = s[keep], d[keep]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "g")
+        with open(path, "w") as f:
+            f.write("".join(f"{a}\t{b}\n" for a, b in zip(s.tolist(), d.tolist())))
+        p = subprocess.run([exe, "file", path, "niters", str(niters)], cwd=tmp, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, stdin=subprocess.DEVNULL, timeout=300)
+        out = p.stdout.decode(errors="replace")
+    m = re.search(r"execute-updates:\s*([0-9.]+) s", out)
+    if p.returncode != 0 or not m:
+        return {"failed": out[-500:]}
+    return {"workload": f"apps/psw_probe on R-MAT scale {scale}, edge factor {ef}, seed {seed}, {int(keep.sum())} "
+                        f"edges, {niters} iterations", "execute_updates_s": float(m.group(1))}
+
+
 # ------------------------------------------------------------------ many graphs
 def bench_multigraph(gchi, torch, ngraphs, sweeps, seed=9):
     """Many graphs of uneven size in one plan: each graph's CSC is built on its
@@ -385,7 +413,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--hot-vertices", type=int, default=0)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
-    ap.add_argument("--only", default="", help="comma list of sections: pagerank,load,ppr,sort,cc,tc,multigraph")
+    ap.add_argument("--only", default="",
+                    help="comma list of sections: pagerank,load,ppr,sort,cc,tc,multigraph,generic")
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--ppr-plain-queries", type=int, default=64)
     ap.add_argument("--ppr-check", type=int, default=8)
@@ -446,7 +475,8 @@ def main():
                 ("sort", lambda: bench_sort_and_shards(gchi, torch, src, dst, E, maxid)),
                 ("cc", lambda: bench_cc(gchi, torch, args.cc_scale)),
                 ("tc", lambda: bench_tc(gchi, torch, args.tc_scale)),
-                ("multigraph", lambda: bench_multigraph(gchi, torch, args.graphs, 10))]
+                ("multigraph", lambda: bench_multigraph(gchi, torch, args.graphs, 10)),
+                ("generic", lambda: bench_generic(gchi))]
     for name, fn in sections:
         if not want(name):
             continue
