@@ -1,8 +1,11 @@
 // Vertex and edge objects of the program API (src/api/graph_objects.hpp:55-376).
 //
 // A vertex is a view: its id, a pointer to its value, and its in- and
-// out-edges as arrays of edge objects (each an adjacent vertex id and a pointer
-// to the edge's value) owned by whoever built the view.  The generic device
+// out-edges as arrays of edge objects (each an adjacent vertex id, a copy of
+// the edge's value and a pointer to the other endpoint's copy) owned by
+// whoever built the view.  The reference's edge object holds a pointer to the
+// value instead (data_ptr); programs reach it through get_data / set_data,
+// which behave the same.  The generic device
 // engine (engine/psw_engine.hpp) builds these views in HBM, so every member a
 // program's update() calls is callable on the device when the program is
 // compiled with hipcc; with a host compiler they are ordinary inline members.
